@@ -1,0 +1,210 @@
+"""ctypes wrapper of the C oracle (oracle/kmc_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+the cpu_baseline leg of bench.py, never by the engine.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkmc_oracle.so")
+SRC_PATH = os.path.join(HERE, "kmc_oracle.c")
+
+STATUS = {0: "ok", 1: "absorbing", 2: "species_mismatch", 3: "del_absent", 4: "bad_arg"}
+
+
+def build(force=False):
+    """Compile the oracle with gcc (no FMA contraction, like the reference's
+    x86-64 build)."""
+    if force or not os.path.exists(LIB_PATH) or \
+            os.path.getmtime(LIB_PATH) < os.path.getmtime(SRC_PATH):
+        subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared",
+                               SRC_PATH, "-o", LIB_PATH, "-lm"])
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, i64, u32, u64, dbl = (ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64,
+                                       ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double)
+        L.orc_create.restype = vp
+        L.orc_create.argtypes = [vp, i64, i32, i32, i32]
+        L.orc_free.argtypes = [vp]
+        L.orc_init.argtypes = [vp, u32, u32]
+        L.orc_init.restype = i32
+        L.orc_set_config.argtypes = [vp, vp]
+        L.orc_set_config.restype = i32
+        L.orc_set_rates.argtypes = [vp, vp]
+        L.orc_run.argtypes = [vp, i64, vp, vp, vp]
+        L.orc_run.restype = i64
+        L.orc_get_next_step.argtypes = [vp, vp, vp]
+        L.orc_get_next_step.restype = i32
+        L.orc_run_proc_nr.argtypes = [vp, i32, i32]
+        L.orc_run_proc_nr.restype = i32
+        L.orc_touchup.argtypes = [vp, i32, i32, i32, i32]
+        L.orc_touchup.restype = i32
+        for name, rt in (("orc_status", i32), ("orc_volume", i32), ("orc_nr_of_proc", i32),
+                         ("orc_kmc_time", dbl), ("orc_kmc_time_step", dbl),
+                         ("orc_kmc_step", i64), ("orc_rng_ctr", u64), ("orc_touched", u64)):
+            getattr(L, name).argtypes = [vp]
+            getattr(L, name).restype = rt
+        L.orc_reset_touched.argtypes = [vp]
+        L.orc_set_kmc_time.argtypes = [vp, dbl]
+        for name in ("orc_get_lattice", "orc_get_nr", "orc_get_procstat", "orc_get_integ",
+                     "orc_get_accum", "orc_get_occupation"):
+            getattr(L, name).argtypes = [vp, vp]
+        L.orc_get_avail.argtypes = [vp, vp, vp]
+        L.orc_get_species.argtypes = [vp, i32]
+        L.orc_get_species.restype = i32
+        L.orc_replace_species.argtypes = [vp, i32, i32, i32]
+        L.orc_replace_species.restype = i32
+        L.orc_step_uniforms.argtypes = [u32, u32, u64, vp]
+        L.orc_philox.argtypes = [vp, vp, vp]
+        L.orc_nr2lattice.argtypes = [vp, i32, vp]
+        L.orc_lattice2nr.argtypes = [vp, i32, i32, i32, i32]
+        L.orc_lattice2nr.restype = i32
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def philox4x32_10(ctr, key):
+    c = np.asarray(ctr, dtype=np.uint32)
+    k = np.asarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    lib().orc_philox(_ptr(c), _ptr(k), _ptr(out))
+    return out
+
+
+def step_uniforms(seed, stream, step):
+    u = np.zeros(3, dtype=np.float64)
+    lib().orc_step_uniforms(seed, stream, step, _ptr(u))
+    return u
+
+
+class Oracle:
+    """One replica of the reference step loop on the CPU."""
+
+    def __init__(self, blob, size):
+        blob = np.ascontiguousarray(blob, dtype=np.int32)
+        X, Y = int(size[0]), int(size[1]) if len(size) > 1 else 1
+        Z = int(size[2]) if len(size) > 2 else 1
+        self._blob = blob
+        self.h = lib().orc_create(_ptr(blob), blob.size, X, Y, Z)
+        if not self.h:
+            raise ValueError("bad model blob")
+        self.P = lib().orc_nr_of_proc(self.h)
+        self.V = lib().orc_volume(self.h)
+        self.S = int(blob[4])
+        self.spuck = int(blob[6])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_free(self.h)
+            self.h = None
+
+    def init(self, seed, stream=0):
+        return lib().orc_init(self.h, seed & 0xFFFFFFFF, stream & 0xFFFFFFFF)
+
+    def set_rates(self, rates):
+        r = np.ascontiguousarray(rates, dtype=np.float64)
+        assert r.size == self.P
+        lib().orc_set_rates(self.h, _ptr(r))
+
+    def set_config(self, config):
+        c = np.ascontiguousarray(config, dtype=np.int8)
+        assert c.size == self.V
+        return lib().orc_set_config(self.h, _ptr(c))
+
+    def run(self, n, trace=False):
+        if trace:
+            p = np.zeros(n, np.int32)
+            s = np.zeros(n, np.int32)
+            t = np.zeros(n, np.float64)
+            k = lib().orc_run(self.h, n, _ptr(p), _ptr(s), _ptr(t))
+            return k, p[:k], s[:k], t[:k]
+        return lib().orc_run(self.h, n, None, None, None)
+
+    def get_next_step(self):
+        p = np.zeros(1, np.int32)
+        s = np.zeros(1, np.int32)
+        lib().orc_get_next_step(self.h, _ptr(p), _ptr(s))
+        return int(p[0]), int(s[0])
+
+    def run_proc_nr(self, proc, site):
+        return lib().orc_run_proc_nr(self.h, proc, site)
+
+    def touchup(self, x, y, z, n):
+        return lib().orc_touchup(self.h, x, y, z, n)
+
+    @property
+    def status(self):
+        return lib().orc_status(self.h)
+
+    @property
+    def kmc_time(self):
+        return lib().orc_kmc_time(self.h)
+
+    @property
+    def kmc_step(self):
+        return lib().orc_kmc_step(self.h)
+
+    @property
+    def rng_ctr(self):
+        return lib().orc_rng_ctr(self.h)
+
+    @property
+    def touched(self):
+        return lib().orc_touched(self.h)
+
+    def reset_touched(self):
+        lib().orc_reset_touched(self.h)
+
+    def lattice(self):
+        out = np.zeros(self.V, np.int8)
+        lib().orc_get_lattice(self.h, _ptr(out))
+        return out
+
+    def nr_of_sites(self):
+        out = np.zeros(self.P, np.int32)
+        lib().orc_get_nr(self.h, _ptr(out))
+        return out
+
+    def procstat(self):
+        out = np.zeros(self.P, np.int64)
+        lib().orc_get_procstat(self.h, _ptr(out))
+        return out
+
+    def integ_rates(self):
+        out = np.zeros(self.P, np.float64)
+        lib().orc_get_integ(self.h, _ptr(out))
+        return out
+
+    def accum_rates(self):
+        out = np.zeros(self.P, np.float64)
+        lib().orc_get_accum(self.h, _ptr(out))
+        return out
+
+    def occupation(self):
+        out = np.zeros(self.S * self.spuck, np.float64)
+        lib().orc_get_occupation(self.h, _ptr(out))
+        return out.reshape(self.S, self.spuck)
+
+    def avail(self):
+        sites = np.zeros((self.P, self.V), np.int32)
+        back = np.zeros((self.P, self.V), np.int32)
+        lib().orc_get_avail(self.h, _ptr(sites), _ptr(back))
+        return sites, back

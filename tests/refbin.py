@@ -1,0 +1,60 @@
+"""Run a reference Fortran driver (oracle/_ref/<name>/ref_driver) and parse
+its output (format documented in oracle/ref_driver.f90)."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_DIR = os.path.join(REPO, "oracle", "_ref")
+
+
+def driver_path(name):
+    p = os.path.join(REF_DIR, name, "ref_driver")
+    return p if os.path.exists(p) else None
+
+
+def run_reference(name, size, seed, stream, nsteps, rates, P, config=None, mode=0):
+    exe = driver_path(name)
+    if exe is None:
+        raise FileNotFoundError("reference build %s missing (run oracle/build_ref.py)" % name)
+    X, Y = size[0], size[1]
+    Z = size[2] if len(size) > 2 else 1
+    with tempfile.TemporaryDirectory() as td:
+        inp = os.path.join(td, "in.txt")
+        out = os.path.join(td, "out.bin")
+        with open(inp, "w") as fh:
+            fh.write("%d %d %d\n%d %d\n%d %d\n%d\n" % (X, Y, Z, seed, stream, nsteps, mode,
+                                                      0 if config is None else 1))
+            fh.write(" ".join(repr(float(r)) for r in rates) + "\n")
+            if config is not None:
+                fh.write(" ".join(str(int(c)) for c in config) + "\n")
+        subprocess.check_call([exe, inp, out])
+        buf = open(out, "rb").read()
+    off = 0
+
+    def take(dtype, count):
+        nonlocal off
+        a = np.frombuffer(buf, dtype=dtype, count=count, offset=off)
+        off += a.nbytes
+        return a.copy()
+
+    res = {}
+    n = int(take("<i4", 1)[0])
+    if mode == 0:
+        res["proc"] = take("<i4", n)
+        res["site"] = take("<i4", n)
+        res["time"] = take("<f8", n)
+    res["kmc_time"] = float(take("<f8", 1)[0])
+    res["kmc_step"] = int(take("<i8", 1)[0])
+    V = (len(buf) - off - 20 * P) // (4 + 8 * P)
+    res["lattice"] = take("<i4", V)
+    res["nr"] = take("<i4", P)
+    res["procstat"] = take("<i8", P)
+    res["integ"] = take("<f8", P)
+    avail = take("<i4", P * V * 2).reshape(2, V, P)   # Fortran column-major (P, V, 2)
+    res["sites"] = avail[0].T.copy()
+    res["back"] = avail[1].T.copy()
+    assert off == len(buf)
+    return res
