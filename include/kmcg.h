@@ -1,0 +1,140 @@
+/*
+ * kmcg.h -- C ABI of the MI355X kMC engine (libkmcgpu.so).
+ *
+ * The engine runs exact serial-trajectory lattice kMC (kmos's local_smart
+ * and lat_int step loops) for many independent replicas at once on one GPU.
+ * Every entry point returns an int status (KMCG_OK or a negative error);
+ * per-replica trajectory outcomes are reported separately as replica status
+ * codes (KMCG_REPLICA_*).  No entry point aborts the process.
+ *
+ * Numbering follows the reference Fortran: processes 1..P, sites 1..V,
+ * species 0..S-1 in name order, null species -1.
+ *
+ * Reference interfaces these replace (paths relative to the kmos tree):
+ *   f2py module kmc_model.{base,lattice,proclist}  kmos/utils/__init__.py:406-521
+ *   WASM C bindings kmc_*                          kmos/utils/__init__.py:1181-1417
+ *   the generated step loop                        kmos/fortran_src (.mpy templates)
+ *
+ * Model blob (int32 words, written by kmos_amd/compiler.py to_blob()):
+ *   header[32]: 0 magic 'KMCG', 1 version, 2 backend (0 local_smart,
+ *   1 lat_int), 3 nr_of_proc, 4 nr_of_species, 5 null_species, 6 spuck,
+ *   7 dimension, 8 off_site_default, 9 off_proc_table, 10 n_routines,
+ *   11 off_routine_table, 12 off_touchup, 13 n_groups, 14 off_group_table,
+ *   15 max_offset, 16 total words.  Section layouts are documented in
+ *   kmos_amd/compiler.py (_to_blob).
+ */
+#ifndef KMCG_H
+#define KMCG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes of entry points */
+#define KMCG_OK 0
+#define KMCG_EINVAL (-1)        /* bad argument */
+#define KMCG_ENOMEM (-2)        /* device allocation failed */
+#define KMCG_EHIP (-3)          /* HIP runtime error */
+#define KMCG_EUNSUPPORTED (-4)  /* model outside the engine's limits */
+
+/* replica status codes (per trajectory) */
+#define KMCG_REPLICA_OK 0
+#define KMCG_REPLICA_ABSORBING 1        /* total rate is zero: no process possible */
+#define KMCG_REPLICA_SPECIES_MISMATCH 2 /* replace_species found another species */
+#define KMCG_REPLICA_DEL_ABSENT 3       /* lat_int del_proc of an unregistered process */
+#define KMCG_REPLICA_CORRUPT 4          /* a process list would exceed the volume */
+
+typedef struct kmcg_model kmcg_model;
+typedef struct kmcg_batch kmcg_batch;
+
+/* Text of the last error in this thread. */
+const char *kmcg_last_error(void);
+
+/* Number of visible GPUs (0 when none). */
+int kmcg_device_count(int32_t *count);
+
+/* Load a compiled model (replaces building the generated Fortran,
+ * kmos/utils/__init__.py:406-521). */
+int kmcg_model_load(const int32_t *blob, int64_t nwords, kmcg_model **out);
+void kmcg_model_free(kmcg_model *m);
+/* nr_of_proc, nr_of_species, spuck, backend, dimension */
+int kmcg_model_info(const kmcg_model *m, int32_t *info5);
+
+/* Device bytes a batch of n replicas of sizes[n][3] needs. */
+int kmcg_batch_footprint(const kmcg_model *m, int32_t n, const int32_t *sizes, int64_t *bytes);
+
+/* Allocate n replicas (system_size per replica in sizes[n][3]) on `device`;
+ * allocate_system (kmos/fortran_src/lattice.mpy:212-317, base.mpy:648-744). */
+int kmcg_batch_create(const kmcg_model *m, int32_t n, const int32_t *sizes,
+                      int32_t device, kmcg_batch **out);
+void kmcg_batch_free(kmcg_batch *b);
+
+/* proclist.init / initialize_state for replicas [r0, r0+n): default
+ * species, empty bookkeeping, touchups in k,j,i order; clocks, procstat and
+ * integ_rates reset; uniform stream keyed (seeds[i], streams[i])
+ * (proclist_generic_subroutines.mpy:166-304). */
+int kmcg_init(kmcg_batch *b, int32_t r0, int32_t n, const uint32_t *seeds,
+              const uint32_t *streams);
+
+/* base.set_rate_const for every process of replicas [r0, r0+n); rates is
+ * [n][P], finite and >= 0 (base.mpy:581-600). */
+int kmcg_set_rates(kmcg_batch *b, int32_t r0, int32_t n, const double *rates);
+int kmcg_get_rates(kmcg_batch *b, int32_t r, double *rates);
+
+/* KMC_Model._set_configuration + _adjust_database for replicas
+ * [r0, r0+n): species per site (config holds the n lattices one after the
+ * other, each V values in site order), then touchups with x outermost
+ * (kmos/run/__init__.py:1408-1457). */
+int kmcg_set_config(kmcg_batch *b, int32_t r0, int32_t n, const int8_t *config);
+
+/* proclist.do_kmc_steps(steps) for the listed replicas
+ * (proclist_generic_subroutines.mpy:1-44).  All replicas advance in one
+ * launch; a replica stops early when its status becomes nonzero.  If
+ * `budget` is non-NULL, replica replicas[i] runs min(steps, budget[i])
+ * steps and budget[i] is decreased by the steps it executed.  Returns once
+ * the work is queued on the batch's stream; kmcg_sync waits for it. */
+int kmcg_run(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps,
+             int64_t *budget);
+int kmcg_sync(kmcg_batch *b);
+/* Launch configuration for kmcg_run: lattice in LDS (1) or HBM (0, slower,
+ * no size limit); -1 = automatic.  Returns the LDS bytes one replica needs. */
+int kmcg_set_launch(kmcg_batch *b, int32_t lattice_in_lds);
+/* Milliseconds the last kmcg_run took on the device (HIP events around the
+ * kernel on the batch's stream). */
+int kmcg_last_run_ms(kmcg_batch *b, float *ms);
+
+/* proclist.get_next_kmc_step: draws one step's uniforms and returns the
+ * process and site it would pick, passing ran_time as the site uniform
+ * (proclist_generic_subroutines.mpy:85-110). */
+int kmcg_get_next_step(kmcg_batch *b, int32_t r, int32_t *proc, int32_t *site);
+/* proclist.run_proc_nr(proc, site) for replica r. */
+int kmcg_run_proc_nr(kmcg_batch *b, int32_t r, int32_t proc, int32_t site);
+/* touchup of one site (local_smart: touchup_<layer>_<site>) or cell
+ * (lat_int: touchup_cell); x, y, z 0-based, n 1-based site type. */
+int kmcg_touchup(kmcg_batch *b, int32_t r, int32_t x, int32_t y, int32_t z, int32_t n);
+/* base.replace_species without bookkeeping (lattice.replace_species) */
+int kmcg_replace_species(kmcg_batch *b, int32_t r, int32_t site, int32_t old_sp, int32_t new_sp);
+
+/* scalars: kmc_time, kmc_time_step (f64), kmc_step, rng counter (i64), status */
+int kmcg_get_scalars(kmcg_batch *b, int32_t r, double *time2, int64_t *step2, int32_t *status);
+int kmcg_get_scalars_all(kmcg_batch *b, double *kmc_time, int64_t *kmc_step, int32_t *status);
+int kmcg_set_kmc_time(kmcg_batch *b, int32_t r, double t);
+int kmcg_get_nr_of_sites(kmcg_batch *b, int32_t r, int32_t *out);
+int kmcg_get_procstat(kmcg_batch *b, int32_t r, int64_t *out);
+int kmcg_get_integ_rates(kmcg_batch *b, int32_t r, double *out);
+/* update_accum_rate then accum_rates (base.mpy:603-623) */
+int kmcg_get_accum_rates(kmcg_batch *b, int32_t r, double *out);
+int kmcg_get_lattice(kmcg_batch *b, int32_t r, int8_t *out);
+/* avail_sites(:,:,1) as sites[P][V] and avail_sites(:,:,2) as back[P][V] */
+int kmcg_get_avail(kmcg_batch *b, int32_t r, int32_t *sites, int32_t *back);
+/* proclist.get_occupation for replicas [r0, r0+n): out[n][S][spuck] */
+int kmcg_get_occupation(kmcg_batch *b, int32_t r0, int32_t n, double *out);
+/* procstat and integ_rates of replicas [r0, r0+n): [n][P] each */
+int kmcg_get_counters(kmcg_batch *b, int32_t r0, int32_t n, int64_t *procstat, double *integ);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KMCG_H */

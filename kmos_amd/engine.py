@@ -1,0 +1,215 @@
+"""Python handles over the C ABI: a loaded model and a batch of replicas.
+
+``Batch`` holds many independent replicas of one model on one GPU; every
+method maps to one entry point of include/kmcg.h.  Arrays go in and out as
+numpy arrays in the reference's numbering (processes 1..P, sites 1..V,
+species 0..S-1).
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from .compiler import CompiledModel
+
+_vp = ctypes.c_void_p
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(_vp)
+
+
+class Model:
+    """A compiled model loaded into the engine."""
+
+    def __init__(self, compiled: CompiledModel):
+        self.compiled = compiled
+        self.blob = np.ascontiguousarray(compiled.to_blob(), dtype=np.int32)
+        lib = _lib.load()
+        h = _vp()
+        _lib.check(lib.kmcg_model_load(_ptr(self.blob), self.blob.size, ctypes.byref(h)))
+        self.h = h
+        info = np.zeros(5, np.int32)
+        _lib.check(lib.kmcg_model_info(self.h, _ptr(info)))
+        self.nr_of_proc, self.nr_of_species, self.spuck, self.backend, self.dimension = \
+            (int(v) for v in info)
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib._lib is not None:
+            _lib._lib.kmcg_model_free(self.h)
+            self.h = None
+
+    def footprint(self, sizes: Sequence[Sequence[int]]) -> int:
+        sz = _sizes(sizes)
+        out = ctypes.c_int64(0)
+        _lib.check(_lib.load().kmcg_batch_footprint(self.h, len(sz), _ptr(sz), ctypes.byref(out)))
+        return out.value
+
+
+def _sizes(sizes) -> np.ndarray:
+    sz = np.zeros((len(sizes), 3), np.int32)
+    for i, s in enumerate(sizes):
+        s = list(s) + [1] * (3 - len(s))
+        sz[i] = s[:3]
+    return sz
+
+
+class Batch:
+    """Replicas of one model resident on one GPU."""
+
+    def __init__(self, model: Model, sizes: Sequence[Sequence[int]], device: int = 0):
+        self.model = model
+        self.sizes = _sizes(sizes)
+        self.n = len(self.sizes)
+        self.P = model.nr_of_proc
+        self.S = model.nr_of_species
+        self.spuck = model.spuck
+        cells = self.sizes.copy()
+        if model.dimension < 3:
+            cells[:, 2] = 1
+        if model.dimension < 2:
+            cells[:, 1] = 1
+        self.volumes = (cells.prod(axis=1) * self.spuck).astype(np.int64)
+        h = _vp()
+        _lib.check(_lib.load().kmcg_batch_create(model.h, self.n, _ptr(self.sizes), device,
+                                                 ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and _lib._lib is not None:
+            _lib._lib.kmcg_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # ---- set-up -----------------------------------------------------------
+    def init(self, seeds, streams=None, r0: int = 0):
+        seeds = np.ascontiguousarray(np.atleast_1d(seeds), dtype=np.uint32)
+        n = seeds.size
+        st = None if streams is None else np.ascontiguousarray(np.atleast_1d(streams), dtype=np.uint32)
+        _lib.check(_lib.load().kmcg_init(self.h, r0, n, _ptr(seeds), _ptr(st)))
+
+    def set_rates(self, rates, r0: int = 0):
+        r = np.ascontiguousarray(np.atleast_2d(rates), dtype=np.float64)
+        assert r.shape[1] == self.P
+        _lib.check(_lib.load().kmcg_set_rates(self.h, r0, r.shape[0], _ptr(r)))
+
+    def get_rates(self, r: int) -> np.ndarray:
+        out = np.zeros(self.P, np.float64)
+        _lib.check(_lib.load().kmcg_get_rates(self.h, r, _ptr(out)))
+        return out
+
+    def set_config(self, configs, r0: int = 0):
+        """Set the lattice of replicas r0, r0+1, ... (one array per replica,
+        or a single array for replica r0) and rebuild their bookkeeping."""
+        if isinstance(configs, np.ndarray) and configs.ndim == 1:
+            configs = [configs]
+        flat = np.concatenate([np.asarray(c).reshape(-1) for c in configs]).astype(np.int8)
+        n = len(configs)
+        assert flat.size == int(self.volumes[r0:r0 + n].sum())
+        _lib.check(_lib.load().kmcg_set_config(self.h, r0, n, _ptr(flat)))
+
+    def set_launch(self, lattice_in_lds: int = -1):
+        _lib.check(_lib.load().kmcg_set_launch(self.h, lattice_in_lds))
+
+    # ---- stepping ---------------------------------------------------------
+    def run(self, steps: int, replicas=None, budget=None, sync: bool = True):
+        """do_kmc_steps(steps) on the given replicas (all by default)."""
+        ids = np.arange(self.n, dtype=np.int32) if replicas is None else \
+            np.ascontiguousarray(replicas, dtype=np.int32)
+        bud = None if budget is None else np.ascontiguousarray(budget, dtype=np.int64)
+        self._ids = ids
+        _lib.check(_lib.load().kmcg_run(self.h, _ptr(ids), ids.size, int(steps), _ptr(bud)))
+        if sync:
+            self.sync()
+        return bud
+
+    def sync(self):
+        _lib.check(_lib.load().kmcg_sync(self.h))
+
+    def last_run_ms(self) -> float:
+        v = ctypes.c_float(0)
+        _lib.check(_lib.load().kmcg_last_run_ms(self.h, ctypes.byref(v)))
+        return float(v.value)
+
+    def get_next_step(self, r: int):
+        p, s = ctypes.c_int32(0), ctypes.c_int32(0)
+        _lib.check(_lib.load().kmcg_get_next_step(self.h, r, ctypes.byref(p), ctypes.byref(s)))
+        return p.value, s.value
+
+    def run_proc_nr(self, r: int, proc: int, site: int):
+        _lib.check(_lib.load().kmcg_run_proc_nr(self.h, r, proc, site))
+
+    def touchup(self, r: int, x: int, y: int, z: int, n: int):
+        _lib.check(_lib.load().kmcg_touchup(self.h, r, x, y, z, n))
+
+    def replace_species(self, r: int, site: int, old: int, new: int):
+        _lib.check(_lib.load().kmcg_replace_species(self.h, r, site, old, new))
+
+    # ---- state ------------------------------------------------------------
+    def scalars(self, r: int):
+        """(kmc_time, kmc_time_step, kmc_step, rng_counter, status)"""
+        t = np.zeros(2, np.float64)
+        s = np.zeros(2, np.int64)
+        st = ctypes.c_int32(0)
+        _lib.check(_lib.load().kmcg_get_scalars(self.h, r, _ptr(t), _ptr(s), ctypes.byref(st)))
+        return float(t[0]), float(t[1]), int(s[0]), int(s[1]), st.value
+
+    def scalars_all(self):
+        t = np.zeros(self.n, np.float64)
+        k = np.zeros(self.n, np.int64)
+        st = np.zeros(self.n, np.int32)
+        _lib.check(_lib.load().kmcg_get_scalars_all(self.h, _ptr(t), _ptr(k), _ptr(st)))
+        return t, k, st
+
+    def set_kmc_time(self, r: int, t: float):
+        _lib.check(_lib.load().kmcg_set_kmc_time(self.h, r, float(t)))
+
+    def nr_of_sites(self, r: int) -> np.ndarray:
+        out = np.zeros(self.P, np.int32)
+        _lib.check(_lib.load().kmcg_get_nr_of_sites(self.h, r, _ptr(out)))
+        return out
+
+    def procstat(self, r: int) -> np.ndarray:
+        out = np.zeros(self.P, np.int64)
+        _lib.check(_lib.load().kmcg_get_procstat(self.h, r, _ptr(out)))
+        return out
+
+    def integ_rates(self, r: int) -> np.ndarray:
+        out = np.zeros(self.P, np.float64)
+        _lib.check(_lib.load().kmcg_get_integ_rates(self.h, r, _ptr(out)))
+        return out
+
+    def accum_rates(self, r: int) -> np.ndarray:
+        out = np.zeros(self.P, np.float64)
+        _lib.check(_lib.load().kmcg_get_accum_rates(self.h, r, _ptr(out)))
+        return out
+
+    def lattice(self, r: int) -> np.ndarray:
+        out = np.zeros(int(self.volumes[r]), np.int8)
+        _lib.check(_lib.load().kmcg_get_lattice(self.h, r, _ptr(out)))
+        return out
+
+    def avail(self, r: int):
+        V = int(self.volumes[r])
+        sites = np.zeros((self.P, V), np.int32)
+        back = np.zeros((self.P, V), np.int32)
+        _lib.check(_lib.load().kmcg_get_avail(self.h, r, _ptr(sites), _ptr(back)))
+        return sites, back
+
+    def occupation(self, r0: int = 0, n: Optional[int] = None) -> np.ndarray:
+        n = self.n - r0 if n is None else n
+        out = np.zeros((n, self.S, self.spuck), np.float64)
+        _lib.check(_lib.load().kmcg_get_occupation(self.h, r0, n, _ptr(out)))
+        return out
+
+    def counters(self, r0: int = 0, n: Optional[int] = None):
+        n = self.n - r0 if n is None else n
+        ps = np.zeros((n, self.P), np.int64)
+        ig = np.zeros((n, self.P), np.float64)
+        _lib.check(_lib.load().kmcg_get_counters(self.h, r0, n, _ptr(ps), _ptr(ig)))
+        return ps, ig

@@ -1,0 +1,103 @@
+"""Engine vs oracle: bit-exact process, site and bookkeeping on every model.
+
+Each case runs a few replicas of different sizes, seeds and rate sets in
+one batch on the GPU and the same replicas one by one on the CPU oracle,
+then compares the complete state (lattice, nr_of_sites, both avail_sites
+tables, procstat) exactly and kmc_time / integ_rates within 1e-12 / 1e-9
+relative (the GPU's log may differ from glibc's by an ulp).
+"""
+import numpy as np
+import pytest
+
+from kmos_amd import workloads as W
+from kmos_amd.compiler import compile_model
+from paritylib import compare_state, oracle_replica
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_config(pt, V, rng):
+    ids = sorted(pt.species_ids().values())
+    return rng.choice(ids, size=V).astype(np.int8)
+
+
+CASES = [
+    # (name, project factory, backend, sizes, steps, random config)
+    ("ruo2_local_smart", W.ruo2_project, "local_smart", [(6, 5), (9, 9), (20, 20)], 3000, True),
+    ("ruo2_lat_int", W.ruo2_project, "lat_int", [(6, 5), (9, 9), (20, 20)], 3000, True),
+    ("zgb_local_smart", W.zgb_project, "local_smart", [(8, 8), (16, 16), (5, 7)], 3000, False),
+    ("quickstart", W.quickstart_project, "local_smart", [(20, 20), (3, 4)], 3000, False),
+    ("pairwise_lat_int", W.pairwise_project, "lat_int", [(8, 8), (12, 10)], 3000, True),
+    ("pairwise_local_smart", W.pairwise_project, "local_smart", [(8, 8), (12, 10)], 3000, True),
+    ("zgb_lat_int", W.zgb_project, "lat_int", [(8, 8), (10, 6)], 3000, False),
+]
+
+
+@pytest.mark.parametrize("name,factory,backend,sizes,steps,use_config", CASES,
+                         ids=[c[0] for c in CASES])
+def test_engine_matches_oracle(engine, oracle, name, factory, backend, sizes, steps, use_config):
+    print("case", name, flush=True)
+    pt = factory()
+    cm = compile_model(pt, backend)
+    rng = np.random.default_rng(len(name))
+    P = cm.nr_of_proc
+    rates = 10.0 ** rng.uniform(-2, 6, (len(sizes), P))
+    m = engine.Model(cm)
+    b = engine.Batch(m, [list(s) + [1] for s in sizes])
+    seeds = np.arange(len(sizes), dtype=np.uint32) + 11
+    streams = np.arange(len(sizes), dtype=np.uint32) * 7
+    b.init(seeds, streams)
+    b.set_rates(rates)
+    configs = None
+    if use_config:
+        configs = [_random_config(pt, int(v), rng) for v in b.volumes]
+        b.set_config(configs)
+    orcs = [oracle_replica(oracle, cm, s, int(seeds[i]), int(streams[i]), rates[i],
+                           None if configs is None else configs[i]) for i, s in enumerate(sizes)]
+    for r, o in enumerate(orcs):
+        compare_state(b, r, o, where="%s r%d after init" % (name, r))
+    done = 0
+    for chunk in (1, 17, steps // 2, steps - steps // 2 - 18):
+        b.run(chunk)
+        for o in orcs:
+            o.run(chunk)
+        done += chunk
+        for r, o in enumerate(orcs):
+            compare_state(b, r, o, where="%s r%d after %d steps" % (name, r, done))
+
+
+def test_single_replica_subset_runs(engine, oracle):
+    """Running a subset of replicas leaves the others untouched."""
+    pt = W.ruo2_project()
+    cm = compile_model(pt, "local_smart")
+    m = engine.Model(cm)
+    b = engine.Batch(m, [(10, 10, 1)] * 3)
+    b.init(np.array([1, 2, 3], np.uint32))
+    rates = np.full((3, cm.nr_of_proc), 10.0)
+    b.set_rates(rates)
+    b.run(500, replicas=[1])
+    assert b.scalars(0)[2] == 0 and b.scalars(2)[2] == 0
+    o = oracle_replica(oracle, cm, (10, 10), 2, 0, rates[1])
+    o.run(500)
+    compare_state(b, 1, o, where="subset")
+
+
+def test_budget_and_absorbing(engine, oracle):
+    """Per-replica budgets, and a replica with all rates zero stops with
+    status 'absorbing' without disturbing the others."""
+    pt = W.zgb_project()
+    cm = compile_model(pt, "local_smart")
+    m = engine.Model(cm)
+    b = engine.Batch(m, [(8, 8, 1)] * 3)
+    b.init(np.array([5, 6, 7], np.uint32))
+    rates = np.tile(np.array([0.5, 1e10, 1e10, 1e10, 1e10, 0.25, 0.25]), (3, 1))
+    rates[1] = 0.0
+    b.set_rates(rates)
+    budget = b.run(1000, budget=np.array([100, 1000, 400], np.int64))
+    assert list(budget) == [0, 1000, 0]
+    t, dt, ks, ctr, st = b.scalars(1)
+    assert st == 1 and ks == 0
+    for r, n in ((0, 100), (2, 400)):
+        o = oracle_replica(oracle, cm, (8, 8), 5 + r, 0, rates[r])
+        o.run(n)
+        compare_state(b, r, o, where="budget r%d" % r)
