@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Benchmark: aggregate kMC steps/s of the batched engine on one MI355X.
+
+Workload (default): the batched ZGB model, 4096 replicas of 128x128, each
+running 1e5 kMC steps from an empty lattice with y ~ U[0.35, 0.60].  One
+bench "step" is one launch that advances every replica by
+steps_per_replica / K kMC steps, so the K timed launches run the whole
+workload (4.1e8 kMC steps); W warm-up launches run on a separate batch.
+
+Printed (one JSON line): whole-job kMC steps/s (value), the plain path (one
+replica per launch) on the same model, their ratio, the HBM roofline of
+the step kernel from the algorithmic bytes per kMC step (int32 avail_sites
+entries read or written, counted by the instrumented oracle), and a CPU
+baseline (the C oracle on the host cores, one process per core, on a
+bounded sample of the same workload).
+"""
+import argparse
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+from kmos_amd import workloads as W  # noqa: E402
+from kmos_amd.compiler import compile_model  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "aggregate kMC steps/s, batched vs plain GPU; p50/p99 request latency"
+
+
+def _oracle_worker(args):
+    blob, size, seed, stream, rates, steps, count_bytes = args
+    from oracle import pyoracle
+    o = pyoracle.Oracle(blob, size)
+    o.init(seed, stream)
+    o.set_rates(rates)
+    o.reset_touched()
+    t0 = time.perf_counter()
+    n = o.run(steps)
+    dt = time.perf_counter() - t0
+    return n, dt, o.touched
+
+
+def algorithmic_bytes_per_step(wl, cm, sample=8, steps=20000):
+    """Mean int32 avail_sites entries touched per kMC step x 4 B (oracle
+    instrumentation), over `sample` replicas of the workload."""
+    from oracle import pyoracle
+    pyoracle.build()
+    blob = cm.to_blob()
+    tot_steps, tot_touch = 0, 0
+    for r in range(min(sample, wl.n)):
+        n, _, touched = _oracle_worker((blob, tuple(wl.sizes[r][:2]), int(wl.seeds[r]),
+                                        int(wl.streams[r]), wl.rates[r], steps, True))
+        tot_steps += n
+        tot_touch += touched
+    return 4.0 * tot_touch / max(tot_steps, 1)
+
+
+def cpu_baseline(wl, cm, seconds=15.0):
+    """The C oracle on all host cores (one process per core), each core
+    running whole replicas of the workload for its full step budget."""
+    from oracle import pyoracle
+    pyoracle.build()
+    cores = len(os.sched_getaffinity(0))
+    cores = min(cores, 16)
+    blob = cm.to_blob()
+    # size the sample so that it is about `seconds` of CPU work: time one
+    # replica first
+    n1, dt1, _ = _oracle_worker((blob, tuple(wl.sizes[0][:2]), int(wl.seeds[0]),
+                                 int(wl.streams[0]), wl.rates[0], min(wl.steps, 20000), False))
+    rate1 = n1 / max(dt1, 1e-9)
+    per_replica = wl.steps / rate1
+    nrep = int(max(cores, min(wl.n, round(seconds * cores / per_replica))))
+    nrep = max(cores, (nrep // cores) * cores)
+    jobs = [(blob, tuple(wl.sizes[r % wl.n][:2]), int(wl.seeds[r % wl.n]), int(wl.streams[r % wl.n]),
+             wl.rates[r % wl.n], wl.steps, False) for r in range(nrep)]
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        res = pool.map(_oracle_worker, jobs, chunksize=1)
+    wall = time.perf_counter() - t0
+    steps = sum(r[0] for r in res)
+    return {"value": steps / wall, "unit": "kMC steps/s", "cores": cores, "kind": "port",
+            "sample": "%d replicas x %d steps of %s (C oracle, one process per core, %.1f s wall)"
+                      % (nrep, wl.steps, wl.name, wall)}
+
+
+def make_batch(engine, model, wl, idx=None):
+    idx = np.arange(wl.n) if idx is None else np.asarray(idx)
+    b = engine.Batch(model, [list(s) for s in wl.sizes[idx]])
+    b.init(wl.seeds[idx], wl.streams[idx])
+    b.set_rates(wl.rates[idx])
+    if wl.configs is not None:
+        b.set_config([wl.configs[i] for i in idx])
+    return b
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="zgb", choices=sorted(W.WORKLOADS))
+    ap.add_argument("--replicas", type=int, default=None)
+    ap.add_argument("--kmc-steps", type=int, default=None, help="kMC steps per replica")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-steps", type=int, default=20000)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: replicas are independent and the engine runs "
+              "them all on a single device; run with --gpus 1", file=sys.stderr)
+        return 2
+
+    kw = {}
+    if a.replicas is not None:
+        kw["n"] = a.replicas
+    if a.kmc_steps is not None:
+        kw["steps"] = a.kmc_steps
+    wl = W.WORKLOADS[a.workload](**kw)
+    cm = compile_model(wl.project, wl.backend)
+
+    from kmos_amd import _lib, engine
+    _lib.load()
+    if _lib.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        return 1
+    model = engine.Model(cm)
+    K = max(1, a.steps)
+    chunk = max(1, wl.steps // K)
+
+    # warm-up on a separate small batch
+    if a.warmup > 0:
+        wb = make_batch(engine, model, wl, np.arange(min(wl.n, 256)))
+        for _ in range(a.warmup):
+            wb.run(min(chunk, 2000))
+        wb.close()
+
+    b = make_batch(engine, model, wl)
+    kernel_ms = []
+    done_before = b.scalars_all()[1].sum()
+    b.sync()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        b.run(chunk, sync=False)
+        kernel_ms.append(b.last_run_ms())   # waits for the launch, reads its HIP events
+    b.sync()
+    wall = time.perf_counter() - t0
+    t_after, k_after, st_after = b.scalars_all()
+    executed = int(k_after.sum() - done_before)
+    value = executed / wall
+    avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
+    steps_per_launch = executed / K
+
+    # spot check: replica 0 against the oracle, bit-exact
+    spot = None
+    try:
+        from oracle import pyoracle
+        pyoracle.build()
+        from tests.paritylib import compare_state, oracle_replica
+        o = oracle_replica(pyoracle, cm, tuple(wl.sizes[0][:2]), int(wl.seeds[0]), int(wl.streams[0]),
+                           wl.rates[0], None if wl.configs is None else wl.configs[0])
+        o.run(int(k_after[0]))
+        compare_state(b, 0, o, where="bench spot check")
+        spot = True
+    except AssertionError as e:
+        print("spot check FAILED: %s" % e, file=sys.stderr)
+        spot = False
+
+    # plain path: one replica per launch, same kernel
+    pb = make_batch(engine, model, wl, [0])
+    pb.run(min(a.plain_steps, 2000))
+    tp0 = time.perf_counter()
+    pb.run(a.plain_steps)
+    plain_wall = time.perf_counter() - tp0
+    plain = a.plain_steps / plain_wall
+    pb.close()
+
+    bytes_per_step = algorithmic_bytes_per_step(wl, cm)
+    achieved = steps_per_launch * bytes_per_step / avg_kernel_s / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "kMC steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": a.warmup,
+        "ms_per_step": wall * 1e3 / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (workload parameters from a seeded Philox generator)",
+        "config": dict(wl.describe(), kmc_steps_per_launch=chunk),
+        "plain_gpu_steps_per_s": plain,
+        "batched_over_plain": value / plain,
+        "replica_status_counts": {str(k): int(v) for k, v in zip(*np.unique(st_after, return_counts=True))},
+        "parity_spot_check": spot,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "algorithmic_bytes_per_kmc_step": bytes_per_step,
+                     "kernel": "kmc_steps_kernel", "avg_launch_ms": avg_kernel_s * 1e3},
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(wl, cm, a.cpu_seconds)
+        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -20,7 +20,7 @@
  *   1 lat_int), 3 nr_of_proc, 4 nr_of_species, 5 null_species, 6 spuck,
  *   7 dimension, 8 off_site_default, 9 off_proc_table, 10 n_routines,
  *   11 off_routine_table, 12 off_touchup, 13 n_groups, 14 off_group_table,
- *   15 max_offset, 16 total words.  Section layouts are documented in
+ *   15 min_size (cells per periodic dimension), 16 total words.  Section layouts are documented in
  *   kmos_amd/compiler.py (_to_blob).
  */
 #ifndef KMCG_H
@@ -99,8 +99,9 @@ int kmcg_run(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps,
              int64_t *budget);
 int kmcg_sync(kmcg_batch *b);
 /* Launch configuration for kmcg_run: lattice in LDS (1) or HBM (0, slower,
- * no size limit); -1 = automatic.  Returns the LDS bytes one replica needs. */
-int kmcg_set_launch(kmcg_batch *b, int32_t lattice_in_lds);
+ * no size limit), -1 = automatic; replica-waves per workgroup sharing one
+ * copy of the op program (1..4, 0 = as many as LDS allows). */
+int kmcg_set_launch(kmcg_batch *b, int32_t lattice_in_lds, int32_t waves_per_group);
 /* Milliseconds the last kmcg_run took on the device (HIP events around the
  * kernel on the batch's stream). */
 int kmcg_last_run_ms(kmcg_batch *b, float *ms);

@@ -103,36 +103,37 @@ class CompiledModel:
     def backend_name(self) -> str:
         return {v: k for k, v in BACKENDS.items()}[self.backend]
 
-    def max_offset(self) -> int:
-        """Largest |cell offset| any op can reach; lattices must be at least
-        this large so that one periodic wrap is enough."""
-        m = 0
-        def upd(site):
-            nonlocal m
-            m = max(m, abs(site[0]), abs(site[1]), abs(site[2]))
+    def _offsets(self):
+        """Every cell offset the engine adds to a (wrapped) cell in one go."""
+        out = []
         if self.backend == BACKEND_LOCAL_SMART:
             for acts in self.proc_actions:
-                for _, s in acts:
-                    upd(s)
+                out += [s[:3] for _, s in acts]
             for r in self.routines:
-                for _, s in r.disables:
-                    upd(s)
+                out += [s[:3] for _, s in r.disables]
                 for e in r.enables:
-                    upd(e.site)
-                    for s, _ in e.conds:
-                        upd(s)
-            # enables/disables are relative to an action site, itself
-            # relative to the executing site
-            ma = max([max(abs(s[0]), abs(s[1]), abs(s[2])) for acts in self.proc_actions for _, s in acts] or [0])
-            return m + ma
-        for g in self.groups:
-            for _, off in g.modified:
-                upd(off + (0,))
-            for s, _, _ in g.updates:
-                upd(s)
-            for node in g.nli_nodes:
-                upd(tuple(node[0:4]))
-        return 2 * m
+                    out.append(e.site[:3])
+                    out += [c[:3] for c, _ in e.conds]
+        else:
+            for g in self.groups:
+                out += [tuple(off) for _, off in g.modified]
+                out += [s[:3] for s, _, _ in g.updates]
+                out += [tuple(nd[0:3]) for nd in g.nli_nodes if nd[4] != -1]
+        return out
+
+    def min_size(self) -> int:
+        """Smallest lattice extent (cells per periodic dimension) the engine
+        accepts: each offset is applied to an already wrapped cell, so one
+        periodic wrap must suffice."""
+        return max([max(abs(o[0]), abs(o[1]), abs(o[2])) for o in self._offsets()] or [0]) or 1
+
+    def consistent_size(self) -> int:
+        """Smallest extent at which no step touches one physical site through
+        two different offsets.  Below it the reference's own bookkeeping can
+        register a process twice or delete an absent one (the engine then
+        reproduces or flags that); the models' steady state is only
+        meaningful above it."""
+        return 2 * max([max(abs(o[0]), abs(o[1]), abs(o[2])) for o in self._offsets()] or [0]) + 1
 
     # -------------------------------------------------------------------
     def to_blob(self) -> np.ndarray:
@@ -574,6 +575,6 @@ def _to_blob(cm: CompiledModel) -> np.ndarray:
             words[base + 6] = len(g.procs)
             words[base + 7] = here()
             words.extend(g.procs)
-    hdr[15] = cm.max_offset()
+    hdr[15] = cm.min_size()
     hdr[16] = len(words)
     return np.asarray(words, dtype=np.int32)

@@ -1,43 +1,46 @@
-// kmcg_device.h -- device-side data layout and step machinery of the engine.
+// kmcg_device.h -- device-side data layout of the kMC engine.
 //
-// One wavefront (64 lanes) runs one replica.  The lattice of the replica
-// sits bit-packed in LDS; the available-site lists (kmos's avail_sites) sit
-// in HBM as two int32 arrays per process, sites[p][0..V] and back[p][0..V]
-// (1-based, slot 0 unused).  Lane (p % 64) owns process p: it holds
-// nr_of_sites, the rate, integ_rate and procstat of p in registers and is
-// the only lane that ever touches sites[p] / back[p], so the order of the
-// bookkeeping operations on one process is plain program order.
+// One wavefront (64 lanes) runs one replica; a workgroup holds up to four
+// replica-waves that share one copy of the model's op program in LDS.  The
+// lattice of each replica sits bit-packed in LDS; the available-site lists
+// (kmos's avail_sites) sit in HBM as two int32 arrays per process,
+// sites[p][0..V] and back[p][0..V] (1-based, slot 0 unused).  Lane (p % 64)
+// owns process p: it keeps nr_of_sites, the rate, integ_rate and procstat of
+// p in registers and is the only lane that ever reads or writes sites[p] /
+// back[p], so the order of the bookkeeping operations on one process is the
+// lane's program order.
 //
 // Every generated routine of the reference (put/take/touchup for
 // local_smart, run_proc_<group>/touchup_cell for lat_int) is a "segment":
 // lattice writes followed by an ordered op list.  Ops are evaluated
-// lane-parallel (conditions, nli case trees), routed through an LDS mailbox
-// to the lane owning their process, and applied in rounds precomputed on the
-// host so that ops in one round touch pairwise different processes and ops
-// on one process keep the reference order.
+// lane-parallel (conditions, nli case trees) and posted through an LDS
+// mailbox to the lane owning their process, in rounds precomputed on the
+// host so that one process's ops keep the reference order.  The owner lane
+// replays its deletions of one segment exactly from a single batch of HBM
+// reads (see apply_segment in kmcg_engine.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace kmcg {
 
-// ---- device program layout (int32 words; built by the host from the blob)
-enum : int {
-    H_P = 0, H_S, H_SPUCK, H_DIM, H_BACKEND, H_NULL, H_MAXOFF, H_MAXR,
-    H_SEG, H_WRITES, H_OPS, H_CONDS, H_PROG, H_ITEMS, H_TOUCHUP, H_NLI,
-    H_NGROUPS, H_SITEDEF, H_WORDS, H_NSEG,
-    HDR_WORDS = 32
-};
-// segment: [n_writes, write_off, n_ops, op_off, n_rounds, 0]
-constexpr int SEG_W = 6;
-// write: [dx, dy, dz, n, old_mask, new_species]
-constexpr int WR_W = 6;
-// op: [kind, round, arg, dx, dy, dz, n, n_cond, cond_off, 0]
-constexpr int OP_W = 10;
-// cond: [dx, dy, dz, n, species]
-constexpr int CD_W = 5;
-// process program: [n_items, item_off]; item: [segment, dx, dy, dz]
-constexpr int IT_W = 4;
+// ---- packed op program (int32 words, copied to LDS) -----------------------
+// segment (6 words): n_writes, write_off, n_ops | n_rounds << 16 | as_add << 24,
+//                    op_off, cond_base, 0
+//                    (as_add: the ops are another segment's nli deletions,
+//                    applied here as additions)
+// write   (2 words): pack3(dx,dy,dz) | n << 24,  old_mask | new_species << 16
+// op      (2 words): kind | round << 3 | n << 11 | n_cond << 17 | arg << 23,
+//                    dx & 127 | (dy & 127) << 7 | (dz & 127) << 14 | cond_rel << 21
+// cond    (1 word) : (dx+32) | (dy+32) << 6 | (dz+32) << 12 | n << 18 | species << 24
+// process (2 words): n_items, item_off;  item (2 words): segment, pack3(dx,dy,dz)
+// touchup (1 word per site type, 1 for lat_int): segment
+// nli     (1 word per group): node offset; node: pack3 | n << 24, n_cases,
+//                    (species_mask, kind, target) x n_cases  (kind 0 child, 1 leaf)
+//         leaf node: 0, -1, value
+constexpr int SEG_W = 6, WR_W = 2, OP_W = 2, IT_W = 2;
+
+__device__ __forceinline__ int sext7(int v) { return (v ^ 0x40) - 0x40; }
 
 enum : int { OP_DEL_IF = 1, OP_ADD_IF = 2, OP_DEL_NLI = 3, OP_ADD_NLI = 4 };
 // mailbox codes (low 2 bits of an entry; site+1 in the upper bits)
@@ -45,15 +48,26 @@ enum : int { MB_DEL_IF = 1, MB_DEL_STRICT = 2, MB_ADD = 3 };
 
 enum : int { ST_OK = 0, ST_ABSORBING = 1, ST_MISMATCH = 2, ST_DEL_ABSENT = 3, ST_CORRUPT = 4 };
 
+constexpr int KMAX = 4;       // deletions of one process replayed per HBM round trip
+constexpr int MAX_WPG = 4;    // replica-waves per workgroup
+
+__host__ __device__ inline int pack3(int dx, int dy, int dz)
+{
+    return (dx & 0xFF) | ((dy & 0xFF) << 8) | ((dz & 0xFF) << 16);
+}
+__device__ __forceinline__ int unpack_x(int w) { return (int)(signed char)(w & 0xFF); }
+__device__ __forceinline__ int unpack_y(int w) { return (int)(signed char)((w >> 8) & 0xFF); }
+__device__ __forceinline__ int unpack_z(int w) { return (int)(signed char)((w >> 16) & 0xFF); }
+
 struct Dims {
     int X, Y, Z, V, spuck, dim;
 };
 
 // Kernel parameters (one struct, passed by value).
 struct KParams {
-    const int *prog;
-    const int *rep_ids;
-    const int4 *dims;              // X, Y, Z, V
+    const int *prog;               // packed op program in HBM (copied to LDS)
+    const int *rep_ids;            // replicas of this launch
+    const int4 *dims;              // X, Y, Z, V per replica
     const long long *lat_off;      // bytes into lat_pool
     const long long *avail_off;    // words into avail_pool
     const unsigned *seed;
@@ -71,12 +85,17 @@ struct KParams {
     int *status;
     long long *executed;
     long long *budget;             // optional, per launch slot
+    int *out;                      // misc kernel outputs
     long long nsteps;
-    int P, S, spuck, dim, maxR;
-    int lat_lds;                   // LDS bytes reserved for the lattice (0: lattice in HBM)
+    int nrep;                      // replicas in this launch
+    int wpg;                       // replica-waves per workgroup
+    int prog_words, prog_lds;      // program size (words) and its LDS bytes
+    int wave_lds;                  // LDS bytes per wave (lattice + mailbox)
+    int lat_lds;                   // LDS bytes of the lattice part (0: lattice in HBM)
+    int P, S, spuck, dim, maxR, backend;
+    int off_seg, off_proc, off_touch, off_nli;
     int mode;                      // misc kernel mode
     int arg0, arg1, arg2, arg3;    // misc kernel arguments
-    int *out;                      // misc kernel outputs
 };
 
 // ---- helpers --------------------------------------------------------------
@@ -92,10 +111,33 @@ __device__ __forceinline__ int wrap1(int a, int n)
 __device__ __forceinline__ int site0(const Dims &d, int x, int y, int z,
                                      int dx, int dy, int dz, int n)
 {
-    int xx = wrap1(x + dx, d.X);
-    int yy = d.dim >= 2 ? wrap1(y + dy, d.Y) : 0;
-    int zz = d.dim >= 3 ? wrap1(z + dz, d.Z) : 0;
+    const int xx = wrap1(x + dx, d.X);
+    const int yy = d.dim >= 2 ? wrap1(y + dy, d.Y) : 0;
+    const int zz = d.dim >= 3 ? wrap1(z + dz, d.Z) : 0;
     return d.spuck * (xx + d.X * (yy + d.Y * zz)) + n - 1;
+}
+
+// a / d for 0 <= a < 2^31 with m = ceil(2^32 / d) (d > 1): one multiply-high
+// and one correction.
+struct Magic {
+    unsigned m;
+    int d;
+};
+__device__ __forceinline__ Magic make_magic(int d)
+{
+    Magic g;
+    g.d = d;
+    g.m = d > 1 ? (0xFFFFFFFFu / (unsigned)d) + 1u : 0u;
+    return g;
+}
+__device__ __forceinline__ int divm(int a, const Magic &g)
+{
+    if (g.d == 1)
+        return a;
+    unsigned q = __umulhi((unsigned)a, g.m);
+    if (q * (unsigned)g.d > (unsigned)a)
+        q -= 1;
+    return (int)q;
 }
 
 __device__ __forceinline__ int readlane_i(int v, int lane)
@@ -105,10 +147,20 @@ __device__ __forceinline__ int readlane_i(int v, int lane)
 
 __device__ __forceinline__ double readlane_d(double v, int lane)
 {
-    long long b = __double_as_longlong(v);
-    unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
-    unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// LDS ordering among the lanes of one wave: LDS instructions of a wave
+// execute in issue order, so only compiler reordering has to be stopped.
+// Unlike __syncthreads() this does not wait for outstanding HBM stores.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Philox4x32-10 (Salmon et al., SC'11)
@@ -117,10 +169,10 @@ __device__ __forceinline__ void philox(unsigned c0, unsigned c1, unsigned c2, un
 {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        unsigned hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
-        unsigned hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
-        unsigned n0 = hi1 ^ c1 ^ k0;
-        unsigned n2 = hi0 ^ c3 ^ k1;
+        const unsigned hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        const unsigned hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        const unsigned n0 = hi1 ^ c1 ^ k0;
+        const unsigned n2 = hi0 ^ c3 ^ k1;
         c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
         k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
     }
@@ -129,7 +181,7 @@ __device__ __forceinline__ void philox(unsigned c0, unsigned c1, unsigned c2, un
 
 __device__ __forceinline__ double u52(unsigned lo, unsigned hi)
 {
-    unsigned long long x = ((unsigned long long)hi << 32) | lo;
+    const unsigned long long x = ((unsigned long long)hi << 32) | lo;
     return ((double)(x >> 12) + 0.5) * 0x1p-52;
 }
 
@@ -145,8 +197,8 @@ __device__ __forceinline__ void step_uniforms(unsigned seed, unsigned stream,
     u[2] = u52(o[0], o[1]);
 }
 
-// Bit-packed lattice: BITS = 8 stores the species as int8; BITS = 2 or 4
-// store species+1 (0 = null species -1).
+// Bit-packed lattice: BITS = 8 stores the species as int8; BITS = 2 stores
+// species+1 (0 = null species -1).
 template <int BITS>
 struct Lattice {
     unsigned char *p;
@@ -154,7 +206,7 @@ struct Lattice {
     {
         if (BITS == 8)
             return (int)(signed char)p[s];
-        int bit = s * BITS;
+        const int bit = s * BITS;
         return (int)((p[bit >> 3] >> (bit & 7)) & ((1 << BITS) - 1)) - 1;
     }
     __device__ __forceinline__ void set(int s, int v)
@@ -163,16 +215,16 @@ struct Lattice {
             p[s] = (unsigned char)(signed char)v;
             return;
         }
-        int bit = s * BITS;
-        unsigned char m = (unsigned char)(((1 << BITS) - 1) << (bit & 7));
-        unsigned char b = p[bit >> 3];
+        const int bit = s * BITS;
+        const unsigned char m = (unsigned char)(((1 << BITS) - 1) << (bit & 7));
+        const unsigned char b = p[bit >> 3];
         p[bit >> 3] = (unsigned char)((b & ~m) | ((((unsigned)(v + 1)) << (bit & 7)) & m));
     }
 };
 
 __host__ __device__ inline long long lattice_bytes(int V, int bits)
 {
-    long long b = ((long long)V * bits + 7) / 8;
+    const long long b = ((long long)V * bits + 7) / 8;
     return (b + 15) & ~15LL;
 }
 

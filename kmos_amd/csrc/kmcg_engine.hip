@@ -38,15 +38,19 @@ constexpr int MODE_TOUCHUP = 2;    // one touchup at (arg0..arg3)
 constexpr int MODE_RUN_PROC = 3;   // run_proc_nr(arg0, arg1)
 constexpr int MODE_NEXT_STEP = 4;  // get_next_kmc_step -> out[0..1]
 
+// One replica driven by one wavefront.
 template <int PPL, int BITS>
-struct Rep {
-    const int *prog;
+struct Wave {
+    const int *prog;   // packed op program (LDS)
     Dims d;
+    Magic mx, my, ms;  // division by X, Y, spuck
     Lattice<BITS> lat;
-    int *mbox;
-    int *avail;        // this replica's avail block: sites[P][V+1] then back[P][V+1]
+    int *mbox;         // [maxR][64*PPL]
+    int *rp;           // lane-private replay area in LDS: [3][KMAX] ints
+    int *avail;        // sites[P][V+1] then back[P][V+1]
     long long V1;
     int P, lane, mstride;
+    int off_seg, off_proc, off_touch, off_nli;
     int nr[PPL];
     double rate[PPL];
     double integ[PPL];
@@ -67,19 +71,22 @@ __device__ __forceinline__ T slot_get(const T (&a)[PPL], int k)
 }
 
 // nli_<group>(cell): walk the case tree (kmos/io/__init__.py:1985-2057).
-template <int BITS>
-__device__ int nli_eval(const int *prog, int pos, const Lattice<BITS> &lat, const Dims &d,
-                        int cx, int cy, int cz)
+template <int PPL, int BITS>
+__device__ int nli_eval(const Wave<PPL, BITS> &W, int group, int cx, int cy, int cz)
 {
+    const int *prog = W.prog;
+    int pos = prog[W.off_nli + group];
     for (;;) {
         const int *nd = prog + pos;
-        const int nc = nd[4];
+        const int nc = nd[1];
         if (nc == -1)
-            return nd[5];
-        const int sp = lat.get(site0(d, cx, cy, cz, nd[0], nd[1], nd[2], nd[3]));
+            return nd[2];
+        const int c = nd[0];
+        const int sp = W.lat.get(site0(W.d, cx, cy, cz, unpack_x(c), unpack_y(c), unpack_z(c),
+                                       (c >> 24) & 0xFF));
         int next = -1;
         for (int k = 0; k < nc; ++k) {
-            const int *e = nd + 5 + 3 * k;
+            const int *e = nd + 2 + 3 * k;
             if (sp >= 0 && ((e[0] >> sp) & 1)) {
                 if (e[1] == 1)
                     return e[2];
@@ -93,127 +100,109 @@ __device__ int nli_eval(const int *prog, int pos, const Lattice<BITS> &lat, cons
     }
 }
 
-// Execute one segment at base cell (bx, by, bz).  Wave-uniform call.
+// Apply the ops posted to this lane's processes (rounds 0..nrounds-1).
+// Per process the posted ops are deletions followed by additions.  All
+// deletions of a process are replayed from ONE batch of HBM reads: the
+// back-index of every site to delete and the last KMAX list slots; the
+// swap-with-last of del_proc (base.mpy:211-265) is then done in registers,
+// tracking the slots and back-indices it changes, and written out in order.
 template <int PPL, int BITS>
-__device__ int run_segment(Rep<PPL, BITS> &R, int seg, int bx, int by, int bz)
+__device__ int apply_segment(Wave<PPL, BITS> &W, int nrounds)
 {
-    const int *prog = R.prog;
-    const int *sg = prog + prog[H_SEG] + SEG_W * seg;
-    const int nw = sg[0], woff = sg[1], nops = sg[2], ooff = sg[3], nrounds = sg[4];
-
-    // lattice writes with the replace_species check (base.mpy:1187-1231)
-    for (int w = 0; w < nw; ++w) {
-        const int *wr = prog + woff + WR_W * w;
-        const int s = site0(R.d, bx, by, bz, wr[0], wr[1], wr[2], wr[3]);
-        const int cur = R.lat.get(s);
-        if (cur < 0 || !((wr[4] >> cur) & 1))
-            return ST_MISMATCH;
-        __syncthreads();
-        if (R.lane == 0)
-            R.lat.set(s, wr[5]);
-        __syncthreads();
-    }
-    __syncthreads();
-
-    // evaluate ops lane-parallel and post them to the owner's mailbox
-    for (int base = 0; base < nops; base += 64) {
-        const int j = base + R.lane;
-        if (j < nops) {
-            const int *op = prog + ooff + OP_W * j;
-            const int kind = op[0], rnd = op[1], arg = op[2];
-            int proc = 0, code = 0, s;
-            if (kind == OP_DEL_IF || kind == OP_ADD_IF) {
-                s = site0(R.d, bx, by, bz, op[3], op[4], op[5], op[6]);
-                if (kind == OP_DEL_IF) {
-                    proc = arg;
-                    code = MB_DEL_IF;
-                } else {
-                    bool ok = true;
-                    const int *cd = prog + op[8];
-                    for (int c = 0; c < op[7]; ++c, cd += CD_W) {
-                        if (R.lat.get(site0(R.d, bx, by, bz, cd[0], cd[1], cd[2], cd[3])) != cd[4]) {
-                            ok = false;
-                            break;
-                        }
-                    }
-                    if (ok) {
-                        proc = arg;
-                        code = MB_ADD;
-                    }
-                }
-            } else {
-                const int cx = wrap1(bx + op[3], R.d.X);
-                const int cy = R.d.dim >= 2 ? wrap1(by + op[4], R.d.Y) : 0;
-                const int cz = R.d.dim >= 3 ? wrap1(bz + op[5], R.d.Z) : 0;
-                s = site0(R.d, cx, cy, cz, 0, 0, 0, 1);
-                proc = nli_eval<BITS>(prog, prog[prog[H_NLI] + arg], R.lat, R.d, cx, cy, cz);
-                code = (kind == OP_DEL_NLI) ? MB_DEL_STRICT : MB_ADD;
-            }
-            if (proc > 0)
-                R.mbox[rnd * R.mstride + proc - 1] = ((s + 1) << 2) | code;
-        }
-    }
-    __syncthreads();
-
-    // apply: in round r each lane executes the op posted for its process(es)
     int err = 0;
-    for (int r = 0; r < nrounds; ++r) {
-        int v[PPL], m[PPL], moved[PPL];
 #pragma unroll
-        for (int k = 0; k < PPL; ++k) {
-            const int p = R.lane + 64 * k;
-            v[k] = 0;
-            if (p < R.P) {
-                int *slot = R.mbox + r * R.mstride + p;
-                v[k] = *slot;
-                if (v[k])
-                    *slot = 0;
+    for (int k = 0; k < PPL; ++k) {
+        const int p = W.lane + 64 * k;
+        if (p >= W.P)
+            continue;
+        int *mb = W.mbox + p;
+        int *sp = W.sites(p);
+        int *bp = W.back(p);
+        // lane-private replay area: ds[KMAX] sites to delete, pos[KMAX]
+        // their current slots, tail[KMAX] the content of slots nr0, nr0-1, ..
+        int *ds = W.rp, *pos = W.rp + KMAX, *tail = W.rp + 2 * KMAX;
+        int r = 0;
+        for (;;) {
+            // collect up to KMAX deletions in round order
+            int strict = 0, nd = 0;
+            while (r < nrounds && nd < KMAX) {
+                const int v = mb[r * W.mstride];
+                if (v != 0) {
+                    if ((v & 3) == MB_ADD)
+                        break;
+                    mb[r * W.mstride] = 0;
+                    ds[nd] = v >> 2;
+                    strict |= ((v & 3) == MB_DEL_STRICT) << nd;
+                    nd += 1;
+                }
+                r += 1;
             }
-        }
-        // issue the reads of every slot before any write (different
-        // processes never alias)
+            if (nd == 0)
+                break;
+            // one batch of HBM reads: back-index of each site, last nd slots
+            const int nr0 = W.nr[k];
+            int bv[KMAX], tv[KMAX];
 #pragma unroll
-        for (int k = 0; k < PPL; ++k) {
-            m[k] = 0;
-            moved[k] = 0;
-            if (v[k] && (v[k] & 3) != MB_ADD) {
-                const int p = R.lane + 64 * k;
-                m[k] = R.back(p)[v[k] >> 2];
-                moved[k] = R.sites(p)[R.nr[k]];
+            for (int i = 0; i < KMAX; ++i) {
+                bv[i] = 0;
+                tv[i] = 0;
+                if (i < nd) {
+                    bv[i] = bp[ds[i]];
+                    if (nr0 - i >= 0)
+                        tv[i] = sp[nr0 - i];
+                }
             }
-        }
 #pragma unroll
-        for (int k = 0; k < PPL; ++k) {
-            if (!v[k])
-                continue;
-            const int p = R.lane + 64 * k;
-            const int site = v[k] >> 2, code = v[k] & 3;
-            int *sp = R.sites(p);
-            int *bp = R.back(p);
-            if (code == MB_ADD) {             // base.mpy:268-301 add_proc
-                const int n1 = R.nr[k] + 1;
-                if (n1 > R.d.V) {             // only a double add can get here
-                    err |= 2;
+            for (int i = 0; i < KMAX; ++i) {
+                pos[i] = bv[i];
+                tail[i] = tv[i];
+            }
+            // replay; the j-th successful deletion moves slot nr0-j
+            int done = 0;
+            for (int i = 0; i < nd; ++i) {
+                const int site = ds[i];
+                const int m = pos[i];
+                if (m == 0) {                       // not registered: can_do false
+                    if ((strict >> i) & 1)
+                        err |= 1;
                     continue;
                 }
-                sp[n1] = site;
-                bp[site] = n1;
-                R.nr[k] = n1;
-            } else if (m[k] == 0) {           // not registered
-                if (code == MB_DEL_STRICT)
-                    err |= 1;
-            } else {                          // base.mpy:211-265 del_proc
-                const int last = R.nr[k];
-                if (m[k] < last) {
-                    sp[m[k]] = moved[k];
+                const int last = nr0 - done;
+                if (m < last) {
+                    const int moved = tail[done];
+                    sp[m] = moved;
                     sp[last] = 0;
-                    bp[moved[k]] = m[k];
+                    bp[moved] = m;
+                    if (nr0 - m < KMAX)
+                        tail[nr0 - m] = moved;
+                    for (int j = i + 1; j < nd; ++j)
+                        if (ds[j] == moved) pos[j] = m;
                 } else {
-                    sp[m[k]] = 0;
+                    sp[m] = 0;
                 }
+                tail[done] = 0;
                 bp[site] = 0;
-                R.nr[k] = last - 1;
+                for (int j = i + 1; j < nd; ++j)
+                    if (ds[j] == site) pos[j] = 0;
+                done += 1;
             }
+            W.nr[k] = nr0 - done;
+        }
+        // additions (base.mpy:268-301 add_proc)
+        for (; r < nrounds; ++r) {
+            const int v = mb[r * W.mstride];
+            if (v == 0)
+                continue;
+            mb[r * W.mstride] = 0;
+            const int site = v >> 2;
+            const int n1 = W.nr[k] + 1;
+            if (n1 > W.d.V) {                       // only a double add can get here
+                err |= 2;
+                continue;
+            }
+            sp[n1] = site;
+            bp[site] = n1;
+            W.nr[k] = n1;
         }
     }
     if (__ballot(err & 2))
@@ -221,23 +210,98 @@ __device__ int run_segment(Rep<PPL, BITS> &R, int seg, int bx, int by, int bz)
     return __ballot(err & 1) ? ST_DEL_ABSENT : ST_OK;
 }
 
-// run_proc_nr(proc, site): procstat is counted by the caller.
+// Execute one segment at base cell (bx, by, bz).  Wave-uniform call.
 template <int PPL, int BITS>
-__device__ int run_proc(Rep<PPL, BITS> &R, int proc, int site)
+__device__ int run_segment(Wave<PPL, BITS> &W, int seg, int bx, int by, int bz)
+{
+    const int *prog = W.prog;
+    const int *sg = prog + W.off_seg + SEG_W * seg;
+    const int nw = sg[0], woff = sg[1], nops = sg[2] & 0xFFFF, nrounds = (sg[2] >> 16) & 0xFF;
+    const bool as_add = (sg[2] >> 24) & 1;
+    const int ooff = sg[3], cbase = sg[4];
+
+    // lattice writes with the replace_species check (base.mpy:1187-1231)
+    for (int w = 0; w < nw; ++w) {
+        const int w0 = prog[woff + WR_W * w], w1 = prog[woff + WR_W * w + 1];
+        const int s = site0(W.d, bx, by, bz, unpack_x(w0), unpack_y(w0), unpack_z(w0), (w0 >> 24) & 0xFF);
+        const int cur = W.lat.get(s);
+        if (cur < 0 || !((w1 >> cur) & 1))
+            return ST_MISMATCH;
+        wave_sync();
+        if (W.lane == 0)
+            W.lat.set(s, w1 >> 16);
+        wave_sync();
+    }
+
+    // evaluate ops lane-parallel and post them to the owner's mailbox
+    for (int base = 0; base < nops; base += 64) {
+        const int j = base + W.lane;
+        if (j < nops) {
+            const int2 op = *(const int2 *)(prog + ooff + OP_W * j);
+            int kind = op.x & 7;
+            if (as_add && kind == OP_DEL_NLI)
+                kind = OP_ADD_NLI;
+            const int rnd = (op.x >> 3) & 0xFF, n = (op.x >> 11) & 0x3F;
+            const int arg = (int)((unsigned)op.x >> 23);
+            const int dx = sext7(op.y & 0x7F), dy = sext7((op.y >> 7) & 0x7F), dz = sext7((op.y >> 14) & 0x7F);
+            int proc = 0, code = 0, s;
+            if (kind == OP_DEL_IF || kind == OP_ADD_IF) {
+                s = site0(W.d, bx, by, bz, dx, dy, dz, n);
+                if (kind == OP_DEL_IF) {
+                    proc = arg;
+                    code = MB_DEL_IF;
+                } else {
+                    const int nc = (op.x >> 17) & 0x3F;
+                    const int *cd = prog + cbase + (int)((unsigned)op.y >> 21);
+                    bool ok = true;
+                    for (int c = 0; c < nc && ok; ++c) {
+                        const int cw = cd[c];
+                        const int q = W.lat.get(site0(W.d, bx, by, bz, (cw & 63) - 32, ((cw >> 6) & 63) - 32,
+                                                      ((cw >> 12) & 63) - 32, (cw >> 18) & 63));
+                        ok = (q == ((cw >> 24) & 0xFF));
+                    }
+                    if (ok) {
+                        proc = arg;
+                        code = MB_ADD;
+                    }
+                }
+            } else {
+                const int cx = wrap1(bx + dx, W.d.X);
+                const int cy = W.d.dim >= 2 ? wrap1(by + dy, W.d.Y) : 0;
+                const int cz = W.d.dim >= 3 ? wrap1(bz + dz, W.d.Z) : 0;
+                s = site0(W.d, cx, cy, cz, 0, 0, 0, 1);
+                proc = nli_eval<PPL, BITS>(W, arg, cx, cy, cz);
+                code = (kind == OP_DEL_NLI) ? MB_DEL_STRICT : MB_ADD;
+            }
+            if (proc > 0)
+                W.mbox[rnd * W.mstride + proc - 1] = ((s + 1) << 2) | code;
+        }
+    }
+    wave_sync();
+    const int e = apply_segment<PPL, BITS>(W, nrounds);
+    wave_sync();
+    return e;
+}
+
+// run_proc_nr(proc, site); procstat is counted by the caller.
+template <int PPL, int BITS>
+__device__ int run_proc(Wave<PPL, BITS> &W, int proc, int site)
 {
     const int s0 = site - 1;
-    const int cell = s0 / R.d.spuck;
-    const int x = cell % R.d.X;
-    const int y = (cell / R.d.X) % R.d.Y;
-    const int z = cell / (R.d.X * R.d.Y);
-    const int *pp = R.prog + R.prog[H_PROG] + 2 * (proc - 1);
+    const int cell = divm(s0, W.ms);
+    const int yz = divm(cell, W.mx);
+    const int x = cell - yz * W.d.X;
+    const int z = divm(yz, W.my);
+    const int y = yz - z * W.d.Y;
+    const int *pp = W.prog + W.off_proc + 2 * (proc - 1);
     const int nit = pp[0];
-    const int *it = R.prog + pp[1];
+    const int *it = W.prog + pp[1];
     for (int i = 0; i < nit; ++i, it += IT_W) {
-        const int bx = wrap1(x + it[1], R.d.X);
-        const int by = R.d.dim >= 2 ? wrap1(y + it[2], R.d.Y) : 0;
-        const int bz = R.d.dim >= 3 ? wrap1(z + it[3], R.d.Z) : 0;
-        const int e = run_segment<PPL, BITS>(R, it[0], bx, by, bz);
+        const int o = it[1];
+        const int bx = wrap1(x + unpack_x(o), W.d.X);
+        const int by = W.d.dim >= 2 ? wrap1(y + unpack_y(o), W.d.Y) : 0;
+        const int bz = W.d.dim >= 3 ? wrap1(z + unpack_z(o), W.d.Z) : 0;
+        const int e = run_segment<PPL, BITS>(W, it[0], bx, by, bz);
         if (e)
             return e;
     }
@@ -291,90 +355,118 @@ __device__ __forceinline__ int interval_search(const double (&acc)[PPL], double 
     return pick;
 }
 
+// Copy the op program to LDS (whole workgroup), then set up this wave.
+// Returns the replica index, or -1 for a wave without one.
 template <int PPL, int BITS, bool LDS_LAT>
-__device__ __forceinline__ void rep_load(Rep<PPL, BITS> &R, const KParams &kp, int r, unsigned char *smem)
+__device__ __forceinline__ int wave_setup(Wave<PPL, BITS> &W, const KParams &kp, unsigned char *smem)
 {
-    R.prog = kp.prog;
-    R.lane = threadIdx.x;
-    R.P = kp.P;
-    R.mstride = 64 * PPL;
-    const int4 dd = kp.dims[r];
-    R.d.X = dd.x; R.d.Y = dd.y; R.d.Z = dd.z; R.d.V = dd.w;
-    R.d.spuck = kp.spuck; R.d.dim = kp.dim;
-    R.V1 = (long long)dd.w + 1;
-    R.avail = kp.avail_pool + kp.avail_off[r];
+    int *sprog = (int *)smem;
+    for (int i = threadIdx.x; i < kp.prog_words; i += blockDim.x)
+        sprog[i] = kp.prog[i];
+    __syncthreads();
+    // wave-uniform values: keep them in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int slot = blockIdx.x * kp.wpg + wave;
+    if (slot >= kp.nrep)
+        return -1;
+    const int r = __builtin_amdgcn_readfirstlane(kp.rep_ids[slot]);
+    unsigned char *wsm = smem + kp.prog_lds + (size_t)wave * kp.wave_lds;
+    W.prog = sprog;
+    W.lane = threadIdx.x & 63;
+    W.P = kp.P;
+    W.mstride = 64 * PPL;
+    W.off_seg = kp.off_seg;
+    W.off_proc = kp.off_proc;
+    W.off_touch = kp.off_touch;
+    W.off_nli = kp.off_nli;
+    const int4 dd0 = kp.dims[r];
+    int4 dd;
+    dd.x = __builtin_amdgcn_readfirstlane(dd0.x);
+    dd.y = __builtin_amdgcn_readfirstlane(dd0.y);
+    dd.z = __builtin_amdgcn_readfirstlane(dd0.z);
+    dd.w = __builtin_amdgcn_readfirstlane(dd0.w);
+    W.d.X = dd.x; W.d.Y = dd.y; W.d.Z = dd.z; W.d.V = dd.w;
+    W.d.spuck = kp.spuck; W.d.dim = kp.dim;
+    W.mx = make_magic(dd.x);
+    W.my = make_magic(dd.y);
+    W.ms = make_magic(kp.spuck);
+    W.V1 = (long long)dd.w + 1;
+    W.avail = kp.avail_pool + kp.avail_off[r];
     unsigned char *glat = kp.lat_pool + kp.lat_off[r];
-    const long long lb = lattice_bytes(dd.w, BITS);
     if (LDS_LAT) {
+        const long long lb = lattice_bytes(dd.w, BITS);
         const uint4 *src = (const uint4 *)glat;
-        uint4 *dst = (uint4 *)smem;
-        for (long long i = R.lane; i < lb / 16; i += 64)
+        uint4 *dst = (uint4 *)wsm;
+        for (long long i = W.lane; i < lb / 16; i += 64)
             dst[i] = src[i];
-        R.lat.p = smem;
-        R.mbox = (int *)(smem + kp.lat_lds);
+        W.lat.p = wsm;
+        W.mbox = (int *)(wsm + kp.lat_lds);
     } else {
-        R.lat.p = glat;
-        R.mbox = (int *)smem;
+        W.lat.p = glat;
+        W.mbox = (int *)wsm;
     }
-    for (int i = R.lane; i < kp.maxR * R.mstride; i += 64)
-        R.mbox[i] = 0;
+    for (int i = W.lane; i < kp.maxR * W.mstride; i += 64)
+        W.mbox[i] = 0;
+    W.rp = W.mbox + kp.maxR * W.mstride + 3 * KMAX * W.lane;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-        const int p = R.lane + 64 * k;
-        if (p < R.P) {
-            const long long o = (long long)r * R.P + p;
-            R.nr[k] = kp.nr[o];
-            R.rate[k] = kp.rates[o];
-            R.integ[k] = kp.integ[o];
-            R.pstat[k] = kp.pstat[o];
+        const int p = W.lane + 64 * k;
+        if (p < W.P) {
+            const long long o = (long long)r * W.P + p;
+            W.nr[k] = kp.nr[o];
+            W.rate[k] = kp.rates[o];
+            W.integ[k] = kp.integ[o];
+            W.pstat[k] = kp.pstat[o];
         } else {
-            R.nr[k] = 0; R.rate[k] = 0.0; R.integ[k] = 0.0; R.pstat[k] = 0;
+            W.nr[k] = 0; W.rate[k] = 0.0; W.integ[k] = 0.0; W.pstat[k] = 0;
         }
     }
-    __syncthreads();
+    wave_sync();
+    return r;
 }
 
 template <int PPL, int BITS, bool LDS_LAT>
-__device__ __forceinline__ void rep_store(Rep<PPL, BITS> &R, const KParams &kp, int r, unsigned char *smem,
-                                          bool lattice_changed)
+__device__ __forceinline__ void wave_store(Wave<PPL, BITS> &W, const KParams &kp, int r, bool lattice_changed)
 {
-    __syncthreads();
+    wave_sync();
     if (LDS_LAT && lattice_changed) {
-        const long long lb = lattice_bytes(R.d.V, BITS);
+        const long long lb = lattice_bytes(W.d.V, BITS);
         uint4 *dst = (uint4 *)(kp.lat_pool + kp.lat_off[r]);
-        const uint4 *src = (const uint4 *)smem;
-        for (long long i = R.lane; i < lb / 16; i += 64)
+        const uint4 *src = (const uint4 *)W.lat.p;
+        for (long long i = W.lane; i < lb / 16; i += 64)
             dst[i] = src[i];
     }
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-        const int p = R.lane + 64 * k;
-        if (p < R.P) {
-            const long long o = (long long)r * R.P + p;
-            kp.nr[o] = R.nr[k];
-            kp.integ[o] = R.integ[k];
-            kp.pstat[o] = R.pstat[k];
+        const int p = W.lane + 64 * k;
+        if (p < W.P) {
+            const long long o = (long long)r * W.P + p;
+            kp.nr[o] = W.nr[k];
+            kp.integ[o] = W.integ[k];
+            kp.pstat[o] = W.pstat[k];
         }
     }
 }
 
-// do_kmc_steps for one replica per workgroup (proclist_generic_subroutines.mpy:1-44)
+// do_kmc_steps for one replica per wave (proclist_generic_subroutines.mpy:1-44)
 template <int PPL, int BITS, bool LDS_LAT>
-__global__ __launch_bounds__(64) void kmc_steps_kernel(KParams kp)
+__global__ __launch_bounds__(64 * MAX_WPG) void kmc_steps_kernel(KParams kp)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int r = kp.rep_ids[blockIdx.x];
+    Wave<PPL, BITS> W;
+    const int r = wave_setup<PPL, BITS, LDS_LAT>(W, kp, smem);
+    if (r < 0)
+        return;
+    const int slot = blockIdx.x * kp.wpg + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = W.lane, P = W.P;
     if (kp.status[r] != ST_OK) {
-        if (threadIdx.x == 0)
-            kp.executed[blockIdx.x] = 0;
+        if (lane == 0)
+            kp.executed[slot] = 0;
         return;
     }
-    Rep<PPL, BITS> R;
-    rep_load<PPL, BITS, LDS_LAT>(R, kp, r, smem);
-    const int lane = R.lane, P = R.P;
     long long n = kp.nsteps;
     if (kp.budget)
-        n = min(n, kp.budget[blockIdx.x]);
+        n = min(n, kp.budget[slot]);
     double t = kp.ktime[r], dt_last = kp.kdt[r];
     long long ks = kp.kstep[r];
     unsigned long long ctr = kp.rngctr[r];
@@ -385,7 +477,7 @@ __global__ __launch_bounds__(64) void kmc_steps_kernel(KParams kp)
         double c[PPL], acc[PPL];
 #pragma unroll
         for (int k = 0; k < PPL; ++k)
-            c[k] = (double)R.nr[k] * R.rate[k];
+            c[k] = (double)W.nr[k] * W.rate[k];
         const double total = accumulate<PPL>(c, acc, P, lane);
         if (!(total > 0.0)) {
             st = ST_ABSORBING;
@@ -402,76 +494,70 @@ __global__ __launch_bounds__(64) void kmc_steps_kernel(KParams kp)
         // update_integ_rate (base.mpy:626-645): (nr*rate)*dt, pre-step nr
 #pragma unroll
         for (int k = 0; k < PPL; ++k)
-            R.integ[k] = R.integ[k] + c[k] * dt;
+            W.integ[k] = W.integ[k] + c[k] * dt;
         // determine_procsite (base.mpy:1075-1120)
         const int pick = interval_search<PPL>(acc, u[1] * total, P, lane);
         const int pk = pick >> 6, pl = pick & 63;
-        const int nrp = readlane_i(slot_get<PPL>(R.nr, pk), pl);
+        const int nrp = readlane_i(slot_get<PPL>(W.nr, pk), pl);
         int kk = (int)(1.0 + u[2] * (double)nrp);
         kk = kk > nrp ? nrp : kk;
         int sv = 0;
         if (lane == pl) {
-            sv = R.sites(pick)[kk];
+            sv = W.sites(pick)[kk];
 #pragma unroll
             for (int k = 0; k < PPL; ++k)
-                if (k == pk) R.pstat[k] += 1;
+                if (k == pk) W.pstat[k] += 1;
         }
         const int site = readlane_i(sv, pl);
         done += 1;
-        st = run_proc<PPL, BITS>(R, pick + 1, site);
+        st = run_proc<PPL, BITS>(W, pick + 1, site);
         if (st)
             break;
     }
-    rep_store<PPL, BITS, LDS_LAT>(R, kp, r, smem, true);
+    wave_store<PPL, BITS, LDS_LAT>(W, kp, r, true);
     if (lane == 0) {
         kp.ktime[r] = t;
         kp.kdt[r] = dt_last;
         kp.kstep[r] = ks;
         kp.rngctr[r] = ctr;
         kp.status[r] = st;
-        kp.executed[blockIdx.x] = done;
+        kp.executed[slot] = done;
         if (kp.budget)
-            kp.budget[blockIdx.x] -= done;
+            kp.budget[slot] -= done;
     }
 }
 
-// touchups, run_proc_nr, get_next_kmc_step (one replica per workgroup)
+// touchups, run_proc_nr, get_next_kmc_step (one replica per wave)
 template <int PPL, int BITS, bool LDS_LAT>
-__global__ __launch_bounds__(64) void kmc_misc_kernel(KParams kp)
+__global__ __launch_bounds__(64 * MAX_WPG) void kmc_misc_kernel(KParams kp)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int r = kp.rep_ids[blockIdx.x];
-    Rep<PPL, BITS> R;
-    rep_load<PPL, BITS, LDS_LAT>(R, kp, r, smem);
-    const int lane = R.lane, P = R.P;
-    const int *prog = R.prog;
+    Wave<PPL, BITS> W;
+    const int r = wave_setup<PPL, BITS, LDS_LAT>(W, kp, smem);
+    if (r < 0)
+        return;
+    const int lane = W.lane, P = W.P;
+    const int *prog = W.prog;
     int st = kp.status[r];
     bool changed = false;
     if (kp.mode == MODE_INIT || kp.mode == MODE_ADJUST || kp.mode == MODE_TOUCHUP) {
-        const bool lat_int = prog[H_BACKEND] == 1;
-        const int nsite = lat_int ? 1 : R.d.spuck;
-        auto touch = [&](int x, int y, int z) -> int {
-            for (int n = 1; n <= nsite; ++n) {
-                const int seg = prog[prog[H_TOUCHUP] + n - 1];
-                const int e = run_segment<PPL, BITS>(R, seg, x, y, z);
-                if (e)
-                    return e;
-            }
-            return 0;
-        };
+        const bool lat_int = kp.backend == 1;
+        const int nsite = lat_int ? 1 : W.d.spuck;
         if (kp.mode == MODE_INIT) {
-            for (int z = 0; z < R.d.Z && !st; ++z)
-                for (int y = 0; y < R.d.Y && !st; ++y)
-                    for (int x = 0; x < R.d.X && !st; ++x)
-                        st = touch(x, y, z);
+            for (int z = 0; z < W.d.Z && !st; ++z)
+                for (int y = 0; y < W.d.Y && !st; ++y)
+                    for (int x = 0; x < W.d.X && !st; ++x)
+                        for (int n = 1; n <= nsite && !st; ++n)
+                            st = run_segment<PPL, BITS>(W, prog[W.off_touch + n - 1], x, y, z);
         } else if (kp.mode == MODE_ADJUST) {
-            for (int x = 0; x < R.d.X && !st; ++x)
-                for (int y = 0; y < R.d.Y && !st; ++y)
-                    for (int z = 0; z < R.d.Z && !st; ++z)
-                        st = touch(x, y, z);
+            for (int x = 0; x < W.d.X && !st; ++x)
+                for (int y = 0; y < W.d.Y && !st; ++y)
+                    for (int z = 0; z < W.d.Z && !st; ++z)
+                        for (int n = 1; n <= nsite && !st; ++n)
+                            st = run_segment<PPL, BITS>(W, prog[W.off_touch + n - 1], x, y, z);
         } else {
-            const int seg = prog[prog[H_TOUCHUP] + (lat_int ? 0 : kp.arg3 - 1)];
-            st = run_segment<PPL, BITS>(R, seg, kp.arg0, kp.arg1, kp.arg2);
+            const int seg = prog[W.off_touch + (lat_int ? 0 : kp.arg3 - 1)];
+            st = run_segment<PPL, BITS>(W, seg, kp.arg0, kp.arg1, kp.arg2);
         }
     } else if (kp.mode == MODE_RUN_PROC) {
         const int proc = kp.arg0, site = kp.arg1;
@@ -479,15 +565,15 @@ __global__ __launch_bounds__(64) void kmc_misc_kernel(KParams kp)
         if (lane == pl) {
 #pragma unroll
             for (int k = 0; k < PPL; ++k)
-                if (k == pk) R.pstat[k] += 1;
+                if (k == pk) W.pstat[k] += 1;
         }
-        st = run_proc<PPL, BITS>(R, proc, site);
+        st = run_proc<PPL, BITS>(W, proc, site);
         changed = true;
     } else if (kp.mode == MODE_NEXT_STEP) {
         double c[PPL], acc[PPL];
 #pragma unroll
         for (int k = 0; k < PPL; ++k)
-            c[k] = (double)R.nr[k] * R.rate[k];
+            c[k] = (double)W.nr[k] * W.rate[k];
         const double total = accumulate<PPL>(c, acc, P, lane);
         if (!(total > 0.0)) {
             st = ST_ABSORBING;
@@ -497,12 +583,12 @@ __global__ __launch_bounds__(64) void kmc_misc_kernel(KParams kp)
             step_uniforms(kp.seed[r], kp.stream[r], ctr, u);
             const int pick = interval_search<PPL>(acc, u[1] * total, P, lane);
             const int pk = pick >> 6, pl = pick & 63;
-            const int nrp = readlane_i(slot_get<PPL>(R.nr, pk), pl);
+            const int nrp = readlane_i(slot_get<PPL>(W.nr, pk), pl);
             int kk = (int)(1.0 + u[0] * (double)nrp);
             kk = kk > nrp ? nrp : kk;
             int sv = 0;
             if (lane == pl)
-                sv = R.sites(pick)[kk];
+                sv = W.sites(pick)[kk];
             const int site = readlane_i(sv, pl);
             if (lane == 0) {
                 kp.out[0] = pick + 1;
@@ -511,24 +597,24 @@ __global__ __launch_bounds__(64) void kmc_misc_kernel(KParams kp)
             }
         }
     }
-    rep_store<PPL, BITS, LDS_LAT>(R, kp, r, smem, changed);
+    wave_store<PPL, BITS, LDS_LAT>(W, kp, r, changed);
     if (lane == 0)
         kp.status[r] = st;
 }
 
-// default species of every site (initialize_state)
+// default species of every site (initialize_state); site defaults are the
+// first spuck words of the program
 template <int BITS>
 __global__ void kmc_fill_default_kernel(KParams kp)
 {
     const int r = kp.rep_ids[blockIdx.x];
     const int4 dd = kp.dims[r];
-    Lattice<BITS> lat;
-    lat.p = kp.lat_pool + kp.lat_off[r];
-    const int *sdef = kp.prog + kp.prog[H_SITEDEF];
+    unsigned char *lat = kp.lat_pool + kp.lat_off[r];
+    const int *sdef = kp.prog;
     const long long lb = lattice_bytes(dd.w, BITS);
     if (BITS == 8) {
         for (long long s = threadIdx.x; s < lb; s += blockDim.x)
-            lat.p[s] = (unsigned char)(signed char)(s < dd.w ? sdef[s % kp.spuck] : 0);
+            lat[s] = (unsigned char)(signed char)(s < dd.w ? sdef[s % kp.spuck] : 0);
     } else {
         const int per = 8 / BITS;
         for (long long b = threadIdx.x; b < lb; b += blockDim.x) {
@@ -538,7 +624,7 @@ __global__ void kmc_fill_default_kernel(KParams kp)
                 const unsigned code = s < dd.w ? (unsigned)(sdef[s % kp.spuck] + 1) : 0u;
                 v |= code << (i * BITS);
             }
-            lat.p[b] = (unsigned char)v;
+            lat[b] = (unsigned char)v;
         }
     }
 }
@@ -600,14 +686,16 @@ struct SegH {
     std::vector<int> writes;  // 6 ints per write
     std::vector<OpH> ops;
     int nrounds = 0;
+    int alias_of = -1;        // lat_int additions: the ops of this deletion segment
 };
 
 } // namespace
 
 struct kmcg_model {
     std::vector<int> blob;
-    std::vector<int> prog;
+    std::vector<int> prog;   // packed op program (layout in kmcg_device.h)
     int P = 0, S = 0, spuck = 0, dim = 0, backend = 0, nullsp = -1, maxoff = 0, maxR = 1;
+    int off_seg = 0, off_proc = 0, off_touch = 0, off_nli = 0;
 };
 
 struct kmcg_batch {
@@ -616,6 +704,7 @@ struct kmcg_batch {
     int n = 0;
     int bits = 8;
     int lds_mode = -1;
+    int max_wpg = MAX_WPG;
     int max_lds = 65536;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -642,6 +731,7 @@ struct kmcg_batch {
     int *out = nullptr;
     double *occ = nullptr;
     float last_ms = 0.f;
+    bool timed_run_pending = false;   // ev0/ev1 bracket a launch not yet read
 };
 
 namespace {
@@ -771,6 +861,7 @@ int build_prog(kmcg_model *m)
                 for (int i = 0; i < nupd; ++i)
                     for (int q = 0; q < 6; ++q)
                         add.writes.push_back(W(uoff + 6 * i + q));
+                add.alias_of = (int)segs.size();
                 segs.push_back(del);
                 segs.push_back(add);
             }
@@ -800,113 +891,174 @@ int build_prog(kmcg_model *m)
             m->maxR = std::max(m->maxR, sg.nrounds);
         }
 
-        // flatten
+        // deletions of a process precede its additions in every segment
+        for (auto &sg : segs) {
+            bool seen_add = false;
+            for (auto &op : sg.ops) {
+                const bool add = op.kind == OP_ADD_IF || op.kind == OP_ADD_NLI;
+                if (!add && seen_add)
+                    return fail(KMCG_EINVAL, "segment deletes after it adds");
+                seen_add |= add;
+            }
+        }
+        auto fits = [](int v, int lo, int hi) { return v >= lo && v <= hi; };
+        // pack (layout in kmcg_device.h)
         std::vector<int> &pg = m->prog;
-        pg.assign(HDR_WORDS, 0);
-        pg[H_P] = m->P; pg[H_S] = m->S; pg[H_SPUCK] = m->spuck; pg[H_DIM] = m->dim;
-        pg[H_BACKEND] = m->backend; pg[H_NULL] = m->nullsp; pg[H_MAXOFF] = m->maxoff;
-        pg[H_MAXR] = m->maxR;
-        pg[H_SITEDEF] = (int)pg.size();
+        pg.clear();
         for (int n = 0; n < m->spuck; ++n)
             pg.push_back(W(W(8) + n));
-        pg[H_NSEG] = (int)segs.size();
-        pg[H_SEG] = (int)pg.size();
+        auto align4 = [&]() { while (pg.size() % 4) pg.push_back(0); };
+        align4();
+        m->off_seg = (int)pg.size();
         pg.resize(pg.size() + SEG_W * segs.size(), 0);
         for (size_t si = 0; si < segs.size(); ++si) {
             SegH &sg = segs[si];
-            const int sb = pg[H_SEG] + SEG_W * (int)si;
-            pg[sb + 0] = (int)sg.writes.size() / WR_W;
+            if (sg.ops.size() > 0xFFFF || sg.nrounds > 255)
+                return fail(KMCG_EUNSUPPORTED, "segment too large");
+            const int nw = (int)sg.writes.size() / 6;
+            const int sb = m->off_seg + SEG_W * (int)si;
+            pg[sb + 0] = nw;
             pg[sb + 1] = (int)pg.size();
-            pg.insert(pg.end(), sg.writes.begin(), sg.writes.end());
-            pg[sb + 2] = (int)sg.ops.size();
+            for (int w = 0; w < nw; ++w) {
+                const int *wr = &sg.writes[6 * w];
+                if (!fits(wr[0], -127, 127) || !fits(wr[1], -127, 127) || !fits(wr[2], -127, 127))
+                    return fail(KMCG_EUNSUPPORTED, "offset too large");
+                pg.push_back(pack3(wr[0], wr[1], wr[2]) | (wr[3] << 24));
+                pg.push_back((wr[4] & 0xFFFF) | (wr[5] << 16));
+            }
+            if (sg.alias_of >= 0) {
+                // additions of a lat_int group: same op list as its deletions
+                const SegH &src = segs[sg.alias_of];
+                bool same = src.ops.size() == sg.ops.size() && src.nrounds == sg.nrounds;
+                for (size_t j = 0; same && j < sg.ops.size(); ++j)
+                    same = src.ops[j].kind == OP_DEL_NLI && sg.ops[j].kind == OP_ADD_NLI &&
+                           src.ops[j].arg == sg.ops[j].arg && src.ops[j].round == sg.ops[j].round &&
+                           src.ops[j].dx == sg.ops[j].dx && src.ops[j].dy == sg.ops[j].dy &&
+                           src.ops[j].dz == sg.ops[j].dz;
+                if (!same || sg.alias_of >= (int)si)
+                    return fail(KMCG_EINVAL, "lat_int addition segment does not mirror its deletions");
+                const int ab = m->off_seg + SEG_W * sg.alias_of;
+                pg[sb + 2] = (int)sg.ops.size() | (sg.nrounds << 16) | (1 << 24);
+                pg[sb + 3] = pg[ab + 3];
+                pg[sb + 4] = pg[ab + 4];
+                continue;
+            }
+            pg[sb + 2] = (int)sg.ops.size() | (sg.nrounds << 16);
+            if (pg.size() % 2)
+                pg.push_back(0);
             const int ob = (int)pg.size();
             pg[sb + 3] = ob;
-            pg[sb + 4] = sg.nrounds;
             pg.resize(pg.size() + OP_W * sg.ops.size(), 0);
+            const int cb = (int)pg.size();
+            pg[sb + 4] = cb;
             for (size_t j = 0; j < sg.ops.size(); ++j) {
                 const OpH &op = sg.ops[j];
                 const int o = ob + OP_W * (int)j;
-                pg[o + 0] = op.kind; pg[o + 1] = op.round; pg[o + 2] = op.arg;
-                pg[o + 3] = op.dx; pg[o + 4] = op.dy; pg[o + 5] = op.dz; pg[o + 6] = op.n;
-                pg[o + 7] = (int)op.conds.size() / CD_W;
-                pg[o + 8] = (int)pg.size();
-                pg.insert(pg.end(), op.conds.begin(), op.conds.end());
+                const int nc = (int)op.conds.size() / 5;
+                const int crel = (int)pg.size() - cb;
+                if (!fits(op.dx, -63, 63) || !fits(op.dy, -63, 63) || !fits(op.dz, -63, 63) ||
+                    nc > 63 || op.round > 255 || op.n > 63 || op.arg < 0 || op.arg > 511 || crel > 2047)
+                    return fail(KMCG_EUNSUPPORTED, "op out of packing range");
+                pg[o + 0] = op.kind | (op.round << 3) | (op.n << 11) | (nc << 17) | (op.arg << 23);
+                pg[o + 1] = (op.dx & 127) | ((op.dy & 127) << 7) | ((op.dz & 127) << 14) | (crel << 21);
+                for (int c = 0; c < nc; ++c) {
+                    const int *cd = &op.conds[5 * c];
+                    if (!fits(cd[0], -32, 31) || !fits(cd[1], -32, 31) || !fits(cd[2], -32, 31) ||
+                        cd[3] > 63 || cd[4] < 0 || cd[4] > 255)
+                        return fail(KMCG_EUNSUPPORTED, "condition out of packing range");
+                    pg.push_back((cd[0] + 32) | ((cd[1] + 32) << 6) | ((cd[2] + 32) << 12) |
+                                 (cd[3] << 18) | (cd[4] << 24));
+                }
             }
         }
-        pg[H_PROG] = (int)pg.size();
+        m->off_proc = (int)pg.size();
         pg.resize(pg.size() + 2 * m->P, 0);
         for (int p = 0; p < m->P; ++p) {
-            pg[pg[H_PROG] + 2 * p] = (int)proc_items[p].size() / IT_W;
-            pg[pg[H_PROG] + 2 * p + 1] = (int)pg.size();
-            for (size_t i = 0; i < proc_items[p].size(); i += IT_W)
-                if (proc_items[p][i] < 0 || proc_items[p][i] >= (int)segs.size())
+            pg[m->off_proc + 2 * p] = (int)proc_items[p].size() / 4;
+            pg[m->off_proc + 2 * p + 1] = (int)pg.size();
+            for (size_t i = 0; i < proc_items[p].size(); i += 4) {
+                const int *it = &proc_items[p][i];
+                if (it[0] < 0 || it[0] >= (int)segs.size())
                     return fail(KMCG_EINVAL, "process program names a bad segment");
-            pg.insert(pg.end(), proc_items[p].begin(), proc_items[p].end());
+                if (!fits(it[1], -127, 127) || !fits(it[2], -127, 127) || !fits(it[3], -127, 127))
+                    return fail(KMCG_EUNSUPPORTED, "offset too large");
+                pg.push_back(it[0]);
+                pg.push_back(pack3(it[1], it[2], it[3]));
+            }
         }
-        pg[H_TOUCHUP] = (int)pg.size();
+        for (int t : touchup)
+            if (t < 0 || t >= (int)segs.size())
+                return fail(KMCG_EINVAL, "bad touchup segment");
+        m->off_touch = (int)pg.size();
         pg.insert(pg.end(), touchup.begin(), touchup.end());
-        pg[H_NGROUPS] = (int)nli_nodes.size();
-        pg[H_NLI] = (int)pg.size();
+        m->off_nli = (int)pg.size();
         pg.resize(pg.size() + nli_nodes.size(), 0);
         for (size_t g = 0; g < nli_nodes.size(); ++g) {
-            const int base = (int)pg.size();
-            pg[pg[H_NLI] + g] = base;
-            std::vector<int> nodes = nli_nodes[g];
-            // child offsets become absolute
-            size_t pos = 0;
-            while (pos < nodes.size()) {
-                const int nc = nodes[pos + 4];
-                if (nc == -1) {
-                    pos += 6;
-                    continue;
-                }
-                for (int k = 0; k < nc; ++k)
-                    if (nodes[pos + 6 + 3 * k] == 0)
-                        nodes[pos + 7 + 3 * k] += base;
-                pos += 5 + 3 * nc;
-            }
-            pg.insert(pg.end(), nodes.begin(), nodes.end());
-        }
-        // every nli tree: nodes in bounds, children strictly after their
-        // parent (so the device walk terminates), leaves name processes
-        for (size_t g = 0; g < nli_nodes.size(); ++g) {
-            const int base = pg[pg[H_NLI] + g];
-            const int end = base + (int)nli_nodes[g].size();
-            int pos = base;
-            while (pos < end) {
-                if (pos + 5 > end)
+            // blob nodes: [dx, dy, dz, n, n_cases, (mask, kind, target)*] or
+            // [.., -1, value]; children at blob-relative offsets
+            const std::vector<int> &src = nli_nodes[g];
+            std::vector<int> old_pos, new_pos;
+            size_t pos = 0, npos = pg.size();
+            while (pos < src.size()) {
+                if (pos + 5 > src.size())
                     return fail(KMCG_EINVAL, "truncated nli node");
-                const int nc = pg[pos + 4];
+                const int nc = src[pos + 4];
+                old_pos.push_back((int)pos);
+                new_pos.push_back((int)npos);
                 if (nc == -1) {
-                    if (pos + 6 > end || pg[pos + 5] < 0 || pg[pos + 5] > m->P)
-                        return fail(KMCG_EINVAL, "bad nli leaf");
+                    if (pos + 6 > src.size())
+                        return fail(KMCG_EINVAL, "truncated nli leaf");
                     pos += 6;
+                    npos += 3;
+                } else {
+                    if (nc < 0 || pos + 5 + 3 * (size_t)nc > src.size())
+                        return fail(KMCG_EINVAL, "bad nli node");
+                    pos += 5 + 3 * nc;
+                    npos += 2 + 3 * nc;
+                }
+            }
+            pg[m->off_nli + g] = (int)pg.size();
+            for (size_t q = 0; q < old_pos.size(); ++q) {
+                const int *nd = &src[old_pos[q]];
+                const int nc = nd[4];
+                if (nc == -1) {
+                    if (nd[5] < 0 || nd[5] > m->P)
+                        return fail(KMCG_EINVAL, "bad nli leaf");
+                    pg.insert(pg.end(), {0, -1, nd[5]});
                     continue;
                 }
-                if (nc < 0 || pos + 5 + 3 * nc > end || pg[pos + 3] < 1 || pg[pos + 3] > m->spuck)
-                    return fail(KMCG_EINVAL, "bad nli node");
+                if (nd[3] < 1 || nd[3] > m->spuck || !fits(nd[0], -127, 127) ||
+                    !fits(nd[1], -127, 127) || !fits(nd[2], -127, 127))
+                    return fail(KMCG_EINVAL, "bad nli node coordinate");
+                pg.push_back(pack3(nd[0], nd[1], nd[2]) | (nd[3] << 24));
+                pg.push_back(nc);
                 for (int k = 0; k < nc; ++k) {
-                    const int kind = pg[pos + 6 + 3 * k], tgt = pg[pos + 7 + 3 * k];
-                    if (kind == 1 && (tgt < 0 || tgt > m->P))
-                        return fail(KMCG_EINVAL, "bad nli leaf value");
-                    if (kind == 0 && (tgt <= pos || tgt >= end))
-                        return fail(KMCG_EINVAL, "bad nli child offset");
-                    if (kind != 0 && kind != 1)
+                    const int mask = nd[5 + 3 * k], kind = nd[6 + 3 * k], tgt = nd[7 + 3 * k];
+                    int t = tgt;
+                    if (kind == 1) {
+                        if (tgt < 0 || tgt > m->P)
+                            return fail(KMCG_EINVAL, "bad nli leaf value");
+                    } else if (kind == 0) {
+                        // child: must come later in this tree (so the walk ends)
+                        auto it = std::find(old_pos.begin(), old_pos.end(), tgt);
+                        if (it == old_pos.end() || *it <= old_pos[q])
+                            return fail(KMCG_EINVAL, "bad nli child offset");
+                        t = new_pos[it - old_pos.begin()];
+                    } else {
                         return fail(KMCG_EINVAL, "bad nli case kind");
+                    }
+                    pg.insert(pg.end(), {mask, kind, t});
                 }
-                pos += 5 + 3 * nc;
             }
         }
         for (size_t si = 0; si < segs.size(); ++si)
-            for (size_t w = 0; w < segs[si].writes.size(); w += WR_W) {
+            for (size_t w = 0; w < segs[si].writes.size(); w += 6) {
                 const int n = segs[si].writes[w + 3], nw = segs[si].writes[w + 5];
                 if (n < 1 || n > m->spuck || nw < 0 || nw >= m->S)
                     return fail(KMCG_EINVAL, "bad lattice write");
             }
-        for (int t : touchup)
-            if (t < 0 || t >= (int)segs.size())
-                return fail(KMCG_EINVAL, "bad touchup segment");
-        pg[H_WORDS] = (int)pg.size();
+        if (pg.size() * sizeof(int) > 64 * 1024)
+            return fail(KMCG_EUNSUPPORTED, "op program larger than 64 KiB");
     } catch (const std::exception &e) {
         return fail(KMCG_EINVAL, std::string("malformed model blob: ") + e.what());
     }
@@ -918,6 +1070,7 @@ int ppl_of(int P) { return P <= 64 ? 1 : (P <= 128 ? 2 : 4); }
 KParams make_params(kmcg_batch *b)
 {
     KParams kp{};
+    const kmcg_model *m = b->m;
     kp.prog = b->prog;
     kp.rep_ids = b->rep_ids;
     kp.dims = b->dims;
@@ -938,58 +1091,83 @@ KParams make_params(kmcg_batch *b)
     kp.status = b->status;
     kp.executed = b->executed;
     kp.out = b->out;
-    kp.P = b->m->P;
-    kp.S = b->m->S;
-    kp.spuck = b->m->spuck;
-    kp.dim = b->m->dim;
-    kp.maxR = b->m->maxR;
+    kp.P = m->P;
+    kp.S = m->S;
+    kp.spuck = m->spuck;
+    kp.dim = m->dim;
+    kp.maxR = m->maxR;
+    kp.backend = m->backend;
+    kp.off_seg = m->off_seg;
+    kp.off_proc = m->off_proc;
+    kp.off_touch = m->off_touch;
+    kp.off_nli = m->off_nli;
+    kp.prog_words = (int)m->prog.size();
+    kp.prog_lds = (int)((m->prog.size() * sizeof(int) + 15) & ~(size_t)15);
     return kp;
 }
 
-// LDS bytes a launch over `ids` needs, and whether the lattice goes to LDS.
-void launch_lds(kmcg_batch *b, const int *ids, int n, bool &lds_lat, size_t &bytes, int &lat_bytes)
+// Launch shape for replicas `ids`: lattice in LDS when it fits, and as many
+// replica-waves per workgroup (sharing one program copy) as LDS allows.
+struct Shape {
+    bool lds_lat;
+    int wpg, grid;
+    size_t lds;
+};
+
+Shape launch_shape(kmcg_batch *b, const int *ids, int n, KParams &kp, int max_wpg)
 {
     long long maxlb = 0;
     for (int i = 0; i < n; ++i)
         maxlb = std::max(maxlb, lattice_bytes(b->V[ids[i]], b->bits));
-    const size_t mbox = (size_t)b->m->maxR * 64 * ppl_of(b->m->P) * sizeof(int);
-    const bool fits = (size_t)maxlb + mbox <= (size_t)b->max_lds;
-    lds_lat = b->lds_mode == 1 ? fits : (b->lds_mode == 0 ? false : fits);
-    lat_bytes = lds_lat ? (int)maxlb : 0;
-    bytes = (size_t)lat_bytes + mbox;
+    const size_t mbox = ((size_t)b->m->maxR * 64 * ppl_of(b->m->P) + 64 * 3 * KMAX) * sizeof(int);
+    const size_t prog = (size_t)kp.prog_lds;
+    Shape s;
+    s.lds_lat = b->lds_mode == 0 ? false : (prog + (size_t)maxlb + mbox <= (size_t)b->max_lds);
+    const size_t per_wave = (s.lds_lat ? (size_t)maxlb : 0) + mbox;
+    s.wpg = std::max(1, std::min(max_wpg, std::min(MAX_WPG, n)));
+    while (s.wpg > 1 && prog + s.wpg * per_wave > (size_t)b->max_lds)
+        s.wpg -= 1;
+    s.grid = (n + s.wpg - 1) / s.wpg;
+    s.lds = prog + s.wpg * per_wave;
+    kp.lat_lds = s.lds_lat ? (int)maxlb : 0;
+    kp.wave_lds = (int)per_wave;
+    kp.wpg = s.wpg;
+    kp.nrep = n;
+    return s;
 }
 
 template <int PPL, int BITS>
-hipError_t launch_t(bool steps, bool lds_lat, int grid, size_t lds, hipStream_t st, const KParams &kp)
+hipError_t launch_t(bool steps, const Shape &s, hipStream_t st, const KParams &kp)
 {
+    const dim3 grid(s.grid), block(64 * s.wpg);
     if (steps) {
-        if (lds_lat)
-            hipLaunchKernelGGL((kmc_steps_kernel<PPL, BITS, true>), dim3(grid), dim3(64), lds, st, kp);
+        if (s.lds_lat)
+            hipLaunchKernelGGL((kmc_steps_kernel<PPL, BITS, true>), grid, block, s.lds, st, kp);
         else
-            hipLaunchKernelGGL((kmc_steps_kernel<PPL, BITS, false>), dim3(grid), dim3(64), lds, st, kp);
+            hipLaunchKernelGGL((kmc_steps_kernel<PPL, BITS, false>), grid, block, s.lds, st, kp);
     } else {
-        if (lds_lat)
-            hipLaunchKernelGGL((kmc_misc_kernel<PPL, BITS, true>), dim3(grid), dim3(64), lds, st, kp);
+        if (s.lds_lat)
+            hipLaunchKernelGGL((kmc_misc_kernel<PPL, BITS, true>), grid, block, s.lds, st, kp);
         else
-            hipLaunchKernelGGL((kmc_misc_kernel<PPL, BITS, false>), dim3(grid), dim3(64), lds, st, kp);
+            hipLaunchKernelGGL((kmc_misc_kernel<PPL, BITS, false>), grid, block, s.lds, st, kp);
     }
     return hipGetLastError();
 }
 
 template <int PPL>
-hipError_t launch_b(int bits, bool steps, bool lds_lat, int grid, size_t lds, hipStream_t st, const KParams &kp)
+hipError_t launch_b(int bits, bool steps, const Shape &s, hipStream_t st, const KParams &kp)
 {
     if (bits == 2)
-        return launch_t<PPL, 2>(steps, lds_lat, grid, lds, st, kp);
-    return launch_t<PPL, 8>(steps, lds_lat, grid, lds, st, kp);
+        return launch_t<PPL, 2>(steps, s, st, kp);
+    return launch_t<PPL, 8>(steps, s, st, kp);
 }
 
-hipError_t launch(kmcg_batch *b, bool steps, bool lds_lat, int grid, size_t lds, const KParams &kp)
+hipError_t launch(kmcg_batch *b, bool steps, const Shape &s, const KParams &kp)
 {
     switch (ppl_of(b->m->P)) {
-    case 1: return launch_b<1>(b->bits, steps, lds_lat, grid, lds, b->stream, kp);
-    case 2: return launch_b<2>(b->bits, steps, lds_lat, grid, lds, b->stream, kp);
-    default: return launch_b<4>(b->bits, steps, lds_lat, grid, lds, b->stream, kp);
+    case 1: return launch_b<1>(b->bits, steps, s, b->stream, kp);
+    case 2: return launch_b<2>(b->bits, steps, s, b->stream, kp);
+    default: return launch_b<4>(b->bits, steps, s, b->stream, kp);
     }
 }
 
@@ -1005,15 +1183,11 @@ int run_misc(kmcg_batch *b, const std::vector<int> &ids, int mode, int a0, int a
 {
     HIPCHK(hipSetDevice(b->device));
     HIPCHK(hipMemcpyAsync(b->rep_ids, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
-    bool lds_lat;
-    size_t lds;
-    int lat_bytes;
-    launch_lds(b, ids.data(), (int)ids.size(), lds_lat, lds, lat_bytes);
     KParams kp = make_params(b);
+    const Shape s = launch_shape(b, ids.data(), (int)ids.size(), kp, MAX_WPG);
     kp.mode = mode;
     kp.arg0 = a0; kp.arg1 = a1; kp.arg2 = a2; kp.arg3 = a3;
-    kp.lat_lds = lat_bytes;
-    HIPCHK(launch(b, false, lds_lat, (int)ids.size(), lds, kp));
+    HIPCHK(launch(b, false, s, kp));
     HIPCHK(hipStreamSynchronize(b->stream));
     return KMCG_OK;
 }
@@ -1056,7 +1230,7 @@ int kmcg_device_count(int32_t *count)
 
 int kmcg_model_load(const int32_t *blob, int64_t nwords, kmcg_model **out)
 {
-    if (!blob || !out || nwords < HDR_WORDS)
+    if (!blob || !out || nwords < 32)
         return fail(KMCG_EINVAL, "null or short model blob");
     if (blob[0] != 0x4B4D4347 || blob[1] != 1 || blob[16] != nwords)
         return fail(KMCG_EINVAL, "not a version-1 model blob");
@@ -1091,8 +1265,8 @@ static int sizes_ok(const kmcg_model *m, int32_t n, const int32_t *sizes)
             return fail(KMCG_EINVAL, "system size must be positive");
         const int lim = std::min(X, std::min(m->dim >= 2 ? Y : X, m->dim >= 3 ? Z : X));
         if (lim < m->maxoff)
-            return fail(KMCG_EINVAL, "system size smaller than the model's largest offset ("
-                                         + std::to_string(m->maxoff) + ")");
+            return fail(KMCG_EINVAL, "system size below the model's minimum of "
+                                         + std::to_string(m->maxoff) + " cells per dimension");
         if ((long long)X * Y * Z * m->spuck >= (1LL << 28))
             return fail(KMCG_EUNSUPPORTED, "more than 2^28 sites per replica");
     }
@@ -1225,11 +1399,12 @@ int kmcg_batch_create(const kmcg_model *m, int32_t n, const int32_t *sizes, int3
     return KMCG_OK;
 }
 
-int kmcg_set_launch(kmcg_batch *b, int32_t lattice_in_lds)
+int kmcg_set_launch(kmcg_batch *b, int32_t lattice_in_lds, int32_t waves_per_group)
 {
-    if (!b || lattice_in_lds < -1 || lattice_in_lds > 1)
+    if (!b || lattice_in_lds < -1 || lattice_in_lds > 1 || waves_per_group < 0 || waves_per_group > MAX_WPG)
         return fail(KMCG_EINVAL, "bad argument");
     b->lds_mode = lattice_in_lds;
+    b->max_wpg = waves_per_group == 0 ? MAX_WPG : waves_per_group;
     return KMCG_OK;
 }
 
@@ -1328,17 +1503,14 @@ int kmcg_run(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps, i
     HIPCHK(hipMemcpyAsync(b->rep_ids, replicas, n * sizeof(int), hipMemcpyHostToDevice, b->stream));
     if (budget)
         HIPCHK(hipMemcpyAsync(b->budget, budget, n * sizeof(long long), hipMemcpyHostToDevice, b->stream));
-    bool lds_lat;
-    size_t lds;
-    int lat_bytes;
-    launch_lds(b, replicas, n, lds_lat, lds, lat_bytes);
     KParams kp = make_params(b);
+    const Shape s = launch_shape(b, replicas, n, kp, b->max_wpg);
     kp.nsteps = steps;
     kp.budget = budget ? b->budget : nullptr;
-    kp.lat_lds = lat_bytes;
     HIPCHK(hipEventRecord(b->ev0, b->stream));
-    HIPCHK(launch(b, true, lds_lat, n, lds, kp));
+    HIPCHK(launch(b, true, s, kp));
     HIPCHK(hipEventRecord(b->ev1, b->stream));
+    b->timed_run_pending = true;
     if (budget) {
         HIPCHK(hipMemcpyAsync(budget, b->budget, n * sizeof(long long), hipMemcpyDeviceToHost, b->stream));
         HIPCHK(hipStreamSynchronize(b->stream));
@@ -1352,9 +1524,12 @@ int kmcg_sync(kmcg_batch *b)
         return fail(KMCG_EINVAL, "null batch");
     HIPCHK(hipSetDevice(b->device));
     HIPCHK(hipStreamSynchronize(b->stream));
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, b->ev0, b->ev1) == hipSuccess)
+    if (b->timed_run_pending) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
         b->last_ms = ms;
+        b->timed_run_pending = false;
+    }
     return KMCG_OK;
 }
 

@@ -113,8 +113,8 @@ class Batch:
         assert flat.size == int(self.volumes[r0:r0 + n].sum())
         _lib.check(_lib.load().kmcg_set_config(self.h, r0, n, _ptr(flat)))
 
-    def set_launch(self, lattice_in_lds: int = -1):
-        _lib.check(_lib.load().kmcg_set_launch(self.h, lattice_in_lds))
+    def set_launch(self, lattice_in_lds: int = -1, waves_per_group: int = 0):
+        _lib.check(_lib.load().kmcg_set_launch(self.h, lattice_in_lds, waves_per_group))
 
     # ---- stepping ---------------------------------------------------------
     def run(self, steps: int, replicas=None, budget=None, sync: bool = True):

@@ -333,6 +333,12 @@ orc_t *orc_create(const int32_t *blob, int64_t nwords, int X, int Y, int Z)
     o->Y = o->dim >= 2 ? Y : 1;
     o->Z = o->dim >= 3 ? Z : 1;
     o->V = o->X * o->Y * o->Z * o->spuck;
+    /* header word 15: smallest lattice extent the model runs on exactly */
+    if (o->X < blob[15] || (o->dim >= 2 && o->Y < blob[15]) || (o->dim >= 3 && o->Z < blob[15])) {
+        free(o->blob);
+        free(o);
+        return NULL;
+    }
     int64_t pv = (int64_t)o->P * (o->V + 1);
     o->lat = (int8_t *)malloc(o->V + 1);
     o->sites = (int32_t *)calloc(pv, sizeof(int32_t));

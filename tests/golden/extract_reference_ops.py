@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Extract golden op sequences from the reference's generated Fortran.
+
+Reads the Fortran that the reference ships as export fixtures (under
+/root/reference/tests) and writes, as JSON data, the ordered bookkeeping
+operations each generated routine performs:
+
+local_smart (proclist.f90)
+  routines  put_/take_<species>_<layer>_<site> and touchup_<layer>_<site>:
+            ["del", proc, [dx,dy,dz,n]] in text order, then
+            ["add", proc, [dx,dy,dz,n], [[dx,dy,dz,n,species], ...]] in text
+            order, the list being the select-case path that guards the add
+  run_proc  per process: [[routine, [dx,dy,dz,n]], ...]
+lat_int (run_proc_*.f90, nli_*.f90, proclist.f90)
+  groups    per group: dels [[group, [dx,dy,dz]]], updates
+            [[[dx,dy,dz,n], [old species], new]], adds (same as dels)
+  nli       per group: leaves [[[[dx,dy,dz,n,[species]]...], result]] in
+            text order (result 0 for the default branches)
+  dispatch  process -> group
+
+Coordinates are made absolute: n is the 1-based site index in layer order.
+Usage: python tests/golden/extract_reference_ops.py [--ref /root/reference]
+"""
+import argparse
+import glob
+import json
+import os
+import re
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+
+FIXTURES = [
+    # (output name, fixture dir, model file, backend)
+    ("ruo2_local_smart", "tests/export_test/reference_export", "ruo2_default.xml", "local_smart"),
+    ("ruo2_lat_int", "tests/export_test/reference_export_lat_int", "ruo2_default.xml", "lat_int"),
+    ("cli_local_smart", "tests/test_cli_generated_model", "cli_model.xml", "local_smart"),
+]
+
+VEC = re.compile(r"\(/\s*([^/]*)/\)")
+
+
+def site_names(model_file):
+    sys.path.insert(0, REPO)
+    from kmos_amd.model import load_project
+    pt = load_project(os.path.join(HERE, model_file))
+    return ["%s_%s" % ls for ls in pt.site_list()], pt
+
+
+def parse_coord(text, own_n, names):
+    """'site' / 'site + (/dx, dy, dz, X/)' -> [dx, dy, dz, n]."""
+    m = VEC.search(text)
+    if not m:
+        return [0, 0, 0, own_n]
+    parts = [p.strip() for p in m.group(1).split(",")]
+    dx, dy, dz = (int(p) for p in parts[:3])
+    last = parts[3]
+    if last == "0":
+        n = own_n
+    elif " - " in last:
+        n = names.index(last.split(" - ")[0].strip()) + 1
+    elif last in names:
+        n = names.index(last) + 1
+    else:
+        n = int(last)
+    return [dx, dy, dz, n]
+
+
+def subroutines(text):
+    out = {}
+    for m in re.finditer(r"^subroutine (\w+)\((.*?)\)\s*$(.*?)^end subroutine \1", text, re.M | re.S):
+        out[m.group(1)] = m.group(3)
+    return out
+
+
+def parse_local_smart(fdir, names):
+    text = open(os.path.join(fdir, "proclist.f90")).read()
+    subs = subroutines(text)
+    routines = {}
+    for name, body in subs.items():
+        if not (name.startswith("put_") or name.startswith("take_") or name.startswith("touchup_")):
+            continue
+        own = None
+        for i, nm in enumerate(names):
+            if name.endswith("_" + nm):
+                own = i + 1
+        ops = []
+        path = []            # stack of [coord, species or None]
+        pending_coord = []
+        for line in body.splitlines():
+            s = line.strip()
+            if not s or s.startswith("!"):
+                continue
+            m = re.match(r"call del_proc\((\w+),\s*(.*)\)$", s)
+            if m:
+                ops.append(["del", m.group(1), parse_coord(m.group(2), own, names)])
+                continue
+            m = re.match(r"select case\(get_species\((.*)\)\)$", s)
+            if m:
+                pending_coord.append(parse_coord(m.group(1), own, names))
+                path.append(None)
+                continue
+            m = re.match(r"case\((\w+)\)$", s)
+            if m and pending_coord:
+                path[-1] = pending_coord[-1] + [m.group(1)]
+                continue
+            if s == "end select":
+                pending_coord.pop()
+                path.pop()
+                continue
+            m = re.match(r"call add_proc\((\w+),\s*(.*)\)$", s)
+            if m:
+                conds = [list(p) for p in path if p is not None]
+                ops.append(["add", m.group(1), parse_coord(m.group(2), own, names), conds])
+        routines[name] = ops
+    run_proc = {}
+    body = subs["run_proc_nr"]
+    cur = None
+    for line in body.splitlines():
+        s = line.strip()
+        m = re.match(r"case\((\w+)\)$", s)
+        if m:
+            cur = m.group(1)
+            run_proc[cur] = []
+            continue
+        m = re.match(r"call ((?:put|take)_\w+)\((.*)\)$", s)
+        if m and cur:
+            rname = m.group(1)
+            own = [i + 1 for i, nm in enumerate(names) if rname.endswith("_" + nm)][0]
+            run_proc[cur].append([rname, parse_coord(m.group(2), own, names)])
+    return {"routines": routines, "run_proc": run_proc}
+
+
+def parse_lat_int(fdir, names):
+    groups = {}
+    for path in sorted(glob.glob(os.path.join(fdir, "run_proc_*.f90"))):
+        text = open(path).read()
+        m = re.search(r"subroutine run_proc_(\w+)\(cell\)", text)
+        gname = m.group(1)
+        dels, adds, updates = [], [], []
+        for line in text.splitlines():
+            s = line.strip()
+            m = re.match(r"call (del_proc|add_proc)\(nli_(\w+)\(cell \+ \(/([^/]*)/\)\), cell", s)
+            if m:
+                off = [int(v) for v in m.group(3).split(",")[:3]]
+                (dels if m.group(1) == "del_proc" else adds).append([m.group(2), off])
+                continue
+            m = re.match(r"call replace_species\(cell(.*?), (\w+), (\w+)\)$", s)
+            if m:
+                updates.append([parse_coord(m.group(1), 1, names), [m.group(2)], m.group(3)])
+        groups[gname] = {"dels": dels, "updates": updates, "adds": adds}
+    nli = {}
+    for path in sorted(glob.glob(os.path.join(fdir, "nli_*.f90"))):
+        text = open(path).read()
+        m = re.search(r"function nli_(\w+)\(cell\)", text)
+        gname = m.group(1)
+        leaves = []
+        stack = []
+        pending = []
+        for line in text.splitlines():
+            s = line.strip()
+            m = re.match(r"select case\(get_species\(cell(.*)\)\)$", s)
+            if m:
+                pending.append(parse_coord(m.group(1), 1, names))
+                stack.append(None)
+                continue
+            m = re.match(r"case\((.*)\)$", s)
+            if m and pending:
+                stack[-1] = pending[-1] + [[x.strip() for x in m.group(1).split(",")]]
+                continue
+            if s == "case default" and pending:
+                stack[-1] = pending[-1] + [["default"]]
+                continue
+            if s == "end select":
+                pending.pop()
+                stack.pop()
+                continue
+            m = re.match(r"nli_\w+ = (\w+); return$", s)
+            if m:
+                leaves.append([[list(p) for p in stack if p is not None], m.group(1)])
+        nli[gname] = leaves
+    text = open(os.path.join(fdir, "proclist.f90")).read()
+    body = subroutines(text)["run_proc_nr"]
+    dispatch = {}
+    body = body.replace("&\n", " ")
+    for m in re.finditer(r"case\(([^)]*)\)\s*\n\s*call run_proc_(\w+)\(cell\)", body):
+        for p in m.group(1).replace("&", " ").split(","):
+            dispatch[p.strip()] = m.group(2)
+    tu = subroutines(text)["touchup_cell"]
+    touch_groups = re.findall(r"call add_proc\(nli_(\w+)\(cell\), site\)", tu)
+    return {"groups": groups, "nli": nli, "dispatch": dispatch, "touchup_groups": touch_groups}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ref", default="/root/reference")
+    a = ap.parse_args()
+    for out, fdir, model, backend in FIXTURES:
+        names, pt = site_names(model)
+        src = os.path.join(a.ref, fdir)
+        data = parse_local_smart(src, names) if backend == "local_smart" else parse_lat_int(src, names)
+        data["model"] = model
+        data["backend"] = backend
+        data["site_names"] = names
+        data["source"] = fdir
+        path = os.path.join(HERE, "ref_ops_%s.json" % out)
+        with open(path, "w") as fh:
+            json.dump(data, fh, indent=0, sort_keys=True)
+        print(path)
+
+
+if __name__ == "__main__":
+    main()

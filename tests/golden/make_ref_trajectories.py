@@ -1,0 +1,58 @@
+#!/usr/bin/env python3
+"""Write golden trajectories of the reference Fortran engines.
+
+Runs the reference builds of oracle/_ref (oracle/build_ref.py: the
+reference's own generated Fortran, compiled with the injected Philox
+uniform stream) and stores, per case, the per-step (process, site,
+kmc_time) and the final state (lattice, nr_of_sites, procstat, integ_rates,
+both avail_sites tables) as compressed numpy arrays under tests/golden/.
+
+Usage: python tests/golden/make_ref_trajectories.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+sys.path.insert(0, os.path.join(REPO, "tests"))
+
+from refbin import run_reference  # noqa: E402
+
+# (name, reference build, size, seed, stream, steps, rate seed, random config)
+CASES = [
+    ("ruo2_local_smart_10x10", "ruo2_local_smart", (10, 10), 3, 0, 5000, 11, False),
+    ("ruo2_local_smart_8x6_config", "ruo2_local_smart", (8, 6), 4, 9, 5000, 12, True),
+    ("ruo2_lat_int_10x10", "ruo2_lat_int", (10, 10), 5, 1, 5000, 13, False),
+    ("ruo2_lat_int_8x6_config", "ruo2_lat_int", (8, 6), 6, 2, 5000, 14, True),
+]
+P = 36
+
+
+def case_inputs(size, rate_seed, use_config):
+    g = np.random.default_rng(rate_seed)
+    rates = 10.0 ** g.uniform(-2.0, 9.0, P)
+    config = None
+    if use_config:
+        config = g.choice([0, 1, 2], size=size[0] * size[1] * 2, p=[0.3, 0.4, 0.3]).astype(np.int8)
+    return rates, config
+
+
+def main():
+    for name, build, size, seed, stream, steps, rseed, use_cfg in CASES:
+        rates, config = case_inputs(size, rseed, use_cfg)
+        res = run_reference(build, size, seed, stream, steps, rates, P, config=config)
+        out = os.path.join(HERE, "ref_traj_%s.npz" % name)
+        np.savez_compressed(
+            out, size=np.array(size), seed=seed, stream=stream, steps=steps, rates=rates,
+            config=config if config is not None else np.zeros(0, np.int8),
+            proc=res["proc"], site=res["site"], time=res["time"], lattice=res["lattice"],
+            nr=res["nr"], procstat=res["procstat"], integ=res["integ"],
+            sites=res["sites"], back=res["back"], kmc_step=res["kmc_step"])
+        print(out, os.path.getsize(out))
+
+
+if __name__ == "__main__":
+    main()
