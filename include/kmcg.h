@@ -71,12 +71,14 @@ int kmcg_batch_create(const kmcg_model *m, int32_t n, const int32_t *sizes,
                       int32_t device, kmcg_batch **out);
 void kmcg_batch_free(kmcg_batch *b);
 
-/* proclist.init / initialize_state for replicas [r0, r0+n): default
- * species, empty bookkeeping, touchups in k,j,i order; clocks, procstat and
- * integ_rates reset; uniform stream keyed (seeds[i], streams[i])
+/* proclist.init / initialize_state for replicas [r0, r0+n): the default
+ * species of every site (or, if configs is non-NULL, the given lattices,
+ * one after the other, V values each in site order), empty bookkeeping and
+ * the touchups in k,j,i order, clocks, procstat and integ_rates reset, and
+ * the uniform stream keyed (seeds[i], streams[i]) (streams may be NULL)
  * (proclist_generic_subroutines.mpy:166-304). */
 int kmcg_init(kmcg_batch *b, int32_t r0, int32_t n, const uint32_t *seeds,
-              const uint32_t *streams);
+              const uint32_t *streams, const int8_t *configs);
 
 /* base.set_rate_const for every process of replicas [r0, r0+n); rates is
  * [n][P], finite and >= 0 (base.mpy:581-600). */
@@ -125,6 +127,9 @@ int kmcg_set_kmc_time(kmcg_batch *b, int32_t r, double t);
 int kmcg_get_nr_of_sites(kmcg_batch *b, int32_t r, int32_t *out);
 int kmcg_get_procstat(kmcg_batch *b, int32_t r, int64_t *out);
 int kmcg_get_integ_rates(kmcg_batch *b, int32_t r, double *out);
+/* update_integ_rate: integ_rates += nr_of_sites * rates * kmc_time_step
+ * (base.mpy:626-645) */
+int kmcg_update_integ_rate(kmcg_batch *b, int32_t r);
 /* update_accum_rate then accum_rates (base.mpy:603-623) */
 int kmcg_get_accum_rates(kmcg_batch *b, int32_t r, double *out);
 int kmcg_get_lattice(kmcg_batch *b, int32_t r, int8_t *out);

@@ -25,7 +25,7 @@ SIGNATURES = [
     ("kmcg_batch_footprint", _i32, [_vp, _i32, _vp, _vp]),
     ("kmcg_batch_create", _i32, [_vp, _i32, _vp, _i32, _vp]),
     ("kmcg_batch_free", None, [_vp]),
-    ("kmcg_init", _i32, [_vp, _i32, _i32, _vp, _vp]),
+    ("kmcg_init", _i32, [_vp, _i32, _i32, _vp, _vp, _vp]),
     ("kmcg_set_rates", _i32, [_vp, _i32, _i32, _vp]),
     ("kmcg_get_rates", _i32, [_vp, _i32, _vp]),
     ("kmcg_set_config", _i32, [_vp, _i32, _i32, _vp]),
@@ -43,6 +43,7 @@ SIGNATURES = [
     ("kmcg_get_nr_of_sites", _i32, [_vp, _i32, _vp]),
     ("kmcg_get_procstat", _i32, [_vp, _i32, _vp]),
     ("kmcg_get_integ_rates", _i32, [_vp, _i32, _vp]),
+    ("kmcg_update_integ_rate", _i32, [_vp, _i32]),
     ("kmcg_get_accum_rates", _i32, [_vp, _i32, _vp]),
     ("kmcg_get_lattice", _i32, [_vp, _i32, _vp]),
     ("kmcg_get_avail", _i32, [_vp, _i32, _vp, _vp]),

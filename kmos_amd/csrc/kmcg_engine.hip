@@ -629,6 +629,111 @@ __global__ void kmc_fill_default_kernel(KParams kp)
     }
 }
 
+// nli_<group>(cell) on the program in HBM (initialisation kernel).
+template <int BITS>
+__device__ int nli_eval_g(const int *prog, int off_nli, int group, const Lattice<BITS> &lat,
+                          const Dims &d, int cx, int cy, int cz)
+{
+    int pos = prog[off_nli + group];
+    for (;;) {
+        const int *nd = prog + pos;
+        const int nc = nd[1];
+        if (nc == -1)
+            return nd[2];
+        const int c = nd[0];
+        const int sp = lat.get(site0(d, cx, cy, cz, unpack_x(c), unpack_y(c), unpack_z(c), (c >> 24) & 0xFF));
+        int next = -1;
+        for (int k = 0; k < nc; ++k) {
+            const int *e = nd + 2 + 3 * k;
+            if (sp >= 0 && ((e[0] >> sp) & 1)) {
+                if (e[1] == 1)
+                    return e[2];
+                next = e[2];
+                break;
+            }
+        }
+        if (next < 0)
+            return 0;
+        pos = next;
+    }
+}
+
+// initialize_state on an empty database (proclist_generic_subroutines.mpy:
+// 236-304).  The touchups, run in k, j, i order, append every (process,
+// site) pair at most once and in increasing site order, so the list of each
+// process is the ordered set of sites where it is enabled.  One workgroup
+// builds one process's list of one replica by a block-wide compaction.
+// init entry per process (3 words at kp.arg0): site type of its anchor, the
+// word offset of its touchup addition op (local_smart, -1: never enabled)
+// or its group (lat_int), the op's condition base.
+template <int BITS>
+__global__ __launch_bounds__(256) void kmc_init_lists_kernel(KParams kp)
+{
+    __shared__ int wave_tot[4];
+    const int P = kp.P;
+    const int slot = blockIdx.x / P, p = blockIdx.x % P;
+    const int r = kp.rep_ids[slot];
+    const int4 dd = kp.dims[r];
+    Dims d;
+    d.X = dd.x; d.Y = dd.y; d.Z = dd.z; d.V = dd.w; d.spuck = kp.spuck; d.dim = kp.dim;
+    Lattice<BITS> lat;
+    lat.p = kp.lat_pool + kp.lat_off[r];
+    const int *prog = kp.prog;
+    const int *ie = prog + kp.arg0 + 3 * p;
+    const int n_exec = ie[0], arg = ie[1], cbase = ie[2];
+    const bool lat_int = kp.backend == 1;
+    if (!lat_int && arg < 0)
+        return;
+    const long long V1 = (long long)dd.w + 1;
+    int *sp = kp.avail_pool + kp.avail_off[r] + (long long)p * V1;
+    int *bp = kp.avail_pool + kp.avail_off[r] + (long long)(P + p) * V1;
+    const int ncell = dd.x * dd.y * dd.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int w0 = 0, w1 = 0;
+    if (!lat_int) {
+        w0 = prog[arg];
+        w1 = prog[arg + 1];
+    }
+    const int nc = (w0 >> 17) & 0x3F;
+    const int *cd = prog + cbase + (int)((unsigned)w1 >> 21);
+    int base = 0;
+    for (int t = 0; t < ncell; t += 256) {
+        const int c = t + threadIdx.x;
+        bool en = false;
+        if (c < ncell) {
+            const int x = c % dd.x, y = (c / dd.x) % dd.y, z = c / (dd.x * dd.y);
+            if (lat_int) {
+                en = nli_eval_g<BITS>(prog, kp.off_nli, arg, lat, d, x, y, z) == p + 1;
+            } else {
+                en = true;
+                for (int k = 0; k < nc && en; ++k) {
+                    const int cw = cd[k];
+                    en = lat.get(site0(d, x, y, z, (cw & 63) - 32, ((cw >> 6) & 63) - 32,
+                                       ((cw >> 12) & 63) - 32, (cw >> 18) & 63)) == ((cw >> 24) & 0xFF);
+                }
+            }
+        }
+        const unsigned long long mask = __ballot(en);
+        if (lane == 0)
+            wave_tot[wave] = __popcll(mask);
+        __syncthreads();
+        int before = base;
+        for (int w = 0; w < wave; ++w)
+            before += wave_tot[w];
+        const int total = wave_tot[0] + wave_tot[1] + wave_tot[2] + wave_tot[3];
+        if (en) {
+            const int pos = before + __popcll(mask & ((1ull << lane) - 1ull)) + 1;
+            const int site = kp.spuck * c + n_exec;
+            sp[pos] = site;
+            bp[site] = pos;
+        }
+        base += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        kp.nr[(long long)r * P + p] = base;
+}
+
 // get_occupation (proclist_generic_subroutines.mpy:113-158): counts per
 // (species, site type); out[slot][S][spuck] as fractions of the cell count.
 template <int BITS>
@@ -695,7 +800,7 @@ struct kmcg_model {
     std::vector<int> blob;
     std::vector<int> prog;   // packed op program (layout in kmcg_device.h)
     int P = 0, S = 0, spuck = 0, dim = 0, backend = 0, nullsp = -1, maxoff = 0, maxR = 1;
-    int off_seg = 0, off_proc = 0, off_touch = 0, off_nli = 0;
+    int off_seg = 0, off_proc = 0, off_touch = 0, off_nli = 0, off_init = 0;
 };
 
 struct kmcg_batch {
@@ -1057,6 +1162,26 @@ int build_prog(kmcg_model *m)
                 if (n < 1 || n > m->spuck || nw < 0 || nw >= m->S)
                     return fail(KMCG_EINVAL, "bad lattice write");
             }
+        // per process: anchor site type, its touchup addition op (local_smart)
+        // or its group (lat_int), for the parallel initialisation
+        m->off_init = (int)pg.size();
+        for (int p = 1; p <= m->P; ++p) {
+            if (m->backend == 1) {
+                pg.insert(pg.end(), {1, W(W(9) + p - 1), 0});
+                continue;
+            }
+            int n_exec = W(W(9) + 3 * (p - 1)), opw = -1, cb = 0;
+            if (n_exec < 1 || n_exec > m->spuck)
+                return fail(KMCG_EINVAL, "bad anchor site type");
+            const int seg = touchup[n_exec - 1];
+            const int sb = m->off_seg + SEG_W * seg;
+            for (size_t j = 0; j < segs[seg].ops.size(); ++j)
+                if (segs[seg].ops[j].kind == OP_ADD_IF && segs[seg].ops[j].arg == p) {
+                    opw = pg[sb + 3] + OP_W * (int)j;
+                    cb = pg[sb + 4];
+                }
+            pg.insert(pg.end(), {n_exec, opw, cb});
+        }
         if (pg.size() * sizeof(int) > 64 * 1024)
             return fail(KMCG_EUNSUPPORTED, "op program larger than 64 KiB");
     } catch (const std::exception &e) {
@@ -1408,7 +1533,25 @@ int kmcg_set_launch(kmcg_batch *b, int32_t lattice_in_lds, int32_t waves_per_gro
     return KMCG_OK;
 }
 
-int kmcg_init(kmcg_batch *b, int32_t r0, int32_t n, const uint32_t *seeds, const uint32_t *streams)
+static int pack_config(kmcg_batch *b, int r, const int8_t *config, std::vector<unsigned char> &packed)
+{
+    const int V = b->V[r];
+    for (int s = 0; s < V; ++s)
+        if (config[s] < 0 || config[s] >= b->m->S)
+            return fail(KMCG_EINVAL, "configuration holds an unknown species");
+    packed.assign(lattice_bytes(V, b->bits), 0);
+    if (b->bits == 8) {
+        for (int s = 0; s < V; ++s)
+            packed[s] = (unsigned char)config[s];
+    } else {
+        for (int s = 0; s < V; ++s)
+            packed[(s * 2) >> 3] |= (unsigned char)(((config[s] + 1) & 3) << ((s * 2) & 7));
+    }
+    return KMCG_OK;
+}
+
+int kmcg_init(kmcg_batch *b, int32_t r0, int32_t n, const uint32_t *seeds, const uint32_t *streams,
+              const int8_t *configs)
 {
     if (!b || r0 < 0 || n < 1 || r0 + n > b->n || !seeds)
         return fail(KMCG_EINVAL, "bad replica range");
@@ -1436,12 +1579,32 @@ int kmcg_init(kmcg_batch *b, int32_t r0, int32_t n, const uint32_t *seeds, const
         ids[i] = r0 + i;
     HIPCHK(hipMemcpy(b->rep_ids, ids.data(), n * sizeof(int), hipMemcpyHostToDevice));
     KParams kp = make_params(b);
-    if (b->bits == 2)
+    if (configs) {
+        long long off = 0;
+        for (int i = 0; i < n; ++i) {
+            std::vector<unsigned char> packed;
+            int rc = pack_config(b, r0 + i, configs + off, packed);
+            if (rc)
+                return rc;
+            HIPCHK(hipMemcpy(b->lat_pool + b->lat_off[r0 + i], packed.data(), packed.size(),
+                             hipMemcpyHostToDevice));
+            off += b->V[r0 + i];
+        }
+    } else if (b->bits == 2) {
         hipLaunchKernelGGL((kmc_fill_default_kernel<2>), dim3(n), dim3(256), 0, b->stream, kp);
-    else
+    } else {
         hipLaunchKernelGGL((kmc_fill_default_kernel<8>), dim3(n), dim3(256), 0, b->stream, kp);
+    }
     HIPCHK(hipGetLastError());
-    return run_misc(b, ids, MODE_INIT, 0, 0, 0, 0);
+    kp.arg0 = b->m->off_init;
+    const dim3 grid((unsigned)(n * P));
+    if (b->bits == 2)
+        hipLaunchKernelGGL((kmc_init_lists_kernel<2>), grid, dim3(256), 0, b->stream, kp);
+    else
+        hipLaunchKernelGGL((kmc_init_lists_kernel<8>), grid, dim3(256), 0, b->stream, kp);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return KMCG_OK;
 }
 
 int kmcg_set_rates(kmcg_batch *b, int32_t r0, int32_t n, const double *rates)
@@ -1471,22 +1634,14 @@ int kmcg_set_config(kmcg_batch *b, int32_t r0, int32_t n, const int8_t *config)
     std::vector<int> ids(n);
     for (int i = 0; i < n; ++i) {
         const int r = r0 + i;
-        const int V = b->V[r];
-        for (int s = 0; s < V; ++s)
-            if (config[off + s] < 0 || config[off + s] >= b->m->S)
-                return fail(KMCG_EINVAL, "configuration holds an unknown species");
-        std::vector<unsigned char> packed(lattice_bytes(V, b->bits), 0);
-        if (b->bits == 8) {
-            for (int s = 0; s < V; ++s)
-                packed[s] = (unsigned char)config[off + s];
-        } else {
-            for (int s = 0; s < V; ++s)
-                packed[(s * 2) >> 3] |= (unsigned char)(((config[off + s] + 1) & 3) << ((s * 2) & 7));
-        }
-        const int rc = copy_in(b, b->lat_pool + b->lat_off[r], packed.data(), packed.size());
+        std::vector<unsigned char> packed;
+        int rc = pack_config(b, r, config + off, packed);
         if (rc)
             return rc;
-        off += V;
+        rc = copy_in(b, b->lat_pool + b->lat_off[r], packed.data(), packed.size());
+        if (rc)
+            return rc;
+        off += b->V[r];
         ids[i] = r;
     }
     return run_misc(b, ids, MODE_ADJUST, 0, 0, 0, 0);
@@ -1687,6 +1842,23 @@ int kmcg_get_accum_rates(kmcg_batch *b, int32_t r, double *out)
         out[i] = a;
     }
     return KMCG_OK;
+}
+
+int kmcg_update_integ_rate(kmcg_batch *b, int32_t r)
+{
+    int rc = check_replica(b, r);
+    if (rc)
+        return rc;
+    const int P = b->m->P;
+    std::vector<int> nr(P);
+    std::vector<double> rt(P), ig(P);
+    double dt;
+    if ((rc = kmcg_get_nr_of_sites(b, r, nr.data())) || (rc = kmcg_get_rates(b, r, rt.data())) ||
+        (rc = kmcg_get_integ_rates(b, r, ig.data())) || (rc = copy_out(b, &dt, b->kdt + r, 1)))
+        return rc;
+    for (int i = 0; i < P; ++i)
+        ig[i] = ig[i] + (double)nr[i] * rt[i] * dt;
+    return copy_in(b, b->integ + (long long)r * P, ig.data(), (size_t)P);
 }
 
 int kmcg_get_lattice(kmcg_batch *b, int32_t r, int8_t *out)

@@ -87,11 +87,19 @@ class Batch:
         self.close()
 
     # ---- set-up -----------------------------------------------------------
-    def init(self, seeds, streams=None, r0: int = 0):
+    def init(self, seeds, streams=None, r0: int = 0, configs=None):
+        """initialize_state for replicas r0.. (one per seed), from the
+        default species or from the given lattices."""
         seeds = np.ascontiguousarray(np.atleast_1d(seeds), dtype=np.uint32)
         n = seeds.size
         st = None if streams is None else np.ascontiguousarray(np.atleast_1d(streams), dtype=np.uint32)
-        _lib.check(_lib.load().kmcg_init(self.h, r0, n, _ptr(seeds), _ptr(st)))
+        cf = None
+        if configs is not None:
+            if isinstance(configs, np.ndarray) and configs.ndim == 1:
+                configs = [configs]
+            cf = np.ascontiguousarray(np.concatenate([np.asarray(c).reshape(-1) for c in configs]), dtype=np.int8)
+            assert cf.size == int(self.volumes[r0:r0 + n].sum())
+        _lib.check(_lib.load().kmcg_init(self.h, r0, n, _ptr(seeds), _ptr(st), _ptr(cf)))
 
     def set_rates(self, rates, r0: int = 0):
         r = np.ascontiguousarray(np.atleast_2d(rates), dtype=np.float64)
@@ -183,6 +191,9 @@ class Batch:
         out = np.zeros(self.P, np.float64)
         _lib.check(_lib.load().kmcg_get_integ_rates(self.h, r, _ptr(out)))
         return out
+
+    def update_integ_rate(self, r: int):
+        _lib.check(_lib.load().kmcg_update_integ_rate(self.h, r))
 
     def accum_rates(self, r: int) -> np.ndarray:
         out = np.zeros(self.P, np.float64)

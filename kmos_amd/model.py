@@ -422,3 +422,98 @@ def evaluate_numeric_rate(expr: str, parameters: Dict[str, float]) -> float:
     env = dict(_SAFE_FUNCS)
     env.update({k: float(v) for k, v in parameters.items()})
     return float(eval(compile(tree, "<rate>", "eval"), {"__builtins__": {}}, env))
+
+
+# ---------------------------------------------------------------------------
+# XML writer (the layout kmos/types.py Project._get_xml_string produces)
+# ---------------------------------------------------------------------------
+
+def to_xml(pt: Project) -> str:
+    """Serialise a project as a kmos XML (version 0.2) document."""
+    root = ET.Element("kmc", version="(0, 2)")
+    ET.SubElement(root, "meta", author=pt.author, email=pt.email, debug="0",
+                  model_dimension=str(pt.model_dimension), model_name=pt.model_name)
+    sl = ET.SubElement(root, "species_list", default_species=pt.default_species)
+    for name in pt.species:
+        ET.SubElement(sl, "species", name=name, color="#ffffff",
+                      representation=pt.representations.get(name, ""))
+    pl = ET.SubElement(root, "parameter_list")
+    for name, par in pt.parameters.items():
+        ET.SubElement(pl, "parameter", name=name, value=str(par.get("value")),
+                      adjustable=str(bool(par.get("adjustable", False))),
+                      min=str(par.get("min", 0.0)), max=str(par.get("max", 0.0)),
+                      scale=str(par.get("scale", "linear")))
+    cell = " ".join(str(float(v)) for row in pt.cell for v in row)
+    lat = ET.SubElement(root, "lattice", cell_size=cell, default_layer=pt.default_layer,
+                        substrate_layer=pt.substrate_layer, representation="")
+    for layer in pt.layers:
+        le = ET.SubElement(lat, "layer", name=layer.name, color="#ffffff")
+        for s in layer.sites:
+            ET.SubElement(le, "site", type=s.name, pos=" ".join(str(float(v)) for v in s.pos),
+                          default_species=s.default_species)
+    prl = ET.SubElement(root, "process_list")
+    for p in pt.processes:
+        attrs = dict(name=p.name, rate_constant=p.rate_constant, enabled=str(bool(p.enabled)))
+        if p.tof_count:
+            attrs["tof_count"] = repr(p.tof_count)
+        pe = ET.SubElement(prl, "process", **attrs)
+        for tag, lst in (("condition", p.conditions), ("action", p.actions)):
+            for ca in lst:
+                ET.SubElement(pe, tag, species=ca.species, coord_layer=ca.coord.layer,
+                              coord_name=ca.coord.name,
+                              coord_offset=" ".join(str(int(v)) for v in ca.coord.offset))
+    ET.SubElement(root, "output_list")
+    ET.indent(root)
+    return '<?xml version="1.0" ?>\n' + ET.tostring(root, encoding="unicode") + "\n"
+
+
+# ---------------------------------------------------------------------------
+# rate expressions in kmos's vocabulary (kmos/__init__.py:64-140, units.py)
+# ---------------------------------------------------------------------------
+
+UNITS = {
+    "pi": 3.14159265358979323846, "c": 2.99792458e8, "h": 6.62606957e-34,
+    "hbar": 1.054571726e-34, "eV": 1.602176565e-19, "kboltzmann": 1.3806488e-23,
+    "umass": 1.660538921e-27, "angstrom": 1.0e-10, "bar": 1.0e5,
+}
+
+
+def evaluate_rate_expression(rate_expr: str, parameters: Dict[str, dict]) -> float:
+    """Rate constant of a kmos rate expression.
+
+    Supports numbers, the model's parameters (whose values may themselves
+    be expressions), kmos's unit constants, the alias
+    ``beta = 1/(kboltzmann*T)`` and exp/log/sqrt/sin/cos/pow.  Species
+    masses (``m_X``) and JANAF chemical potentials (``mu_X``) need ASE and
+    the JANAF tables, which this engine does not ship; such expressions
+    raise ValueError.
+    """
+    if not rate_expr or not str(rate_expr).strip():
+        return 0.0
+    expr = str(rate_expr).replace("beta", "(1/(kboltzmann*T))")
+    env: Dict[str, float] = dict(UNITS)
+    env.update({"exp": math.exp, "log": math.log, "sqrt": math.sqrt, "sin": math.sin,
+                "cos": math.cos, "pow": math.pow})
+    resolving = set()
+
+    def param_value(name):
+        if name in resolving:
+            raise ValueError("parameter %r is defined in terms of itself" % name)
+        resolving.add(name)
+        v = parameters[name]["value"] if isinstance(parameters[name], dict) else parameters[name]
+        out = float(v) if isinstance(v, (int, float)) else evaluate_rate_expression(str(v).split()[0], parameters)
+        resolving.discard(name)
+        return out
+
+    tree = ast.parse(expr.strip(), mode="eval")
+    for node in ast.walk(tree):
+        if isinstance(node, ast.Name) and node.id not in env:
+            if node.id in parameters:
+                env[node.id] = param_value(node.id)
+            elif node.id.startswith("m_") or node.id.startswith("mu_"):
+                raise ValueError("%s needs ASE/JANAF data, which is not available" % node.id)
+            else:
+                raise ValueError("rate expression %r: unknown name %r" % (rate_expr, node.id))
+        if isinstance(node, (ast.Attribute, ast.Subscript, ast.Lambda, ast.comprehension)):
+            raise ValueError("rate expression %r uses unsupported syntax" % rate_expr)
+    return float(eval(compile(tree, "<rate>", "eval"), {"__builtins__": {}}, env))

@@ -374,8 +374,9 @@ int orc_touchup(orc_t *o, int x, int y, int z, int n)
     return 0;
 }
 
-/* initialize_state (proclist_generic_subroutines.mpy:236-304) */
-int orc_init(orc_t *o, uint32_t seed, uint32_t stream)
+/* initialize_state (proclist_generic_subroutines.mpy:236-304); with a
+ * configuration, the lattice starts from it instead of the defaults. */
+int orc_init_config(orc_t *o, uint32_t seed, uint32_t stream, const int8_t *config)
 {
     const int32_t *sdef = o->blob + HDR(o, 8);
     o->seed = seed;
@@ -392,7 +393,7 @@ int orc_init(orc_t *o, uint32_t seed, uint32_t stream)
     memset(o->sites, 0, sizeof(int32_t) * (int64_t)o->P * (o->V + 1));
     memset(o->back, 0, sizeof(int32_t) * (int64_t)o->P * (o->V + 1));
     for (int s = 1; s <= o->V; ++s)
-        o->lat[s] = (int8_t)sdef[(s - 1) % o->spuck];
+        o->lat[s] = config ? config[s - 1] : (int8_t)sdef[(s - 1) % o->spuck];
     for (int k = 0; k < o->Z; ++k)
         for (int j = 0; j < o->Y; ++j)
             for (int i = 0; i < o->X; ++i) {
@@ -404,6 +405,11 @@ int orc_init(orc_t *o, uint32_t seed, uint32_t stream)
                 }
             }
     return o->status;
+}
+
+int orc_init(orc_t *o, uint32_t seed, uint32_t stream)
+{
+    return orc_init_config(o, seed, stream, NULL);
 }
 
 /* The lattice of KMC_Model._set_configuration followed by

@@ -42,6 +42,8 @@ def lib():
         L.orc_free.argtypes = [vp]
         L.orc_init.argtypes = [vp, u32, u32]
         L.orc_init.restype = i32
+        L.orc_init_config.argtypes = [vp, u32, u32, vp]
+        L.orc_init_config.restype = i32
         L.orc_set_config.argtypes = [vp, vp]
         L.orc_set_config.restype = i32
         L.orc_set_rates.argtypes = [vp, vp]
@@ -116,8 +118,12 @@ class Oracle:
             lib().orc_free(self.h)
             self.h = None
 
-    def init(self, seed, stream=0):
-        return lib().orc_init(self.h, seed & 0xFFFFFFFF, stream & 0xFFFFFFFF)
+    def init(self, seed, stream=0, config=None):
+        if config is None:
+            return lib().orc_init(self.h, seed & 0xFFFFFFFF, stream & 0xFFFFFFFF)
+        c = np.ascontiguousarray(config, dtype=np.int8)
+        assert c.size == self.V
+        return lib().orc_init_config(self.h, seed & 0xFFFFFFFF, stream & 0xFFFFFFFF, _ptr(c))
 
     def set_rates(self, rates):
         r = np.ascontiguousarray(rates, dtype=np.float64)

@@ -101,3 +101,40 @@ def test_budget_and_absorbing(engine, oracle):
         o = oracle_replica(oracle, cm, (8, 8), 5 + r, 0, rates[r])
         o.run(n)
         compare_state(b, r, o, where="budget r%d" % r)
+
+
+@pytest.mark.parametrize("factory,backend", [
+    (W.ruo2_project, "local_smart"), (W.ruo2_project, "lat_int"),
+    (W.pairwise_project, "lat_int"), (W.zgb_project, "local_smart"),
+    (W.quickstart_project, "local_smart"),
+])
+def test_init_from_configuration_matches_oracle(engine, oracle, factory, backend):
+    """The parallel initialisation (per-process compaction) builds the same
+    lists as the oracle's sequential touchups, from defaults and from given
+    lattices, on ragged sizes, then steps identically."""
+    pt = factory()
+    cm = compile_model(pt, backend)
+    print("init", cm.model_name if hasattr(cm, "model_name") else "", backend, flush=True)
+    rng = np.random.default_rng(17)
+    sizes = [(7, 5), (33, 9), (64, 64), (5, 5)]
+    P = cm.nr_of_proc
+    rates = 10.0 ** rng.uniform(-2, 6, (len(sizes), P))
+    m = engine.Model(cm)
+    b = engine.Batch(m, [list(s) + [1] for s in sizes])
+    seeds = np.arange(len(sizes), dtype=np.uint32) + 3
+    configs = [_random_config(pt, int(v), rng) for v in b.volumes]
+    for use in (False, True):
+        b.init(seeds, None, configs=configs if use else None)
+        b.set_rates(rates)
+        orcs = []
+        for i, s in enumerate(sizes):
+            o = oracle.Oracle(cm.to_blob(), s)
+            o.init(int(seeds[i]), 0, configs[i] if use else None)
+            o.set_rates(rates[i])
+            orcs.append(o)
+        for r, o in enumerate(orcs):
+            compare_state(b, r, o, where="init config=%s r%d" % (use, r))
+        b.run(500)
+        for r, o in enumerate(orcs):
+            o.run(500)
+            compare_state(b, r, o, where="config=%s r%d after 500" % (use, r))

@@ -182,6 +182,35 @@ def test_oracle_bookkeeping_invariants(oracle, factory, backend, size):
             break
 
 
+@pytest.mark.parametrize("factory,backend", [
+    (W.ruo2_project, "local_smart"), (W.ruo2_project, "lat_int"),
+    (W.pairwise_project, "lat_int"), (W.zgb_project, "local_smart"),
+])
+def test_init_from_configuration(oracle, factory, backend):
+    """initialize_state from a given lattice: every list holds exactly the
+    sites where its process is enabled (brute-force scan), in increasing
+    site order; with the default lattice it is proclist.init itself."""
+    pt = factory()
+    cm = compile_model(pt, backend)
+    size = (6, 5)
+    g = np.random.default_rng(3)
+    config = g.choice(sorted(pt.species_ids().values()), size=size[0] * size[1] * cm.spuck).astype(np.int8)
+    o = oracle.Oracle(cm.to_blob(), size)
+    assert o.init(5, 0, config) == 0
+    np.testing.assert_array_equal(o.lattice(), config)
+    reg = _check_bookkeeping(o, cm.nr_of_proc, o.V)
+    assert reg == _enabled_bruteforce(pt, cm, config, size)
+    sites, _ = o.avail()
+    for p, k in enumerate(o.nr_of_sites()):
+        assert np.all(np.diff(sites[p, :k]) > 0)
+    d = oracle.Oracle(cm.to_blob(), size)
+    d.init(5, 0)
+    o.init(5, 0, d.lattice())
+    for a, b in ((o.avail(), d.avail()),):
+        np.testing.assert_array_equal(a[0], b[0])
+        np.testing.assert_array_equal(a[1], b[1])
+
+
 def _interval_search_brute(arr, value):
     """Definition: the smallest index i with arr[i] > value (else the last),
     moved left over equal entries; 1-based."""
