@@ -96,7 +96,11 @@ int kmcg_set_config(kmcg_batch *b, int32_t r0, int32_t n, const int8_t *config);
  * launch; a replica stops early when its status becomes nonzero.  If
  * `budget` is non-NULL, replica replicas[i] runs min(steps, budget[i])
  * steps and budget[i] is decreased by the steps it executed.  Returns once
- * the work is queued on the batch's stream; kmcg_sync waits for it. */
+ * the work is queued on the batch's stream (after waiting for the batch's
+ * previous run, if any); kmcg_sync waits for it and only then writes the
+ * decreased budgets back, so `budget` must stay valid until kmcg_sync (or
+ * any other call on the batch, all of which wait for the run first).
+ * Batches have their own streams: runs of different batches overlap. */
 int kmcg_run(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps,
              int64_t *budget);
 int kmcg_sync(kmcg_batch *b);
@@ -139,6 +143,16 @@ int kmcg_get_avail(kmcg_batch *b, int32_t r, int32_t *sites, int32_t *back);
 int kmcg_get_occupation(kmcg_batch *b, int32_t r0, int32_t n, double *out);
 /* procstat and integ_rates of replicas [r0, r0+n): [n][P] each */
 int kmcg_get_counters(kmcg_batch *b, int32_t r0, int32_t n, int64_t *procstat, double *integ);
+
+/* One sampling record per listed replica, KMCG_SAMPLE_WIDTH(S, spuck, P)
+ * doubles each, in list order: kmc_time, kmc_time_step, kmc_step, status,
+ * occupation[S][spuck] (proclist.get_occupation,
+ * proclist_generic_subroutines.mpy:113-158), integ_rates[P], procstat[P]:
+ * what KMC_Model.get_atoms(geometry=False) reads (kmos/run/__init__.py:
+ * 680-850), gathered on the device and copied in one transfer.  Waits for
+ * the batch's stream. */
+#define KMCG_SAMPLE_WIDTH(S, spuck, P) (4 + (S) * (spuck) + 2 * (P))
+int kmcg_sample(kmcg_batch *b, const int32_t *replicas, int32_t n, double *out);
 
 #ifdef __cplusplus
 }

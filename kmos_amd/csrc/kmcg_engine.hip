@@ -759,6 +759,44 @@ __global__ void kmc_occupation_kernel(KParams kp, double *out)
         out[(long long)blockIdx.x * S * spuck + i] = (double)counts[i] / cells;
 }
 
+// One sampling record per listed replica (kmos/run/__init__.py:680-850
+// reads the same quantities through get_atoms): kmc_time, kmc_time_step,
+// kmc_step, status, occupation[S][spuck] (per cell), integ_rates[P],
+// procstat[P], all as doubles.  One workgroup per replica.
+template <int BITS>
+__global__ __launch_bounds__(256) void kmc_sample_kernel(KParams kp, double *out, int width)
+{
+    __shared__ int counts[8 * 32];
+    const int r = kp.rep_ids[blockIdx.x];
+    const int4 dd = kp.dims[r];
+    const int S = kp.S, spuck = kp.spuck, P = kp.P;
+    for (int i = threadIdx.x; i < S * spuck; i += blockDim.x)
+        counts[i] = 0;
+    __syncthreads();
+    Lattice<BITS> lat;
+    lat.p = kp.lat_pool + kp.lat_off[r];
+    for (int s = threadIdx.x; s < dd.w; s += blockDim.x) {
+        const int sp = lat.get(s);
+        if (sp >= 0 && sp < S)
+            atomicAdd(&counts[sp * spuck + s % spuck], 1);
+    }
+    __syncthreads();
+    double *rec = out + (long long)blockIdx.x * width;
+    const double cells = (double)dd.x * dd.y * dd.z;
+    for (int i = threadIdx.x; i < S * spuck; i += blockDim.x)
+        rec[4 + i] = (double)counts[i] / cells;
+    for (int p = threadIdx.x; p < P; p += blockDim.x) {
+        rec[4 + S * spuck + p] = kp.integ[(long long)r * P + p];
+        rec[4 + S * spuck + P + p] = (double)kp.pstat[(long long)r * P + p];
+    }
+    if (threadIdx.x == 0) {
+        rec[0] = kp.ktime[r];
+        rec[1] = kp.kdt[r];
+        rec[2] = (double)kp.kstep[r];
+        rec[3] = (double)kp.status[r];
+    }
+}
+
 } // namespace
 
 // ===========================================================================
@@ -835,6 +873,12 @@ struct kmcg_batch {
     int *rep_ids = nullptr;
     int *out = nullptr;
     double *occ = nullptr;
+    double *samples = nullptr;
+    // pinned staging of a run's replica list and budgets; one run in flight
+    int *h_ids = nullptr;
+    long long *h_budget = nullptr;
+    int64_t *user_budget = nullptr;    // where the budgets go back at the sync
+    int n_last = 0;
     float last_ms = 0.f;
     bool timed_run_pending = false;   // ev0/ev1 bracket a launch not yet read
 };
@@ -1303,10 +1347,14 @@ int check_replica(kmcg_batch *b, int r)
     return KMCG_OK;
 }
 
+int settle(kmcg_batch *b);
+
 // run the misc kernel for replicas [ids] synchronously
 int run_misc(kmcg_batch *b, const std::vector<int> &ids, int mode, int a0, int a1, int a2, int a3)
 {
-    HIPCHK(hipSetDevice(b->device));
+    int rc = settle(b);
+    if (rc)
+        return rc;
     HIPCHK(hipMemcpyAsync(b->rep_ids, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
     KParams kp = make_params(b);
     const Shape s = launch_shape(b, ids.data(), (int)ids.size(), kp, MAX_WPG);
@@ -1317,12 +1365,34 @@ int run_misc(kmcg_batch *b, const std::vector<int> &ids, int mode, int a0, int a
     return KMCG_OK;
 }
 
+// Wait for the batch's stream and finish the bookkeeping of a run in
+// flight (its device time, its budgets back to the caller).
+int settle(kmcg_batch *b)
+{
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (b->timed_run_pending) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        b->last_ms = ms;
+        b->timed_run_pending = false;
+    }
+    if (b->user_budget) {
+        memcpy(b->user_budget, b->h_budget, b->n_last * sizeof(long long));
+        b->user_budget = nullptr;
+    }
+    return KMCG_OK;
+}
+
 template <class T>
 int copy_out(kmcg_batch *b, T *dst, const T *src, size_t count)
 {
     HIPCHK(hipSetDevice(b->device));
+    int rc = settle(b);
+    if (rc)
+        return rc;
+    HIPCHK(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
-    HIPCHK(hipMemcpy(dst, src, count * sizeof(T), hipMemcpyDeviceToHost));
     return KMCG_OK;
 }
 
@@ -1330,8 +1400,11 @@ template <class T>
 int copy_in(kmcg_batch *b, T *dst, const T *src, size_t count)
 {
     HIPCHK(hipSetDevice(b->device));
+    int rc = settle(b);
+    if (rc)
+        return rc;
+    HIPCHK(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
-    HIPCHK(hipMemcpy(dst, src, count * sizeof(T), hipMemcpyHostToDevice));
     return KMCG_OK;
 }
 
@@ -1424,10 +1497,15 @@ void kmcg_batch_free(kmcg_batch *b)
         (void)hipStreamSynchronize(b->stream);
     void *ptrs[] = {b->prog, b->dims, b->lat_off_d, b->avail_off_d, b->seed, b->strm, b->lat_pool,
                     b->avail_pool, b->nr, b->rates, b->integ, b->pstat, b->ktime, b->kdt, b->kstep,
-                    b->rngctr, b->status, b->executed, b->budget, b->rep_ids, b->out, b->occ};
+                    b->rngctr, b->status, b->executed, b->budget, b->rep_ids, b->out, b->occ,
+                    b->samples};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
+    if (b->h_ids)
+        (void)hipHostFree(b->h_ids);
+    if (b->h_budget)
+        (void)hipHostFree(b->h_budget);
     if (b->ev0)
         (void)hipEventDestroy(b->ev0);
     if (b->ev1)
@@ -1508,17 +1586,24 @@ int kmcg_batch_create(const kmcg_model *m, int32_t n, const int32_t *sizes, int3
     BALLOC(b->rep_ids, n);
     BALLOC(b->out, 16);
     BALLOC(b->occ, (size_t)n * m->S * m->spuck);
+    BALLOC(b->samples, (size_t)n * (4 + m->S * m->spuck + 2 * P));
+    if (hipHostMalloc((void **)&b->h_ids, n * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_budget, n * sizeof(long long), hipHostMallocDefault) != hipSuccess)
+        return bail(fail(KMCG_ENOMEM, "pinned host allocation failed"));
 #undef BALLOC
-    auto H2D = [&](void *d, const void *h, size_t bytes) { return hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) == hipSuccess; };
+    auto H2D = [&](void *d, const void *h, size_t bytes) {
+        return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, b->stream) == hipSuccess;
+    };
     if (!H2D(b->prog, m->prog.data(), m->prog.size() * sizeof(int)) ||
         !H2D(b->dims, dims.data(), n * sizeof(int4)) ||
         !H2D(b->lat_off_d, b->lat_off.data(), n * sizeof(long long)) ||
         !H2D(b->avail_off_d, b->avail_off.data(), n * sizeof(long long)))
         return bail(fail(KMCG_EHIP, "copy to device failed"));
-    if (hipMemset(b->rates, 0, n * P * sizeof(double)) != hipSuccess ||
-        hipMemset(b->status, 0, n * sizeof(int)) != hipSuccess ||
-        hipMemset(b->avail_pool, 0, b->avail_total * sizeof(int)) != hipSuccess ||
-        hipMemset(b->lat_pool, 0, b->lat_total) != hipSuccess)
+    if (hipMemsetAsync(b->rates, 0, n * P * sizeof(double), b->stream) != hipSuccess ||
+        hipMemsetAsync(b->status, 0, n * sizeof(int), b->stream) != hipSuccess ||
+        hipMemsetAsync(b->avail_pool, 0, b->avail_total * sizeof(int), b->stream) != hipSuccess ||
+        hipMemsetAsync(b->lat_pool, 0, b->lat_total, b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess)
         return bail(fail(KMCG_EHIP, "memset failed"));
     *out = b;
     return KMCG_OK;
@@ -1555,39 +1640,43 @@ int kmcg_init(kmcg_batch *b, int32_t r0, int32_t n, const uint32_t *seeds, const
 {
     if (!b || r0 < 0 || n < 1 || r0 + n > b->n || !seeds)
         return fail(KMCG_EINVAL, "bad replica range");
-    HIPCHK(hipSetDevice(b->device));
-    HIPCHK(hipStreamSynchronize(b->stream));
+    int rc0 = settle(b);
+    if (rc0)
+        return rc0;
     const long long P = b->m->P;
     std::vector<unsigned> st(n, 0u);
     if (streams)
         st.assign(streams, streams + n);
-    HIPCHK(hipMemcpy(b->seed + r0, seeds, n * sizeof(unsigned), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(b->strm + r0, st.data(), n * sizeof(unsigned), hipMemcpyHostToDevice));
-    HIPCHK(hipMemset(b->nr + r0 * P, 0, n * P * sizeof(int)));
-    HIPCHK(hipMemset(b->integ + r0 * P, 0, n * P * sizeof(double)));
-    HIPCHK(hipMemset(b->pstat + r0 * P, 0, n * P * sizeof(long long)));
-    HIPCHK(hipMemset(b->ktime + r0, 0, n * sizeof(double)));
-    HIPCHK(hipMemset(b->kdt + r0, 0, n * sizeof(double)));
-    HIPCHK(hipMemset(b->kstep + r0, 0, n * sizeof(long long)));
-    HIPCHK(hipMemset(b->rngctr + r0, 0, n * sizeof(unsigned long long)));
-    HIPCHK(hipMemset(b->status + r0, 0, n * sizeof(int)));
+    HIPCHK(hipMemcpyAsync(b->seed + r0, seeds, n * sizeof(unsigned), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->strm + r0, st.data(), n * sizeof(unsigned), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemsetAsync(b->nr + r0 * P, 0, n * P * sizeof(int), b->stream));
+    HIPCHK(hipMemsetAsync(b->integ + r0 * P, 0, n * P * sizeof(double), b->stream));
+    HIPCHK(hipMemsetAsync(b->pstat + r0 * P, 0, n * P * sizeof(long long), b->stream));
+    HIPCHK(hipMemsetAsync(b->ktime + r0, 0, n * sizeof(double), b->stream));
+    HIPCHK(hipMemsetAsync(b->kdt + r0, 0, n * sizeof(double), b->stream));
+    HIPCHK(hipMemsetAsync(b->kstep + r0, 0, n * sizeof(long long), b->stream));
+    HIPCHK(hipMemsetAsync(b->rngctr + r0, 0, n * sizeof(unsigned long long), b->stream));
+    HIPCHK(hipMemsetAsync(b->status + r0, 0, n * sizeof(int), b->stream));
     const long long a0 = b->avail_off[r0];
     const long long a1 = (r0 + n < b->n) ? b->avail_off[r0 + n] : b->avail_total;
-    HIPCHK(hipMemset(b->avail_pool + a0, 0, (a1 - a0) * sizeof(int)));
+    HIPCHK(hipMemsetAsync(b->avail_pool + a0, 0, (a1 - a0) * sizeof(int), b->stream));
     std::vector<int> ids(n);
     for (int i = 0; i < n; ++i)
         ids[i] = r0 + i;
-    HIPCHK(hipMemcpy(b->rep_ids, ids.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(b->rep_ids, ids.data(), n * sizeof(int), hipMemcpyHostToDevice, b->stream));
     KParams kp = make_params(b);
+    std::vector<std::vector<unsigned char>> packed;   // alive until the stream is synchronised
     if (configs) {
         long long off = 0;
+        packed.resize(n);
         for (int i = 0; i < n; ++i) {
-            std::vector<unsigned char> packed;
-            int rc = pack_config(b, r0 + i, configs + off, packed);
-            if (rc)
+            int rc = pack_config(b, r0 + i, configs + off, packed[i]);
+            if (rc) {
+                (void)hipStreamSynchronize(b->stream);
                 return rc;
-            HIPCHK(hipMemcpy(b->lat_pool + b->lat_off[r0 + i], packed.data(), packed.size(),
-                             hipMemcpyHostToDevice));
+            }
+            HIPCHK(hipMemcpyAsync(b->lat_pool + b->lat_off[r0 + i], packed[i].data(), packed[i].size(),
+                                  hipMemcpyHostToDevice, b->stream));
             off += b->V[r0 + i];
         }
     } else if (b->bits == 2) {
@@ -1654,10 +1743,15 @@ int kmcg_run(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps, i
     for (int i = 0; i < n; ++i)
         if (replicas[i] < 0 || replicas[i] >= b->n)
             return fail(KMCG_EINVAL, "replica index out of range");
-    HIPCHK(hipSetDevice(b->device));
-    HIPCHK(hipMemcpyAsync(b->rep_ids, replicas, n * sizeof(int), hipMemcpyHostToDevice, b->stream));
-    if (budget)
-        HIPCHK(hipMemcpyAsync(b->budget, budget, n * sizeof(long long), hipMemcpyHostToDevice, b->stream));
+    int rc = settle(b);      // the staging buffers belong to one run at a time
+    if (rc)
+        return rc;
+    memcpy(b->h_ids, replicas, n * sizeof(int));
+    HIPCHK(hipMemcpyAsync(b->rep_ids, b->h_ids, n * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    if (budget) {
+        memcpy(b->h_budget, budget, n * sizeof(long long));
+        HIPCHK(hipMemcpyAsync(b->budget, b->h_budget, n * sizeof(long long), hipMemcpyHostToDevice, b->stream));
+    }
     KParams kp = make_params(b);
     const Shape s = launch_shape(b, replicas, n, kp, b->max_wpg);
     kp.nsteps = steps;
@@ -1667,8 +1761,9 @@ int kmcg_run(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps, i
     HIPCHK(hipEventRecord(b->ev1, b->stream));
     b->timed_run_pending = true;
     if (budget) {
-        HIPCHK(hipMemcpyAsync(budget, b->budget, n * sizeof(long long), hipMemcpyDeviceToHost, b->stream));
-        HIPCHK(hipStreamSynchronize(b->stream));
+        HIPCHK(hipMemcpyAsync(b->h_budget, b->budget, n * sizeof(long long), hipMemcpyDeviceToHost, b->stream));
+        b->user_budget = budget;
+        b->n_last = n;
     }
     return KMCG_OK;
 }
@@ -1677,15 +1772,7 @@ int kmcg_sync(kmcg_batch *b)
 {
     if (!b)
         return fail(KMCG_EINVAL, "null batch");
-    HIPCHK(hipSetDevice(b->device));
-    HIPCHK(hipStreamSynchronize(b->stream));
-    if (b->timed_run_pending) {
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-        b->last_ms = ms;
-        b->timed_run_pending = false;
-    }
-    return KMCG_OK;
+    return settle(b);
 }
 
 int kmcg_last_run_ms(kmcg_batch *b, float *ms)
@@ -1928,6 +2015,27 @@ int kmcg_get_counters(kmcg_batch *b, int32_t r0, int32_t n, int64_t *procstat, d
     if (integ && (rc = copy_out(b, integ, b->integ + r0 * P, (size_t)(n * P))))
         return rc;
     return KMCG_OK;
+}
+
+int kmcg_sample(kmcg_batch *b, const int32_t *replicas, int32_t n, double *out)
+{
+    if (!b || !replicas || n < 1 || n > b->n || !out)
+        return fail(KMCG_EINVAL, "bad sample arguments");
+    for (int i = 0; i < n; ++i)
+        if (replicas[i] < 0 || replicas[i] >= b->n)
+            return fail(KMCG_EINVAL, "replica index out of range");
+    if (b->m->S * b->m->spuck > 8 * 32)
+        return fail(KMCG_EUNSUPPORTED, "too many species x sites for occupation");
+    HIPCHK(hipSetDevice(b->device));
+    const int width = 4 + b->m->S * b->m->spuck + 2 * b->m->P;
+    HIPCHK(hipMemcpyAsync(b->rep_ids, replicas, n * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    KParams kp = make_params(b);
+    if (b->bits == 2)
+        hipLaunchKernelGGL((kmc_sample_kernel<2>), dim3(n), dim3(256), 0, b->stream, kp, b->samples, width);
+    else
+        hipLaunchKernelGGL((kmc_sample_kernel<8>), dim3(n), dim3(256), 0, b->stream, kp, b->samples, width);
+    HIPCHK(hipGetLastError());
+    return copy_out(b, out, b->samples, (size_t)n * width);
 }
 
 } // extern "C"

@@ -129,8 +129,9 @@ class Batch:
         """do_kmc_steps(steps) on the given replicas (all by default)."""
         ids = np.arange(self.n, dtype=np.int32) if replicas is None else \
             np.ascontiguousarray(replicas, dtype=np.int32)
-        bud = None if budget is None else np.ascontiguousarray(budget, dtype=np.int64)
+        bud = None if budget is None else np.array(budget, dtype=np.int64)
         self._ids = ids
+        self._bud = bud          # written back by the library at the next sync
         _lib.check(_lib.load().kmcg_run(self.h, _ptr(ids), ids.size, int(steps), _ptr(bud)))
         if sync:
             self.sync()
@@ -217,6 +218,25 @@ class Batch:
         out = np.zeros((n, self.S, self.spuck), np.float64)
         _lib.check(_lib.load().kmcg_get_occupation(self.h, r0, n, _ptr(out)))
         return out
+
+    def sample(self, replicas):
+        """One record per replica: dict of arrays kmc_time[n], kmc_time_step[n],
+        kmc_step[n], status[n], occupation[n, S, spuck], integ_rates[n, P],
+        procstat[n, P] (one device gather, one copy)."""
+        ids = np.ascontiguousarray(np.atleast_1d(replicas), dtype=np.int32)
+        S, N, P = self.S, self.spuck, self.P
+        w = 4 + S * N + 2 * P
+        out = np.zeros((ids.size, w), np.float64)
+        _lib.check(_lib.load().kmcg_sample(self.h, _ptr(ids), ids.size, _ptr(out)))
+        return {
+            "kmc_time": out[:, 0].copy(),
+            "kmc_time_step": out[:, 1].copy(),
+            "kmc_step": out[:, 2].astype(np.int64),
+            "status": out[:, 3].astype(np.int32),
+            "occupation": out[:, 4:4 + S * N].reshape(-1, S, N).copy(),
+            "integ_rates": out[:, 4 + S * N:4 + S * N + P].copy(),
+            "procstat": out[:, 4 + S * N + P:].astype(np.int64),
+        }
 
     def counters(self, r0: int = 0, n: Optional[int] = None):
         n = self.n - r0 if n is None else n

@@ -382,8 +382,11 @@ def parse_ini(path_or_text: str) -> Project:
 
 
 def load_project(path: str) -> Project:
-    """Read a kmos ``.xml`` or ``.ini`` project file."""
-    if path.endswith(".ini"):
+    """Read a kmos ``.xml`` or ``.ini`` project file, or kmos project XML
+    text (what ``str(kmos.types.Project)`` returns)."""
+    if path.lstrip().startswith("<"):
+        pt = parse_xml(path)
+    elif path.endswith(".ini"):
         pt = parse_ini(path)
     elif path.endswith(".xml"):
         pt = parse_xml(path)

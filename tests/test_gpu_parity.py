@@ -138,3 +138,37 @@ def test_init_from_configuration_matches_oracle(engine, oracle, factory, backend
         for r, o in enumerate(orcs):
             o.run(500)
             compare_state(b, r, o, where="config=%s r%d after 500" % (use, r))
+
+
+def test_sample_records_and_async_budgets(engine, oracle):
+    """kmcg_sample gathers what the per-replica getters return; runs on two
+    batches overlap and the budgets come back at the sync."""
+    pt = W.ruo2_project()
+    cm = compile_model(pt, "local_smart")
+    rng = np.random.default_rng(5)
+    m = engine.Model(cm)
+    sizes = [(20, 20), (9, 7), (33, 16)]
+    b1 = engine.Batch(m, [list(s) + [1] for s in sizes])
+    b2 = engine.Batch(m, [list(s) + [1] for s in sizes])
+    rates = 10.0 ** rng.uniform(-2, 6, (len(sizes), cm.nr_of_proc))
+    for b in (b1, b2):
+        b.init(np.arange(3, dtype=np.uint32) + 1)
+        b.set_rates(rates)
+    bud1 = b1.run(700, budget=[100, 700, 1000], sync=False)
+    bud2 = b2.run(700, budget=[100, 700, 1000], sync=False)
+    b1.sync()
+    b2.sync()
+    np.testing.assert_array_equal(bud1, [0, 0, 300])
+    np.testing.assert_array_equal(bud2, [0, 0, 300])
+    rec = b1.sample([2, 0])
+    for i, r in enumerate((2, 0)):
+        t, dt, ks, _, st = b1.scalars(r)
+        assert rec["kmc_time"][i] == t and rec["kmc_time_step"][i] == dt
+        assert rec["kmc_step"][i] == ks and rec["status"][i] == st
+        np.testing.assert_array_equal(rec["procstat"][i], b1.procstat(r))
+        np.testing.assert_array_equal(rec["integ_rates"][i], b1.integ_rates(r))
+        np.testing.assert_array_equal(rec["occupation"][i], b1.occupation(r, 1)[0])
+    for r in range(3):
+        o = oracle_replica(oracle, cm, sizes[r], r + 1, 0, rates[r])
+        o.run([100, 700, 700][r])
+        compare_state(b2, r, o, where="async batch r%d" % r)

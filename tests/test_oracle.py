@@ -264,3 +264,16 @@ def test_absorbing_state_stops_replica(oracle):
     o = oracle_replica(oracle, cm, (4, 4), 0, 0, np.zeros(cm.nr_of_proc))
     assert o.run(10) == 0
     assert o.status == 1
+
+
+def test_poisson_stream_is_reproducible():
+    from kmos_amd import serve
+    a = serve.poisson_stream(rate=100.0, n=20, seed=9)
+    b = serve.poisson_stream(rate=100.0, n=20, seed=9)
+    assert [q.seed for q in a] == [q.seed for q in b]
+    assert all(x.arrival < y.arrival for x, y in zip(a, a[1:]))
+    assert all(1e4 <= q.steps <= 1e6 for q in a)
+    assert {q.model for q in a} <= {"quickstart", "zgb", "ruo2"}
+    for q in a:
+        if q.config is not None:
+            assert q.config.size == q.size[0] * q.size[1] * 2
