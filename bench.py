@@ -12,7 +12,15 @@ replica per launch) on the same model, their ratio, the HBM roofline of
 the step kernel from the algorithmic bytes per kMC step (int32 avail_sites
 entries read or written, counted by the instrumented oracle), and a CPU
 baseline (the C oracle on the host cores, one process per core, on a
-bounded sample of the same workload).
+bounded sample of the same workload).  Two more legs ride along:
+
+  ruo2     the batched RuO2 workload (2048 replicas, 20^2:64^2:128^2 =
+           1:2:1, 2e5 steps) on the GPU, next to the reference's own
+           generated Fortran (built by oracle/build_ref.py with the injected
+           uniform stream) run one replica per host core
+  serving  a Poisson stream of requests drawn from the quickstart / ZGB /
+           RuO2 workloads through the continuous-batching server: p50/p99
+           request latency and aggregate steps/s
 """
 import argparse
 import json
@@ -31,6 +39,19 @@ from kmos_amd.compiler import compile_model  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
 METRIC = "aggregate kMC steps/s, batched vs plain GPU; p50/p99 request latency"
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch of the step kernel from the rocprofv3 PMC summary
+    committed for this workload (FETCH_SIZE + WRITE_SIZE, calibrated), or
+    None."""
+    path = os.path.join(REPO, "profiles", "kmc_steps_kernel_%s_pmc.md" % workload)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
 
 
 def _oracle_worker(args):
@@ -90,6 +111,110 @@ def cpu_baseline(wl, cm, seconds=15.0):
                       % (nrep, wl.steps, wl.name, wall)}
 
 
+def _ref_worker(args):
+    exe, size, seed, stream, rates, config, steps = args
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        inp = os.path.join(td, "in.txt")
+        with open(inp, "w") as fh:
+            fh.write("%d %d 1\n%d %d\n%d 1\n%d\n" % (size[0], size[1], seed, stream, steps,
+                                                       0 if config is None else 1))
+            fh.write(" ".join(repr(float(r)) for r in rates) + "\n")
+            if config is not None:
+                fh.write(" ".join(str(int(c)) for c in config) + "\n")
+        out = subprocess.run([exe, inp, os.path.join(td, "out.bin")], capture_output=True, text=True,
+                             check=True).stdout
+    secs = float(out.split("do_kmc_steps_seconds")[1].split()[0])
+    return steps, secs
+
+
+def reference_cpu_baseline(wl, per_core=4):
+    """The reference's generated Fortran (oracle/_ref, built from the
+    reference's checked-in export), one process per host core, each running
+    whole replicas of the workload through the reference's do_kmc_steps."""
+    exe = os.path.join(REPO, "oracle", "_ref", "ruo2_%s" % wl.backend, "ref_driver")
+    if not os.path.exists(exe):
+        return None
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    nrep = min(wl.n, cores * per_core)
+    # an even spread over the size mix
+    idx = np.linspace(0, wl.n - 1, nrep).astype(int)
+    jobs = [(exe, tuple(int(v) for v in wl.sizes[r][:2]), int(wl.seeds[r]), int(wl.streams[r]),
+             wl.rates[r], None if wl.configs is None else wl.configs[r], wl.steps) for r in idx]
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        res = pool.map(_ref_worker, jobs, chunksize=1)
+    wall = time.perf_counter() - t0
+    steps = sum(r[0] for r in res)
+    loop = sum(r[1] for r in res)
+    # all cores ran concurrently; the rate counts the reference's step loop
+    # only (do_kmc_steps wall time), not process start-up or set-up
+    return {"value": cores * steps / loop, "unit": "kMC steps/s", "cores": cores, "kind": "reference",
+            "sample": "%d replicas (sizes spread over the mix) x %d steps of %s through the reference "
+                      "Fortran's do_kmc_steps (flang -O3, injected uniforms), one process per core "
+                      "running concurrently; value = cores x steps / summed loop seconds; %.1f s wall"
+                      % (nrep, wl.steps, wl.name, wall)}
+
+
+def run_batched(engine, model, wl, K, warmup):
+    """Time K launches that together run the whole workload; returns the
+    batch, wall seconds, executed steps and per-launch device ms."""
+    chunk = max(1, wl.steps // K)
+    if warmup > 0:
+        wb = make_batch(engine, model, wl, np.arange(min(wl.n, 256)))
+        for _ in range(warmup):
+            wb.run(min(chunk, 2000))
+        wb.close()
+    b = make_batch(engine, model, wl)
+    kernel_ms = []
+    done_before = b.scalars_all()[1].sum()
+    b.sync()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        b.run(chunk, sync=False)
+        kernel_ms.append(b.last_run_ms())   # waits for the launch, reads its HIP events
+    b.sync()
+    wall = time.perf_counter() - t0
+    executed = int(b.scalars_all()[1].sum() - done_before)
+    return b, wall, executed, kernel_ms, chunk
+
+
+def ruo2_leg(engine, K, no_cpu):
+    wl = W.ruo2()
+    cm = compile_model(wl.project, wl.backend)
+    model = engine.Model(cm)
+    b, wall, executed, kernel_ms, chunk = run_batched(engine, model, wl, K, 1)
+    st = b.scalars_all()[2]
+    b.close()
+    out = {"workload": wl.name, "replicas": wl.n, "steps_per_replica": wl.steps,
+           "gpu_steps_per_s": executed / wall, "avg_launch_ms": float(np.mean(kernel_ms)),
+           "replica_status_counts": {str(k): int(v) for k, v in zip(*np.unique(st, return_counts=True))}}
+    if not no_cpu:
+        ref = reference_cpu_baseline(wl)
+        if ref is not None:
+            out["cpu_baseline"] = ref
+            out["gpu_over_cpu_reference"] = out["gpu_steps_per_s"] / ref["value"]
+    return out
+
+
+def serving_leg(rate, n, seed=1):
+    from kmos_amd import serve
+    reqs = serve.poisson_stream(rate=rate, n=n, seed=seed)
+    srv = serve.Server(hbm_budget=200 << 30)
+    t0 = time.perf_counter()
+    res = srv.serve(reqs, realtime=True)
+    wall = time.perf_counter() - t0
+    out = serve.latency_summary(res, wall)
+    out.update({"arrival_rate_per_s": rate, "ticks": srv.stats["ticks"], "pages": srv.stats["pages"],
+                "peak_active": srv.stats["peak_active"], "peak_hbm_bytes": srv.stats["peak_bytes"],
+                "steps_range": [1e4, 1e6], "sample_every": 1000,
+                "sizes": list(serve.SIZES), "models": ["quickstart", "zgb", "ruo2"]})
+    srv.close()
+    return out
+
+
 def make_batch(engine, model, wl, idx=None):
     idx = np.arange(wl.n) if idx is None else np.asarray(idx)
     b = engine.Batch(model, [list(s) for s in wl.sizes[idx]])
@@ -111,6 +236,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=20000)
+    ap.add_argument("--no-ruo2", action="store_true", help="skip the RuO2 leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-rate", type=float, default=100.0, help="request arrivals per second")
+    ap.add_argument("--serve-requests", type=int, default=200)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: replicas are independent and the engine runs "
@@ -132,27 +261,8 @@ def main():
         return 1
     model = engine.Model(cm)
     K = max(1, a.steps)
-    chunk = max(1, wl.steps // K)
-
-    # warm-up on a separate small batch
-    if a.warmup > 0:
-        wb = make_batch(engine, model, wl, np.arange(min(wl.n, 256)))
-        for _ in range(a.warmup):
-            wb.run(min(chunk, 2000))
-        wb.close()
-
-    b = make_batch(engine, model, wl)
-    kernel_ms = []
-    done_before = b.scalars_all()[1].sum()
-    b.sync()
-    t0 = time.perf_counter()
-    for _ in range(K):
-        b.run(chunk, sync=False)
-        kernel_ms.append(b.last_run_ms())   # waits for the launch, reads its HIP events
-    b.sync()
-    wall = time.perf_counter() - t0
+    b, wall, executed, kernel_ms, chunk = run_batched(engine, model, wl, K, a.warmup)
     t_after, k_after, st_after = b.scalars_all()
-    executed = int(k_after.sum() - done_before)
     value = executed / wall
     avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
     steps_per_launch = executed / K
@@ -181,8 +291,10 @@ def main():
     plain = a.plain_steps / plain_wall
     pb.close()
 
+    b.close()
     bytes_per_step = algorithmic_bytes_per_step(wl, cm)
     achieved = steps_per_launch * bytes_per_step / avg_kernel_s / 1e9
+    traffic = measured_traffic(wl.name)
     out = {
         "metric": METRIC,
         "value": value,
@@ -202,13 +314,19 @@ def main():
         "replica_status_counts": {str(k): int(v) for k, v in zip(*np.unique(st_after, return_counts=True))},
         "parity_spot_check": spot,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_kmc_step": bytes_per_step,
                      "kernel": "kmc_steps_kernel", "avg_launch_ms": avg_kernel_s * 1e3},
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, cm, a.cpu_seconds)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    if not a.no_ruo2 and a.workload == "zgb":
+        out["ruo2"] = ruo2_leg(engine, K, a.no_cpu)
+    if not a.no_serve:
+        out["serving"] = serving_leg(a.serve_rate, a.serve_requests)
+        out["p50_latency_s"] = out["serving"]["p50_latency_s"]
+        out["p99_latency_s"] = out["serving"]["p99_latency_s"]
     print(json.dumps(out))
     return 0
 

@@ -13,11 +13,12 @@
 // Every generated routine of the reference (put/take/touchup for
 // local_smart, run_proc_<group>/touchup_cell for lat_int) is a "segment":
 // lattice writes followed by an ordered op list.  Ops are evaluated
-// lane-parallel (conditions, nli case trees) and posted through an LDS
-// mailbox to the lane owning their process, in rounds precomputed on the
-// host so that one process's ops keep the reference order.  The owner lane
-// replays its deletions of one segment exactly from a single batch of HBM
-// reads (see apply_segment in kmcg_engine.hip).
+// lane-parallel (conditions, nli case trees) and posted through LDS mailbox
+// rows to the lane owning their process, in rounds precomputed on the host
+// so that one process's ops keep the reference order.  The segments of one
+// executed process post to consecutive rows; the owner lane then replays
+// all its entries of the step from one batch of HBM reads per block of
+// rows (replay_block / apply_rows in kmcg_engine.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -48,7 +49,6 @@ enum : int { MB_DEL_IF = 1, MB_DEL_STRICT = 2, MB_ADD = 3 };
 
 enum : int { ST_OK = 0, ST_ABSORBING = 1, ST_MISMATCH = 2, ST_DEL_ABSENT = 3, ST_CORRUPT = 4 };
 
-constexpr int KMAX = 4;       // deletions of one process replayed per HBM round trip
 constexpr int MAX_WPG = 4;    // replica-waves per workgroup
 
 __host__ __device__ inline int pack3(int dx, int dy, int dz)
@@ -93,6 +93,8 @@ struct KParams {
     int wave_lds;                  // LDS bytes per wave (lattice + mailbox)
     int lat_lds;                   // LDS bytes of the lattice part (0: lattice in HBM)
     int P, S, spuck, dim, maxR, backend;
+    int mrows;                     // mailbox rows per wave
+    int merged;                    // 1: the segments of a step share one apply
     int off_seg, off_proc, off_touch, off_nli;
     int mode;                      // misc kernel mode
     int arg0, arg1, arg2, arg3;    // misc kernel arguments
@@ -219,6 +221,18 @@ struct Lattice {
         const unsigned char m = (unsigned char)(((1 << BITS) - 1) << (bit & 7));
         const unsigned char b = p[bit >> 3];
         p[bit >> 3] = (unsigned char)((b & ~m) | ((((unsigned)(v + 1)) << (bit & 7)) & m));
+    }
+    // old -> v at site s; safe against other lanes writing other sites of
+    // the same word at the same time (the buffer is 4-byte aligned)
+    __device__ __forceinline__ void xset(int s, int old, int v)
+    {
+        if (BITS == 8) {
+            p[s] = (unsigned char)(signed char)v;
+            return;
+        }
+        const int bit = s * BITS;
+        const unsigned x = (((unsigned)(old + 1)) ^ ((unsigned)(v + 1))) << (bit & 31);
+        atomicXor((unsigned *)p + (bit >> 5), x);
     }
 };
 

@@ -45,8 +45,8 @@ struct Wave {
     Dims d;
     Magic mx, my, ms;  // division by X, Y, spuck
     Lattice<BITS> lat;
-    int *mbox;         // [maxR][64*PPL]
-    int *rp;           // lane-private replay area in LDS: [3][KMAX] ints
+    int *mbox;         // queues [mrows][64*PPL]: entry j of process p at j*64*PPL + p
+    int *cnt;          // [64*PPL] queue lengths
     int *avail;        // sites[P][V+1] then back[P][V+1]
     long long V1;
     int P, lane, mstride;
@@ -100,119 +100,155 @@ __device__ int nli_eval(const Wave<PPL, BITS> &W, int group, int cx, int cy, int
     }
 }
 
-// Apply the ops posted to this lane's processes (rounds 0..nrounds-1).
-// Per process the posted ops are deletions followed by additions.  All
-// deletions of a process are replayed from ONE batch of HBM reads: the
-// back-index of every site to delete and the last KMAX list slots; the
-// swap-with-last of del_proc (base.mpy:211-265) is then done in registers,
-// tracking the slots and back-indices it changes, and written out in order.
+// ---------------------------------------------------------------------------
+// Replay of a block of queued entries of one process (add_proc/del_proc,
+// base.mpy:211-301; lat_int: base_lat_int.mpy:197-264).
+//
+// v[0..m) are the entries in reference order: deletions and additions may
+// interleave.  One batch of HBM reads serves the block: back[site] of every
+// site to delete and the last D list slots (D = number of deletions).  The
+// swap-with-last of del_proc is replayed in registers over a window of list
+// slots [nr0-RBLK+1, nr0+RBLK] (the only slots a later entry of the block
+// can read), tracking the slots and back-indices the block itself changes,
+// and every change is written through to HBM in program order.
+// ---------------------------------------------------------------------------
+constexpr int RBLK = 4;
+constexpr int WIN = 2 * RBLK;
+
+__device__ __forceinline__ int win_get(const int (&w)[WIN], int idx)
+{
+    int v = 0;
+#pragma unroll
+    for (int q = 0; q < WIN; ++q)
+        v = (idx == q) ? w[q] : v;
+    return v;
+}
+
+__device__ __forceinline__ void win_set(int (&w)[WIN], int idx, int val)
+{
+#pragma unroll
+    for (int q = 0; q < WIN; ++q)
+        w[q] = (idx == q) ? val : w[q];
+}
+
 template <int PPL, int BITS>
-__device__ int apply_segment(Wave<PPL, BITS> &W, int nrounds)
+__device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, int *sp, int *bp,
+                                             const int (&v)[RBLK], int m, int &err)
+{
+    const int nr0 = nr;
+    int bv[RBLK];
+    int nd = 0;
+#pragma unroll
+    for (int i = 0; i < RBLK; ++i) {
+        bv[i] = 0;
+        if (i < m && (v[i] & 3) != MB_ADD) {
+            bv[i] = bp[v[i] >> 2];
+            nd += 1;
+        }
+    }
+    int w[WIN];   // slot q <-> w[q - base]
+#pragma unroll
+    for (int j = 0; j < WIN; ++j)
+        w[j] = 0;
+#pragma unroll
+    for (int j = 0; j < RBLK; ++j)
+        if (j < nd && nr0 - j >= 1)
+            w[RBLK - 1 - j] = sp[nr0 - j];
+    int cur = nr0;
+    const int base = nr0 - (RBLK - 1);
+#pragma unroll
+    for (int i = 0; i < RBLK; ++i) {
+        if (!__ballot(i < m))
+            break;
+        if (i < m) {
+            const int e = v[i];
+            const int site = e >> 2;
+            if ((e & 3) == MB_ADD) {
+                if (cur + 1 > W.d.V) {      // only a double add can get here
+                    err |= 2;
+                } else {
+                    cur += 1;
+                    sp[cur] = site;
+                    bp[site] = cur;
+                    win_set(w, cur - base, site);
+#pragma unroll
+                    for (int j = i + 1; j < RBLK; ++j)
+                        bv[j] = ((v[j] & 3) != MB_ADD && (v[j] >> 2) == site) ? cur : bv[j];
+                }
+            } else {
+                const int mpos = bv[i];
+                if (mpos == 0) {            // not registered: can_do false
+                    if ((e & 3) == MB_DEL_STRICT)
+                        err |= 1;
+                } else {
+                    const int last = cur;
+                    if (mpos < last) {
+                        const int moved = win_get(w, last - base);
+                        sp[mpos] = moved;
+                        bp[moved] = mpos;
+                        win_set(w, mpos - base, moved);
+#pragma unroll
+                        for (int j = i + 1; j < RBLK; ++j)
+                            bv[j] = ((v[j] & 3) != MB_ADD && (v[j] >> 2) == moved) ? mpos : bv[j];
+                    }
+                    sp[last] = 0;
+                    win_set(w, last - base, 0);
+                    bp[site] = 0;
+#pragma unroll
+                    for (int j = i + 1; j < RBLK; ++j)
+                        bv[j] = ((v[j] & 3) != MB_ADD && (v[j] >> 2) == site) ? 0 : bv[j];
+                    cur = last - 1;
+                }
+            }
+        }
+    }
+    nr = cur;
+}
+
+// Apply the queued entries of the step: each owner lane replays its
+// processes' queues in order, RBLK entries per batch of HBM reads, and
+// empties them.
+template <int PPL, int BITS>
+__device__ int apply_queues(Wave<PPL, BITS> &W)
 {
     int err = 0;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
         const int p = W.lane + 64 * k;
-        if (p >= W.P)
-            continue;
-        int *mb = W.mbox + p;
-        int *sp = W.sites(p);
-        int *bp = W.back(p);
-        // lane-private replay area: ds[KMAX] sites to delete, pos[KMAX]
-        // their current slots, tail[KMAX] the content of slots nr0, nr0-1, ..
-        int *ds = W.rp, *pos = W.rp + KMAX, *tail = W.rp + 2 * KMAX;
-        int r = 0;
-        for (;;) {
-            // collect up to KMAX deletions in round order
-            int strict = 0, nd = 0;
-            while (r < nrounds && nd < KMAX) {
-                const int v = mb[r * W.mstride];
-                if (v != 0) {
-                    if ((v & 3) == MB_ADD)
-                        break;
-                    mb[r * W.mstride] = 0;
-                    ds[nd] = v >> 2;
-                    strict |= ((v & 3) == MB_DEL_STRICT) << nd;
-                    nd += 1;
-                }
-                r += 1;
-            }
-            if (nd == 0)
-                break;
-            // one batch of HBM reads: back-index of each site, last nd slots
-            const int nr0 = W.nr[k];
-            int bv[KMAX], tv[KMAX];
-#pragma unroll
-            for (int i = 0; i < KMAX; ++i) {
-                bv[i] = 0;
-                tv[i] = 0;
-                if (i < nd) {
-                    bv[i] = bp[ds[i]];
-                    if (nr0 - i >= 0)
-                        tv[i] = sp[nr0 - i];
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < KMAX; ++i) {
-                pos[i] = bv[i];
-                tail[i] = tv[i];
-            }
-            // replay; the j-th successful deletion moves slot nr0-j
-            int done = 0;
-            for (int i = 0; i < nd; ++i) {
-                const int site = ds[i];
-                const int m = pos[i];
-                if (m == 0) {                       // not registered: can_do false
-                    if ((strict >> i) & 1)
-                        err |= 1;
-                    continue;
-                }
-                const int last = nr0 - done;
-                if (m < last) {
-                    const int moved = tail[done];
-                    sp[m] = moved;
-                    sp[last] = 0;
-                    bp[moved] = m;
-                    if (nr0 - m < KMAX)
-                        tail[nr0 - m] = moved;
-                    for (int j = i + 1; j < nd; ++j)
-                        if (ds[j] == moved) pos[j] = m;
-                } else {
-                    sp[m] = 0;
-                }
-                tail[done] = 0;
-                bp[site] = 0;
-                for (int j = i + 1; j < nd; ++j)
-                    if (ds[j] == site) pos[j] = 0;
-                done += 1;
-            }
-            W.nr[k] = nr0 - done;
+        const bool own = p < W.P;
+        int n = 0;
+        if (own) {
+            n = W.cnt[p];
+            W.cnt[p] = 0;
         }
-        // additions (base.mpy:268-301 add_proc)
-        for (; r < nrounds; ++r) {
-            const int v = mb[r * W.mstride];
-            if (v == 0)
-                continue;
-            mb[r * W.mstride] = 0;
-            const int site = v >> 2;
-            const int n1 = W.nr[k] + 1;
-            if (n1 > W.d.V) {                       // only a double add can get here
-                err |= 2;
-                continue;
+        const int *q = W.mbox + p;
+        for (int i0 = 0; __ballot(i0 < n); i0 += RBLK) {
+            const int m = min(n - i0, RBLK);
+            int v[RBLK];
+#pragma unroll
+            for (int i = 0; i < RBLK; ++i)
+                v[i] = (i < m) ? q[(i0 + i) * W.mstride] : 0;
+            if (m > 0) {
+                int nrk = W.nr[k];
+                replay_block<PPL, BITS>(W, nrk, W.sites(p), W.back(p), v, m, err);
+                W.nr[k] = nrk;
             }
-            sp[n1] = site;
-            bp[site] = n1;
-            W.nr[k] = n1;
         }
     }
+    wave_sync();
     if (__ballot(err & 2))
         return ST_CORRUPT;
     return __ballot(err & 1) ? ST_DEL_ABSENT : ST_OK;
 }
 
-// Execute one segment at base cell (bx, by, bz).  Wave-uniform call.
+// Post one segment at base cell (bx, by, bz) into the process queues: the
+// lattice writes with the replace_species check (base.mpy:
+// 1187-1231), done by one lane per write, then the ops evaluated
+// lane-parallel on the new lattice.  Wave-uniform call.  On a species
+// mismatch the writes before the failing one are done and nothing is
+// posted.
 template <int PPL, int BITS>
-__device__ int run_segment(Wave<PPL, BITS> &W, int seg, int bx, int by, int bz)
+__device__ int post_segment(Wave<PPL, BITS> &W, int seg, int bx, int by, int bz)
 {
     const int *prog = W.prog;
     const int *sg = prog + W.off_seg + SEG_W * seg;
@@ -220,20 +256,32 @@ __device__ int run_segment(Wave<PPL, BITS> &W, int seg, int bx, int by, int bz)
     const bool as_add = (sg[2] >> 24) & 1;
     const int ooff = sg[3], cbase = sg[4];
 
-    // lattice writes with the replace_species check (base.mpy:1187-1231)
-    for (int w = 0; w < nw; ++w) {
-        const int w0 = prog[woff + WR_W * w], w1 = prog[woff + WR_W * w + 1];
-        const int s = site0(W.d, bx, by, bz, unpack_x(w0), unpack_y(w0), unpack_z(w0), (w0 >> 24) & 0xFF);
-        const int cur = W.lat.get(s);
-        if (cur < 0 || !((w1 >> cur) & 1))
-            return ST_MISMATCH;
-        wave_sync();
-        if (W.lane == 0)
-            W.lat.set(s, w1 >> 16);
+    if (nw > 0) {
+        // the writes of a segment hit distinct sites (checked on the host)
+        int s = 0, cur = 0, nsp = 0;
+        bool bad = false;
+        for (int w0i = 0; w0i < nw; w0i += 64) {
+            const int w = w0i + W.lane;
+            if (w < nw) {
+                const int w0 = prog[woff + WR_W * w], w1 = prog[woff + WR_W * w + 1];
+                s = site0(W.d, bx, by, bz, unpack_x(w0), unpack_y(w0), unpack_z(w0), (w0 >> 24) & 0xFF);
+                cur = W.lat.get(s);
+                nsp = w1 >> 16;
+                bad = cur < 0 || !((w1 >> cur) & 1);
+            }
+            const unsigned long long bm = __ballot(w < nw && bad);
+            const int nok = bm ? __ffsll((long long)bm) - 1 : 64;
+            if (w < nw && W.lane < nok)
+                W.lat.xset(s, cur, nsp);
+            if (bm) {
+                wave_sync();
+                return ST_MISMATCH;
+            }
+        }
         wave_sync();
     }
 
-    // evaluate ops lane-parallel and post them to the owner's mailbox
+    const int stride = W.mstride;
     for (int base = 0; base < nops; base += 64) {
         const int j = base + W.lane;
         if (j < nops) {
@@ -251,14 +299,26 @@ __device__ int run_segment(Wave<PPL, BITS> &W, int seg, int bx, int by, int bz)
                     proc = arg;
                     code = MB_DEL_IF;
                 } else {
+                    // all condition reads issued before any is tested
                     const int nc = (op.x >> 17) & 0x3F;
                     const int *cd = prog + cbase + (int)((unsigned)op.y >> 21);
                     bool ok = true;
-                    for (int c = 0; c < nc && ok; ++c) {
-                        const int cw = cd[c];
-                        const int q = W.lat.get(site0(W.d, bx, by, bz, (cw & 63) - 32, ((cw >> 6) & 63) - 32,
-                                                      ((cw >> 12) & 63) - 32, (cw >> 18) & 63));
-                        ok = (q == ((cw >> 24) & 0xFF));
+                    for (int c0 = 0; c0 < nc; c0 += 4) {
+                        int q[4], want[4];
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            q[c] = 0;
+                            want[c] = 0;
+                            if (c0 + c < nc) {
+                                const int cw = cd[c0 + c];
+                                q[c] = W.lat.get(site0(W.d, bx, by, bz, (cw & 63) - 32, ((cw >> 6) & 63) - 32,
+                                                       ((cw >> 12) & 63) - 32, (cw >> 18) & 63));
+                                want[c] = (cw >> 24) & 0xFF;
+                            }
+                        }
+#pragma unroll
+                        for (int c = 0; c < 4; ++c)
+                            ok &= (q[c] == want[c]);
                     }
                     if (ok) {
                         proc = arg;
@@ -273,19 +333,35 @@ __device__ int run_segment(Wave<PPL, BITS> &W, int seg, int bx, int by, int bz)
                 proc = nli_eval<PPL, BITS>(W, arg, cx, cy, cz);
                 code = (kind == OP_DEL_NLI) ? MB_DEL_STRICT : MB_ADD;
             }
+            // append to the owner's queue; ops on one process sit in
+            // different rounds, so issuing the rounds in order keeps the
+            // reference order within every queue
+            int slot = 0;
+            for (int it = 0; it < nrounds; ++it)
+                if (proc > 0 && rnd == it)
+                    slot = atomicAdd(&W.cnt[proc - 1], 1);
             if (proc > 0)
-                W.mbox[rnd * W.mstride + proc - 1] = ((s + 1) << 2) | code;
+                W.mbox[slot * stride + proc - 1] = ((s + 1) << 2) | code;
         }
     }
     wave_sync();
-    const int e = apply_segment<PPL, BITS>(W, nrounds);
-    wave_sync();
-    return e;
+    return ST_OK;
 }
 
-// run_proc_nr(proc, site); procstat is counted by the caller.
+// Execute one segment (a touchup) at base cell (bx, by, bz).
 template <int PPL, int BITS>
-__device__ int run_proc(Wave<PPL, BITS> &W, int proc, int site)
+__device__ int run_segment(Wave<PPL, BITS> &W, int seg, int bx, int by, int bz)
+{
+    const int e = post_segment<PPL, BITS>(W, seg, bx, by, bz);
+    const int a = apply_queues<PPL, BITS>(W);
+    return e ? e : a;
+}
+
+// run_proc_nr(proc, site); procstat is counted by the caller.  With
+// `merged`, all segments of the process are queued and applied once, else
+// each segment is applied before the next is posted.
+template <int PPL, int BITS>
+__device__ int run_proc(Wave<PPL, BITS> &W, int proc, int site, bool merged)
 {
     const int s0 = site - 1;
     const int cell = divm(s0, W.ms);
@@ -296,35 +372,62 @@ __device__ int run_proc(Wave<PPL, BITS> &W, int proc, int site)
     const int *pp = W.prog + W.off_proc + 2 * (proc - 1);
     const int nit = pp[0];
     const int *it = W.prog + pp[1];
+    int e = ST_OK;
     for (int i = 0; i < nit; ++i, it += IT_W) {
         const int o = it[1];
         const int bx = wrap1(x + unpack_x(o), W.d.X);
         const int by = W.d.dim >= 2 ? wrap1(y + unpack_y(o), W.d.Y) : 0;
         const int bz = W.d.dim >= 3 ? wrap1(z + unpack_z(o), W.d.Z) : 0;
-        const int e = run_segment<PPL, BITS>(W, it[0], bx, by, bz);
+        e = post_segment<PPL, BITS>(W, it[0], bx, by, bz);
         if (e)
-            return e;
+            break;
+        if (!merged) {
+            e = apply_queues<PPL, BITS>(W);
+            if (e)
+                return e;
+        }
     }
-    return ST_OK;
+    const int a = merged ? apply_queues<PPL, BITS>(W) : ST_OK;
+    return e ? e : a;
+}
+
+// lane l receives v from lane l-1 (DPP wave_shr:1); lane 0 keeps `old`
+__device__ __forceinline__ double wave_shr1(double v, double old)
+{
+    const long long b = __double_as_longlong(v), o = __double_as_longlong(old);
+    const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)b, 0x138, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(b >> 32), 0x138,
+                                               0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
 // accum_rates (base.mpy:603-623) as a strictly left-to-right FP64 sum; the
-// partial sum of process p ends on lane p%64, slot p/64.  Returns the total.
+// partial sum of process p ends on lane p%64, slot p/64.  Each pass shifts
+// the partial sums one lane up and adds the lane's own term: after pass i
+// lanes 0..i hold the exact sequential prefixes ((c0 + c1) + c2) + ...,
+// the same rounding as the reference's loop.  Returns the total.
 template <int PPL>
 __device__ __forceinline__ double accumulate(const double (&c)[PPL], double (&acc)[PPL], int P, int lane)
 {
-    double a = 0.0;
+    double carry = 0.0;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
         const int lim = min(64, P - 64 * k);
-        acc[k] = 0.0;
-        for (int l = 0; l < lim; ++l) {
-            const double cl = readlane_d(c[k], l);
-            a = (k == 0 && l == 0) ? cl : a + cl;
-            acc[k] = (lane == l) ? a : acc[k];
+        if (lim <= 0) {
+            acc[k] = 0.0;
+            continue;
         }
+        double a = c[k];
+        if (k > 0 && lane == 0)
+            a = carry + c[k];
+        for (int it = 1; it < lim; ++it) {
+            const double sh = wave_shr1(a, a);
+            a = (lane >= 1) ? sh + c[k] : a;
+        }
+        acc[k] = a;
+        carry = readlane_d(a, lim - 1);
     }
-    return a;
+    return carry;
 }
 
 // interval_search_real (base.mpy:1234-1338) on the lane-distributed
@@ -405,9 +508,9 @@ __device__ __forceinline__ int wave_setup(Wave<PPL, BITS> &W, const KParams &kp,
         W.lat.p = glat;
         W.mbox = (int *)wsm;
     }
-    for (int i = W.lane; i < kp.maxR * W.mstride; i += 64)
-        W.mbox[i] = 0;
-    W.rp = W.mbox + kp.maxR * W.mstride + 3 * KMAX * W.lane;
+    W.cnt = W.mbox + kp.mrows * W.mstride;
+    for (int i = W.lane; i < W.mstride; i += 64)
+        W.cnt[i] = 0;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
         const int p = W.lane + 64 * k;
@@ -501,6 +604,10 @@ __global__ __launch_bounds__(64 * MAX_WPG) void kmc_steps_kernel(KParams kp)
         const int nrp = readlane_i(slot_get<PPL>(W.nr, pk), pl);
         int kk = (int)(1.0 + u[2] * (double)nrp);
         kk = kk > nrp ? nrp : kk;
+        if (kk < 1) {                       // only a corrupted database gets here
+            st = ST_CORRUPT;
+            break;
+        }
         int sv = 0;
         if (lane == pl) {
             sv = W.sites(pick)[kk];
@@ -510,7 +617,7 @@ __global__ __launch_bounds__(64 * MAX_WPG) void kmc_steps_kernel(KParams kp)
         }
         const int site = readlane_i(sv, pl);
         done += 1;
-        st = run_proc<PPL, BITS>(W, pick + 1, site);
+        st = run_proc<PPL, BITS>(W, pick + 1, site, kp.merged);
         if (st)
             break;
     }
@@ -567,7 +674,7 @@ __global__ __launch_bounds__(64 * MAX_WPG) void kmc_misc_kernel(KParams kp)
             for (int k = 0; k < PPL; ++k)
                 if (k == pk) W.pstat[k] += 1;
         }
-        st = run_proc<PPL, BITS>(W, proc, site);
+        st = run_proc<PPL, BITS>(W, proc, site, kp.merged);
         changed = true;
     } else if (kp.mode == MODE_NEXT_STEP) {
         double c[PPL], acc[PPL];
@@ -587,7 +694,7 @@ __global__ __launch_bounds__(64 * MAX_WPG) void kmc_misc_kernel(KParams kp)
             int kk = (int)(1.0 + u[0] * (double)nrp);
             kk = kk > nrp ? nrp : kk;
             int sv = 0;
-            if (lane == pl)
+            if (lane == pl && kk >= 1)
                 sv = W.sites(pick)[kk];
             const int site = readlane_i(sv, pl);
             if (lane == 0) {
@@ -838,6 +945,9 @@ struct kmcg_model {
     std::vector<int> blob;
     std::vector<int> prog;   // packed op program (layout in kmcg_device.h)
     int P = 0, S = 0, spuck = 0, dim = 0, backend = 0, nullsp = -1, maxoff = 0, maxR = 1;
+    int rstep = 1;       // mailbox rows of one executed process (all its segments)
+    int mrows = 1;       // mailbox rows per wave
+    int merged = 0;      // 1: one apply per step
     int off_seg = 0, off_proc = 0, off_touch = 0, off_nli = 0, off_init = 0;
 };
 
@@ -1040,6 +1150,31 @@ int build_prog(kmcg_model *m)
             m->maxR = std::max(m->maxR, sg.nrounds);
         }
 
+        // the writes of one segment hit distinct sites (they run in parallel)
+        for (auto &sg : segs) {
+            const int nw = (int)sg.writes.size() / 6;
+            for (int a = 0; a < nw; ++a)
+                for (int c = a + 1; c < nw; ++c)
+                    if (std::equal(sg.writes.begin() + 6 * a, sg.writes.begin() + 6 * a + 4,
+                                   sg.writes.begin() + 6 * c))
+                        return fail(KMCG_EUNSUPPORTED, "a routine writes one site twice");
+        }
+        // rows of one executed process; merge its segments if they fit 8 KiB
+        m->rstep = 1;
+        for (int p = 0; p < m->P; ++p) {
+            int rows = 0;
+            for (size_t i = 0; i < proc_items[p].size(); i += 4) {
+                const int sgi = proc_items[p][i];
+                if (sgi >= 0 && sgi < (int)segs.size())
+                    rows += segs[sgi].nrounds;
+            }
+            m->rstep = std::max(m->rstep, rows);
+        }
+        {
+            const int ppl = m->P <= 64 ? 1 : m->P <= 128 ? 2 : 4;
+            m->merged = (size_t)m->rstep * 64 * ppl * sizeof(int) <= 8192 ? 1 : 0;
+            m->mrows = m->merged ? std::max(m->rstep, m->maxR) : m->maxR;
+        }
         // deletions of a process precede its additions in every segment
         for (auto &sg : segs) {
             bool seen_add = false;
@@ -1265,6 +1400,8 @@ KParams make_params(kmcg_batch *b)
     kp.spuck = m->spuck;
     kp.dim = m->dim;
     kp.maxR = m->maxR;
+    kp.mrows = m->mrows;
+    kp.merged = m->merged;
     kp.backend = m->backend;
     kp.off_seg = m->off_seg;
     kp.off_proc = m->off_proc;
@@ -1288,7 +1425,7 @@ Shape launch_shape(kmcg_batch *b, const int *ids, int n, KParams &kp, int max_wp
     long long maxlb = 0;
     for (int i = 0; i < n; ++i)
         maxlb = std::max(maxlb, lattice_bytes(b->V[ids[i]], b->bits));
-    const size_t mbox = ((size_t)b->m->maxR * 64 * ppl_of(b->m->P) + 64 * 3 * KMAX) * sizeof(int);
+    const size_t mbox = (size_t)(b->m->mrows + 1) * 64 * ppl_of(b->m->P) * sizeof(int);
     const size_t prog = (size_t)kp.prog_lds;
     Shape s;
     s.lds_lat = b->lds_mode == 0 ? false : (prog + (size_t)maxlb + mbox <= (size_t)b->max_lds);

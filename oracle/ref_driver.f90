@@ -8,6 +8,7 @@
 !   X Y Z
 !   seed stream
 !   nsteps mode            mode 0: per-step trace, 1: one do_kmc_steps call
+!                          (its wall time is printed on stdout)
 !   has_config
 !   rates(1:nr_of_proc)
 !   config(1:volume)       species per site, only if has_config /= 0
@@ -48,6 +49,7 @@ program ref_driver
   real(kind=rdouble) :: ran_time, ran_proc, ran_site, t, r
   integer(kind=ilong) :: ps
   integer(kind=iint) :: nrs, u_in, u_out, nexec
+  integer(kind=8) :: c0, c1, crate
 
   call get_command_argument(1, inpath)
   call get_command_argument(2, outpath)
@@ -109,7 +111,11 @@ program ref_driver
     write(u_out) nexec
     write(u_out) tproc, tsite, times
   else
+    call system_clock(c0, crate)
     call do_kmc_steps(nsteps)
+    call system_clock(c1)
+    ! wall seconds of the reference's own step loop (the CPU baseline)
+    write(*, '(A, ES24.16)') 'do_kmc_steps_seconds ', real(c1 - c0, rdouble) / real(crate, rdouble)
     nexec = int(nsteps, iint)
     write(u_out) nexec
   end if
