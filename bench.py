@@ -215,6 +215,19 @@ def serving_leg(rate, n, seed=1):
     return out
 
 
+def calibrate_copy(nbytes=1 << 30, reps=5):
+    """A plain device-to-device copy of known size: what FETCH_SIZE and
+    WRITE_SIZE read for it calibrates the step kernel's counters."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_dispatch": nbytes, "dispatches": reps}))
+    return 0
+
+
 def make_batch(engine, model, wl, idx=None):
     idx = np.arange(wl.n) if idx is None else np.asarray(idx)
     b = engine.Batch(model, [list(s) for s in wl.sizes[idx]])
@@ -240,7 +253,11 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="request arrivals per second")
     ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a device copy of 1 GiB five times (calibrates HBM byte counters)")
     a = ap.parse_args()
+    if a.calibrate_copy:
+        return calibrate_copy()
     if a.gpus != 1:
         print("bench.py measures one GPU: replicas are independent and the engine runs "
               "them all on a single device; run with --gpus 1", file=sys.stderr)

@@ -61,6 +61,28 @@ __device__ __forceinline__ int unpack_z(int w) { return (int)(signed char)((w >>
 
 struct Dims {
     int X, Y, Z, V, spuck, dim;
+    int sX, sXY;        // spuck * X, spuck * X * Y (site strides of y and z)
+    int moff;           // largest offset component any segment uses
+    __device__ __forceinline__ void strides()
+    {
+        sX = spuck * X;
+        sXY = sX * Y;
+    }
+    // base cells at least moff away from every face: no offset wraps
+    __device__ __forceinline__ bool interior(int x, int y, int z) const
+    {
+        return x >= moff && x < X - moff && (dim < 2 || (y >= moff && y < Y - moff)) &&
+               (dim < 3 || (z >= moff && z < Z - moff));
+    }
+    __device__ __forceinline__ int lin(int x, int y, int z) const
+    {
+        return __umul24(x, spuck) + __umul24(y, sX) + __umul24(z, sXY);
+    }
+    // site offset of (dx, dy, dz, n) from a cell, without wrapping
+    __device__ __forceinline__ int off(int dx, int dy, int dz, int n) const
+    {
+        return __mul24(dx, spuck) + __mul24(dy, sX) + __mul24(dz, sXY) + n - 1;
+    }
 };
 
 // Kernel parameters (one struct, passed by value).
@@ -93,6 +115,7 @@ struct KParams {
     int wave_lds;                  // LDS bytes per wave (lattice + mailbox)
     int lat_lds;                   // LDS bytes of the lattice part (0: lattice in HBM)
     int P, S, spuck, dim, maxR, backend;
+    int moff;                      // largest offset component of the program
     int mrows;                     // mailbox rows per wave
     int merged;                    // 1: the segments of a step share one apply
     int off_seg, off_proc, off_touch, off_nli;
@@ -116,7 +139,15 @@ __device__ __forceinline__ int site0(const Dims &d, int x, int y, int z,
     const int xx = wrap1(x + dx, d.X);
     const int yy = d.dim >= 2 ? wrap1(y + dy, d.Y) : 0;
     const int zz = d.dim >= 3 ? wrap1(z + dz, d.Z) : 0;
-    return d.spuck * (xx + d.X * (yy + d.Y * zz)) + n - 1;
+    return d.lin(xx, yy, zz) + n - 1;
+}
+
+// site0 from a base cell: `in` (base cell interior, wave-uniform) skips the
+// wrap, lb is the base cell's first site d.lin(x, y, z)
+__device__ __forceinline__ int site_from(const Dims &d, bool in, int lb, int x, int y, int z,
+                                         int dx, int dy, int dz, int n)
+{
+    return in ? lb + d.off(dx, dy, dz, n) : site0(d, x, y, z, dx, dy, dz, n);
 }
 
 // a / d for 0 <= a < 2^31 with m = ceil(2^32 / d) (d > 1): one multiply-high

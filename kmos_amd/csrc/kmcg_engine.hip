@@ -47,9 +47,10 @@ struct Wave {
     Lattice<BITS> lat;
     int *mbox;         // queues [mrows][64*PPL]: entry j of process p at j*64*PPL + p
     int *cnt;          // [64*PPL] queue lengths
+    int *win;          // [64][WIN] list-tail window of each lane (replay)
     int *avail;        // sites[P][V+1] then back[P][V+1]
     long long V1;
-    int P, lane, mstride;
+    int P, lane, mstride, mrows;
     int off_seg, off_proc, off_touch, off_nli;
     int nr[PPL];
     double rate[PPL];
@@ -72,9 +73,10 @@ __device__ __forceinline__ T slot_get(const T (&a)[PPL], int k)
 
 // nli_<group>(cell): walk the case tree (kmos/io/__init__.py:1985-2057).
 template <int PPL, int BITS>
-__device__ int nli_eval(const Wave<PPL, BITS> &W, int group, int cx, int cy, int cz)
+__device__ int nli_eval(const Wave<PPL, BITS> &W, int group, bool in, int cx, int cy, int cz)
 {
     const int *prog = W.prog;
+    const int lc = W.d.lin(cx, cy, cz);
     int pos = prog[W.off_nli + group];
     for (;;) {
         const int *nd = prog + pos;
@@ -82,8 +84,8 @@ __device__ int nli_eval(const Wave<PPL, BITS> &W, int group, int cx, int cy, int
         if (nc == -1)
             return nd[2];
         const int c = nd[0];
-        const int sp = W.lat.get(site0(W.d, cx, cy, cz, unpack_x(c), unpack_y(c), unpack_z(c),
-                                       (c >> 24) & 0xFF));
+        const int sp = W.lat.get(site_from(W.d, in, lc, cx, cy, cz, unpack_x(c), unpack_y(c), unpack_z(c),
+                                           (c >> 24) & 0xFF));
         int next = -1;
         for (int k = 0; k < nc; ++k) {
             const int *e = nd + 2 + 3 * k;
@@ -115,45 +117,35 @@ __device__ int nli_eval(const Wave<PPL, BITS> &W, int group, int cx, int cy, int
 constexpr int RBLK = 4;
 constexpr int WIN = 2 * RBLK;
 
-__device__ __forceinline__ int win_get(const int (&w)[WIN], int idx)
-{
-    int v = 0;
-#pragma unroll
-    for (int q = 0; q < WIN; ++q)
-        v = (idx == q) ? w[q] : v;
-    return v;
-}
-
-__device__ __forceinline__ void win_set(int (&w)[WIN], int idx, int val)
-{
-#pragma unroll
-    for (int q = 0; q < WIN; ++q)
-        w[q] = (idx == q) ? val : w[q];
-}
-
+// w: this lane's window in LDS, slot q at w[q - (nr0 - RBLK + 1)]
 template <int PPL, int BITS>
-__device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, int *sp, int *bp,
+__device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, int *sp, int *bp, int *w,
                                              const int (&v)[RBLK], int m, int &err)
 {
     const int nr0 = nr;
+    // all reads are issued unconditionally (index 0 = the unused slot 0
+    // when not needed) so that they form one batch with a single wait
     int bv[RBLK];
     int nd = 0;
 #pragma unroll
     for (int i = 0; i < RBLK; ++i) {
-        bv[i] = 0;
-        if (i < m && (v[i] & 3) != MB_ADD) {
-            bv[i] = bp[v[i] >> 2];
-            nd += 1;
-        }
+        const bool del = i < m && (v[i] & 3) != MB_ADD;
+        bv[i] = bp[del ? (v[i] >> 2) : 0];
+        nd += del ? 1 : 0;
     }
-    int w[WIN];   // slot q <-> w[q - base]
-#pragma unroll
-    for (int j = 0; j < WIN; ++j)
-        w[j] = 0;
+    int tv[RBLK];
 #pragma unroll
     for (int j = 0; j < RBLK; ++j)
-        if (j < nd && nr0 - j >= 1)
-            w[RBLK - 1 - j] = sp[nr0 - j];
+        tv[j] = sp[(j < nd && nr0 - j >= 1) ? nr0 - j : 0];
+#pragma unroll
+    for (int i = 0; i < RBLK; ++i)
+        bv[i] = (i < m && (v[i] & 3) != MB_ADD) ? bv[i] : 0;
+    // slots above nr0 are only read after an addition of this block wrote
+    // them, slots below nr0-nd+1 are never read (their entries here are
+    // written anyway, so that no read above is conditional)
+#pragma unroll
+    for (int j = 0; j < RBLK; ++j)
+        w[RBLK - 1 - j] = tv[j];
     int cur = nr0;
     const int base = nr0 - (RBLK - 1);
 #pragma unroll
@@ -170,7 +162,7 @@ __device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, int *s
                     cur += 1;
                     sp[cur] = site;
                     bp[site] = cur;
-                    win_set(w, cur - base, site);
+                    w[cur - base] = site;
 #pragma unroll
                     for (int j = i + 1; j < RBLK; ++j)
                         bv[j] = ((v[j] & 3) != MB_ADD && (v[j] >> 2) == site) ? cur : bv[j];
@@ -183,16 +175,16 @@ __device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, int *s
                 } else {
                     const int last = cur;
                     if (mpos < last) {
-                        const int moved = win_get(w, last - base);
+                        const int moved = w[last - base];
                         sp[mpos] = moved;
                         bp[moved] = mpos;
-                        win_set(w, mpos - base, moved);
+                        if (mpos >= base)
+                            w[mpos - base] = moved;
 #pragma unroll
                         for (int j = i + 1; j < RBLK; ++j)
                             bv[j] = ((v[j] & 3) != MB_ADD && (v[j] >> 2) == moved) ? mpos : bv[j];
                     }
                     sp[last] = 0;
-                    win_set(w, last - base, 0);
                     bp[site] = 0;
 #pragma unroll
                     for (int j = i + 1; j < RBLK; ++j)
@@ -209,28 +201,35 @@ __device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, int *s
 // processes' queues in order, RBLK entries per batch of HBM reads, and
 // empties them.
 template <int PPL, int BITS>
-__device__ int apply_queues(Wave<PPL, BITS> &W)
+__device__ __forceinline__ int apply_queues(Wave<PPL, BITS> &W)
 {
     int err = 0;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
         const int p = W.lane + 64 * k;
         const bool own = p < W.P;
-        int n = 0;
-        if (own) {
-            n = W.cnt[p];
-            W.cnt[p] = 0;
-        }
         const int *q = W.mbox + p;
+        // the count and the first block are read together
+        int n = W.cnt[p];
+        int v[RBLK];
+#pragma unroll
+        for (int i = 0; i < RBLK; ++i)
+            v[i] = q[min(i, W.mrows - 1) * W.mstride];
+        n = own ? n : 0;
+        W.cnt[p] = 0;
         for (int i0 = 0; __ballot(i0 < n); i0 += RBLK) {
             const int m = min(n - i0, RBLK);
-            int v[RBLK];
+            if (i0 > 0) {
+#pragma unroll
+                for (int i = 0; i < RBLK; ++i)
+                    v[i] = q[min(i0 + i, W.mrows - 1) * W.mstride];
+            }
 #pragma unroll
             for (int i = 0; i < RBLK; ++i)
-                v[i] = (i < m) ? q[(i0 + i) * W.mstride] : 0;
+                v[i] = (i < m) ? v[i] : 0;
             if (m > 0) {
                 int nrk = W.nr[k];
-                replay_block<PPL, BITS>(W, nrk, W.sites(p), W.back(p), v, m, err);
+                replay_block<PPL, BITS>(W, nrk, W.sites(p), W.back(p), W.win + WIN * W.lane, v, m, err);
                 W.nr[k] = nrk;
             }
         }
@@ -241,20 +240,128 @@ __device__ int apply_queues(Wave<PPL, BITS> &W)
     return __ballot(err & 1) ? ST_DEL_ABSENT : ST_OK;
 }
 
-// Post one segment at base cell (bx, by, bz) into the process queues: the
-// lattice writes with the replace_species check (base.mpy:
-// 1187-1231), done by one lane per write, then the ops evaluated
-// lane-parallel on the new lattice.  Wave-uniform call.  On a species
-// mismatch the writes before the failing one are done and nothing is
-// posted.
 template <int PPL, int BITS>
-__device__ int post_segment(Wave<PPL, BITS> &W, int seg, int bx, int by, int bz)
+__device__ __forceinline__ int cond_species(const Wave<PPL, BITS> &W, int cw, bool in, int lb, int bx, int by,
+                                            int bz)
+{
+    return W.lat.get(site_from(W.d, in, lb, bx, by, bz, (cw & 63) - 32, ((cw >> 6) & 63) - 32,
+                               ((cw >> 12) & 63) - 32, (cw >> 18) & 63));
+}
+
+// Evaluate op `op` (its first four condition words in cw) at base cell
+// (bx, by, bz) on the current lattice and append it to its owner's queue.
+// Ops on one process sit in different rounds, so issuing the rounds in
+// order keeps the reference order within every queue.
+template <int PPL, int BITS>
+__device__ __forceinline__ void eval_post(Wave<PPL, BITS> &W, bool valid, int2 op, const int (&cw)[4],
+                                          const int *cd, bool as_add, int nrounds, bool in, int lb, int bx,
+                                          int by, int bz)
+{
+    int proc = 0, code = 0, s = 0, rnd = 0;
+    if (valid) {
+        int kind = op.x & 7;
+        if (as_add && kind == OP_DEL_NLI)
+            kind = OP_ADD_NLI;
+        rnd = (op.x >> 3) & 0xFF;
+        const int n = (op.x >> 11) & 0x3F;
+        const int arg = (int)((unsigned)op.x >> 23);
+        const int dx = sext7(op.y & 0x7F), dy = sext7((op.y >> 7) & 0x7F), dz = sext7((op.y >> 14) & 0x7F);
+        if (kind == OP_DEL_IF || kind == OP_ADD_IF) {
+            s = site_from(W.d, in, lb, bx, by, bz, dx, dy, dz, n);
+            if (kind == OP_DEL_IF) {
+                proc = arg;
+                code = MB_DEL_IF;
+            } else {
+                const int nc = (op.x >> 17) & 0x3F;
+                int q[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    q[c] = cond_species<PPL, BITS>(W, (c < nc) ? cw[c] : (32 | 32 << 6 | 32 << 12 | 1 << 18),
+                                                   in, lb, bx, by, bz);
+                bool ok = true;
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    ok &= (c >= nc) || (q[c] == ((cw[c] >> 24) & 0xFF));
+                for (int c = 4; c < nc; ++c) {
+                    const int w = cd[c];
+                    ok &= cond_species<PPL, BITS>(W, w, in, lb, bx, by, bz) == ((w >> 24) & 0xFF);
+                }
+                if (ok) {
+                    proc = arg;
+                    code = MB_ADD;
+                }
+            }
+        } else {
+            const int cx = wrap1(bx + dx, W.d.X);
+            const int cy = W.d.dim >= 2 ? wrap1(by + dy, W.d.Y) : 0;
+            const int cz = W.d.dim >= 3 ? wrap1(bz + dz, W.d.Z) : 0;
+            s = W.d.lin(cx, cy, cz);
+            proc = nli_eval<PPL, BITS>(W, arg, in, cx, cy, cz);
+            code = (kind == OP_DEL_NLI) ? MB_DEL_STRICT : MB_ADD;
+        }
+    }
+    int slot = 0;
+    if (nrounds <= 4) {
+        // one result register per round: the rounds' atomics issue back to
+        // back and are waited for once
+        int sl[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int it = 0; it < 4; ++it)
+            if (it < nrounds && proc > 0 && rnd == it)
+                sl[it] = atomicAdd(&W.cnt[proc - 1], 1);
+#pragma unroll
+        for (int it = 0; it < 4; ++it)
+            slot = (rnd == it) ? sl[it] : slot;
+    } else {
+        for (int it = 0; it < nrounds; ++it)
+            if (proc > 0 && rnd == it)
+                slot = atomicAdd(&W.cnt[proc - 1], 1);
+    }
+    if (proc > 0)
+        W.mbox[slot * W.mstride + proc - 1] = ((s + 1) << 2) | code;
+}
+
+// Load op j of a segment and its first four condition words.
+__device__ __forceinline__ void load_op(const int *prog, int ooff, int cbase, int j, bool valid, int2 &op,
+                                        int (&cw)[4], const int *&cd)
+{
+    op = make_int2(0, 0);
+    cd = prog;
+    if (valid)
+        op = *(const int2 *)(prog + ooff + OP_W * j);
+    const int nc = ((op.x & 7) == OP_ADD_IF) ? (op.x >> 17) & 0x3F : 0;
+    cd = prog + cbase + (int)((unsigned)op.y >> 21);
+    // unconditional reads (the program ends with 4 padding words)
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        cw[c] = cd[c];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        cw[c] = (c < nc) ? cw[c] : 0;
+}
+
+// Post one segment at base cell (bx, by, bz) into the process queues: the
+// lattice writes with the replace_species check (base.mpy:1187-1231), done
+// by one lane per write, then the ops evaluated lane-parallel on the new
+// lattice.  The op and condition words of the first 64 ops are read before
+// the writes (they do not depend on the lattice).  Wave-uniform call.  On
+// a species mismatch the writes before the failing one are done and
+// nothing is posted.
+template <int PPL, int BITS>
+__device__ __forceinline__ int post_segment(Wave<PPL, BITS> &W, int seg, int bx, int by, int bz)
 {
     const int *prog = W.prog;
     const int *sg = prog + W.off_seg + SEG_W * seg;
     const int nw = sg[0], woff = sg[1], nops = sg[2] & 0xFFFF, nrounds = (sg[2] >> 16) & 0xFF;
     const bool as_add = (sg[2] >> 24) & 1;
     const int ooff = sg[3], cbase = sg[4];
+
+    int2 op;
+    int cw[4];
+    const int *cd;
+    load_op(prog, ooff, cbase, W.lane, W.lane < nops, op, cw, cd);
+    const bool in = W.d.interior(bx, by, bz);
+    const int lb = W.d.lin(bx, by, bz);
 
     if (nw > 0) {
         // the writes of a segment hit distinct sites (checked on the host)
@@ -264,7 +371,8 @@ __device__ int post_segment(Wave<PPL, BITS> &W, int seg, int bx, int by, int bz)
             const int w = w0i + W.lane;
             if (w < nw) {
                 const int w0 = prog[woff + WR_W * w], w1 = prog[woff + WR_W * w + 1];
-                s = site0(W.d, bx, by, bz, unpack_x(w0), unpack_y(w0), unpack_z(w0), (w0 >> 24) & 0xFF);
+                s = site_from(W.d, in, lb, bx, by, bz, unpack_x(w0), unpack_y(w0), unpack_z(w0),
+                              (w0 >> 24) & 0xFF);
                 cur = W.lat.get(s);
                 nsp = w1 >> 16;
                 bad = cur < 0 || !((w1 >> cur) & 1);
@@ -281,68 +389,11 @@ __device__ int post_segment(Wave<PPL, BITS> &W, int seg, int bx, int by, int bz)
         wave_sync();
     }
 
-    const int stride = W.mstride;
-    for (int base = 0; base < nops; base += 64) {
+    eval_post<PPL, BITS>(W, W.lane < nops, op, cw, cd, as_add, nrounds, in, lb, bx, by, bz);
+    for (int base = 64; base < nops; base += 64) {
         const int j = base + W.lane;
-        if (j < nops) {
-            const int2 op = *(const int2 *)(prog + ooff + OP_W * j);
-            int kind = op.x & 7;
-            if (as_add && kind == OP_DEL_NLI)
-                kind = OP_ADD_NLI;
-            const int rnd = (op.x >> 3) & 0xFF, n = (op.x >> 11) & 0x3F;
-            const int arg = (int)((unsigned)op.x >> 23);
-            const int dx = sext7(op.y & 0x7F), dy = sext7((op.y >> 7) & 0x7F), dz = sext7((op.y >> 14) & 0x7F);
-            int proc = 0, code = 0, s;
-            if (kind == OP_DEL_IF || kind == OP_ADD_IF) {
-                s = site0(W.d, bx, by, bz, dx, dy, dz, n);
-                if (kind == OP_DEL_IF) {
-                    proc = arg;
-                    code = MB_DEL_IF;
-                } else {
-                    // all condition reads issued before any is tested
-                    const int nc = (op.x >> 17) & 0x3F;
-                    const int *cd = prog + cbase + (int)((unsigned)op.y >> 21);
-                    bool ok = true;
-                    for (int c0 = 0; c0 < nc; c0 += 4) {
-                        int q[4], want[4];
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) {
-                            q[c] = 0;
-                            want[c] = 0;
-                            if (c0 + c < nc) {
-                                const int cw = cd[c0 + c];
-                                q[c] = W.lat.get(site0(W.d, bx, by, bz, (cw & 63) - 32, ((cw >> 6) & 63) - 32,
-                                                       ((cw >> 12) & 63) - 32, (cw >> 18) & 63));
-                                want[c] = (cw >> 24) & 0xFF;
-                            }
-                        }
-#pragma unroll
-                        for (int c = 0; c < 4; ++c)
-                            ok &= (q[c] == want[c]);
-                    }
-                    if (ok) {
-                        proc = arg;
-                        code = MB_ADD;
-                    }
-                }
-            } else {
-                const int cx = wrap1(bx + dx, W.d.X);
-                const int cy = W.d.dim >= 2 ? wrap1(by + dy, W.d.Y) : 0;
-                const int cz = W.d.dim >= 3 ? wrap1(bz + dz, W.d.Z) : 0;
-                s = site0(W.d, cx, cy, cz, 0, 0, 0, 1);
-                proc = nli_eval<PPL, BITS>(W, arg, cx, cy, cz);
-                code = (kind == OP_DEL_NLI) ? MB_DEL_STRICT : MB_ADD;
-            }
-            // append to the owner's queue; ops on one process sit in
-            // different rounds, so issuing the rounds in order keeps the
-            // reference order within every queue
-            int slot = 0;
-            for (int it = 0; it < nrounds; ++it)
-                if (proc > 0 && rnd == it)
-                    slot = atomicAdd(&W.cnt[proc - 1], 1);
-            if (proc > 0)
-                W.mbox[slot * stride + proc - 1] = ((s + 1) << 2) | code;
-        }
+        load_op(prog, ooff, cbase, j, j < nops, op, cw, cd);
+        eval_post<PPL, BITS>(W, j < nops, op, cw, cd, as_add, nrounds, in, lb, bx, by, bz);
     }
     wave_sync();
     return ST_OK;
@@ -402,14 +453,18 @@ __device__ __forceinline__ double wave_shr1(double v, double old)
 }
 
 // accum_rates (base.mpy:603-623) as a strictly left-to-right FP64 sum; the
-// partial sum of process p ends on lane p%64, slot p/64.  Each pass shifts
-// the partial sums one lane up and adds the lane's own term: after pass i
-// lanes 0..i hold the exact sequential prefixes ((c0 + c1) + c2) + ...,
-// the same rounding as the reference's loop.  Returns the total.
+// partial sum of process p ends on lane p%64, slot p/64.  Pass `it` shifts
+// the partial sums one lane up and adds each lane's own term on lanes >= it:
+// lane it then holds the exact sequential prefix ((c0 + c1) + c2) + ...,
+// the same rounding as the reference's loop.  Prefixes below the first
+// process whose term changed (`first`, per slot) are kept from the last
+// step, so only the passes from there on run.  Returns the total.
 template <int PPL>
-__device__ __forceinline__ double accumulate(const double (&c)[PPL], double (&acc)[PPL], int P, int lane)
+__device__ __forceinline__ double accumulate(const double (&c)[PPL], double (&acc)[PPL], int P, int lane,
+                                             const int (&first)[PPL])
 {
     double carry = 0.0;
+    bool dirty = false;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
         const int lim = min(64, P - 64 * k);
@@ -417,15 +472,19 @@ __device__ __forceinline__ double accumulate(const double (&c)[PPL], double (&ac
             acc[k] = 0.0;
             continue;
         }
-        double a = c[k];
-        if (k > 0 && lane == 0)
-            a = carry + c[k];
-        for (int it = 1; it < lim; ++it) {
-            const double sh = wave_shr1(a, a);
-            a = (lane >= 1) ? sh + c[k] : a;
+        const int f = dirty ? 0 : first[k];
+        if (f < lim) {
+            double a = (lane >= f) ? c[k] : acc[k];
+            if (f == 0 && k > 0 && lane == 0)
+                a = carry + c[k];
+            for (int it = max(f, 1); it < lim; ++it) {
+                const double sh = wave_shr1(a, a);
+                a = (lane >= it) ? sh + c[k] : a;
+            }
+            acc[k] = a;
+            dirty = true;
         }
-        acc[k] = a;
-        carry = readlane_d(a, lim - 1);
+        carry = readlane_d(acc[k], lim - 1);
     }
     return carry;
 }
@@ -489,7 +548,8 @@ __device__ __forceinline__ int wave_setup(Wave<PPL, BITS> &W, const KParams &kp,
     dd.z = __builtin_amdgcn_readfirstlane(dd0.z);
     dd.w = __builtin_amdgcn_readfirstlane(dd0.w);
     W.d.X = dd.x; W.d.Y = dd.y; W.d.Z = dd.z; W.d.V = dd.w;
-    W.d.spuck = kp.spuck; W.d.dim = kp.dim;
+    W.d.spuck = kp.spuck; W.d.dim = kp.dim; W.d.moff = kp.moff;
+    W.d.strides();
     W.mx = make_magic(dd.x);
     W.my = make_magic(dd.y);
     W.ms = make_magic(kp.spuck);
@@ -508,7 +568,9 @@ __device__ __forceinline__ int wave_setup(Wave<PPL, BITS> &W, const KParams &kp,
         W.lat.p = glat;
         W.mbox = (int *)wsm;
     }
+    W.mrows = kp.mrows;
     W.cnt = W.mbox + kp.mrows * W.mstride;
+    W.win = W.cnt + W.mstride;
     for (int i = W.lane; i < W.mstride; i += 64)
         W.cnt[i] = 0;
 #pragma unroll
@@ -576,21 +638,33 @@ __global__ __launch_bounds__(64 * MAX_WPG) void kmc_steps_kernel(KParams kp)
     const unsigned seed = kp.seed[r], stream = kp.stream[r];
     int st = ST_OK;
     long long done = 0;
+    // accum_rates stay in registers across steps; `first` is the lowest
+    // lane per slot whose term changed (64 = none), everything at the start
+    double acc[PPL];
+    int first[PPL];
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        acc[k] = 0.0;
+        first[k] = 0;
+    }
+    // the uniforms of the next step and -log of its time uniform are made
+    // while the current step waits for its site
+    double u[3];
+    step_uniforms(seed, stream, ctr, u);
+    double mlog = -log(u[0]);
     while (done < n) {
-        double c[PPL], acc[PPL];
+        double c[PPL];
 #pragma unroll
         for (int k = 0; k < PPL; ++k)
             c[k] = (double)W.nr[k] * W.rate[k];
-        const double total = accumulate<PPL>(c, acc, P, lane);
+        const double total = accumulate<PPL>(c, acc, P, lane, first);
         if (!(total > 0.0)) {
             st = ST_ABSORBING;
             break;
         }
-        double u[3];
-        step_uniforms(seed, stream, ctr, u);
         ctr += 1;
         // update_clocks (base.mpy:1123-1161)
-        const double dt = -log(u[0]) / total;
+        const double dt = mlog / total;
         t = t + dt;
         dt_last = dt;
         ks += 1;
@@ -615,11 +689,22 @@ __global__ __launch_bounds__(64 * MAX_WPG) void kmc_steps_kernel(KParams kp)
             for (int k = 0; k < PPL; ++k)
                 if (k == pk) W.pstat[k] += 1;
         }
+        step_uniforms(seed, stream, ctr, u);
+        mlog = -log(u[0]);
         const int site = readlane_i(sv, pl);
         done += 1;
+        int nr_before[PPL];
+#pragma unroll
+        for (int k = 0; k < PPL; ++k)
+            nr_before[k] = W.nr[k];
         st = run_proc<PPL, BITS>(W, pick + 1, site, kp.merged);
         if (st)
             break;
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) {
+            const unsigned long long ch = __ballot(W.nr[k] != nr_before[k]);
+            first[k] = ch ? __ffsll((long long)ch) - 1 : 64;
+        }
     }
     wave_store<PPL, BITS, LDS_LAT>(W, kp, r, true);
     if (lane == 0) {
@@ -678,10 +763,14 @@ __global__ __launch_bounds__(64 * MAX_WPG) void kmc_misc_kernel(KParams kp)
         changed = true;
     } else if (kp.mode == MODE_NEXT_STEP) {
         double c[PPL], acc[PPL];
+        int first[PPL];
 #pragma unroll
-        for (int k = 0; k < PPL; ++k)
+        for (int k = 0; k < PPL; ++k) {
             c[k] = (double)W.nr[k] * W.rate[k];
-        const double total = accumulate<PPL>(c, acc, P, lane);
+            acc[k] = 0.0;
+            first[k] = 0;
+        }
+        const double total = accumulate<PPL>(c, acc, P, lane, first);
         if (!(total > 0.0)) {
             st = ST_ABSORBING;
         } else {
@@ -782,7 +871,8 @@ __global__ __launch_bounds__(256) void kmc_init_lists_kernel(KParams kp)
     const int r = kp.rep_ids[slot];
     const int4 dd = kp.dims[r];
     Dims d;
-    d.X = dd.x; d.Y = dd.y; d.Z = dd.z; d.V = dd.w; d.spuck = kp.spuck; d.dim = kp.dim;
+    d.X = dd.x; d.Y = dd.y; d.Z = dd.z; d.V = dd.w; d.spuck = kp.spuck; d.dim = kp.dim; d.moff = kp.moff;
+    d.strides();
     Lattice<BITS> lat;
     lat.p = kp.lat_pool + kp.lat_off[r];
     const int *prog = kp.prog;
@@ -945,6 +1035,7 @@ struct kmcg_model {
     std::vector<int> blob;
     std::vector<int> prog;   // packed op program (layout in kmcg_device.h)
     int P = 0, S = 0, spuck = 0, dim = 0, backend = 0, nullsp = -1, maxoff = 0, maxR = 1;
+    int moff = 0;        // largest offset component any segment reaches
     int rstep = 1;       // mailbox rows of one executed process (all its segments)
     int mrows = 1;       // mailbox rows per wave
     int merged = 0;      // 1: one apply per step
@@ -1341,6 +1432,38 @@ int build_prog(kmcg_model *m)
                 if (n < 1 || n > m->spuck || nw < 0 || nw >= m->S)
                     return fail(KMCG_EINVAL, "bad lattice write");
             }
+        // largest offset component any segment reaches from its base cell
+        {
+            int mnode = 0;
+            for (const auto &src : nli_nodes) {
+                size_t pos = 0;
+                while (pos + 5 <= src.size()) {
+                    const int nc = src[pos + 4];
+                    if (nc == -1) {
+                        pos += 6;
+                        continue;
+                    }
+                    for (int q = 0; q < 3; ++q)
+                        mnode = std::max(mnode, std::abs(src[pos + q]));
+                    pos += 5 + 3 * (size_t)nc;
+                }
+            }
+            int mo = 0;
+            for (const auto &sg : segs) {
+                for (size_t w = 0; w + 6 <= sg.writes.size(); w += 6)
+                    for (int q = 0; q < 3; ++q)
+                        mo = std::max(mo, std::abs(sg.writes[w + q]));
+                for (const auto &op : sg.ops) {
+                    const int od = std::max(std::abs(op.dx), std::max(std::abs(op.dy), std::abs(op.dz)));
+                    const bool nli = op.kind == OP_DEL_NLI || op.kind == OP_ADD_NLI;
+                    mo = std::max(mo, nli ? od + mnode : od);
+                    for (size_t c = 0; c + 5 <= op.conds.size(); c += 5)
+                        for (int q = 0; q < 3; ++q)
+                            mo = std::max(mo, std::abs(op.conds[c + q]));
+                }
+            }
+            m->moff = mo;
+        }
         // per process: anchor site type, its touchup addition op (local_smart)
         // or its group (lat_int), for the parallel initialisation
         m->off_init = (int)pg.size();
@@ -1361,6 +1484,7 @@ int build_prog(kmcg_model *m)
                 }
             pg.insert(pg.end(), {n_exec, opw, cb});
         }
+        pg.insert(pg.end(), {0, 0, 0, 0});   // padding: condition words are read four at a time
         if (pg.size() * sizeof(int) > 64 * 1024)
             return fail(KMCG_EUNSUPPORTED, "op program larger than 64 KiB");
     } catch (const std::exception &e) {
@@ -1401,6 +1525,7 @@ KParams make_params(kmcg_batch *b)
     kp.dim = m->dim;
     kp.maxR = m->maxR;
     kp.mrows = m->mrows;
+    kp.moff = m->moff;
     kp.merged = m->merged;
     kp.backend = m->backend;
     kp.off_seg = m->off_seg;
@@ -1425,7 +1550,7 @@ Shape launch_shape(kmcg_batch *b, const int *ids, int n, KParams &kp, int max_wp
     long long maxlb = 0;
     for (int i = 0; i < n; ++i)
         maxlb = std::max(maxlb, lattice_bytes(b->V[ids[i]], b->bits));
-    const size_t mbox = (size_t)(b->m->mrows + 1) * 64 * ppl_of(b->m->P) * sizeof(int);
+    const size_t mbox = ((size_t)(b->m->mrows + 1) * 64 * ppl_of(b->m->P) + 64 * WIN) * sizeof(int);
     const size_t prog = (size_t)kp.prog_lds;
     Shape s;
     s.lds_lat = b->lds_mode == 0 ? false : (prog + (size_t)maxlb + mbox <= (size_t)b->max_lds);
@@ -1602,8 +1727,8 @@ static int sizes_ok(const kmcg_model *m, int32_t n, const int32_t *sizes)
         if (lim < m->maxoff)
             return fail(KMCG_EINVAL, "system size below the model's minimum of "
                                          + std::to_string(m->maxoff) + " cells per dimension");
-        if ((long long)X * Y * Z * m->spuck >= (1LL << 28))
-            return fail(KMCG_EUNSUPPORTED, "more than 2^28 sites per replica");
+        if ((long long)X * Y * Z * m->spuck >= (1LL << 24))
+            return fail(KMCG_EUNSUPPORTED, "more than 2^24 sites per replica");
     }
     return KMCG_OK;
 }
