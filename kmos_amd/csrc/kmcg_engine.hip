@@ -647,12 +647,32 @@ __global__ __launch_bounds__(64 * MAX_WPG) void kmc_steps_kernel(KParams kp)
         acc[k] = 0.0;
         first[k] = 0;
     }
-    // the uniforms of the next step and -log of its time uniform are made
-    // while the current step waits for its site
-    double u[3];
-    step_uniforms(seed, stream, ctr, u);
-    double mlog = -log(u[0]);
+    // uniforms of 64 steps at a time: lane i draws step ubase + i (its
+    // -log(u_time), u_proc, u_site), each step reads its lane's values
+    double lmlog, lu1, lu2;
+    unsigned long long ubase = ctr;
+    int uj = 0;
+    {
+        double uu[3];
+        step_uniforms(seed, stream, ubase + lane, uu);
+        lmlog = -log(uu[0]);
+        lu1 = uu[1];
+        lu2 = uu[2];
+    }
     while (done < n) {
+        if (uj == 64) {
+            ubase += 64;
+            uj = 0;
+            double uu[3];
+            step_uniforms(seed, stream, ubase + lane, uu);
+            lmlog = -log(uu[0]);
+            lu1 = uu[1];
+            lu2 = uu[2];
+        }
+        const double mlog = readlane_d(lmlog, uj);
+        double u[3];
+        u[1] = readlane_d(lu1, uj);
+        u[2] = readlane_d(lu2, uj);
         double c[PPL];
 #pragma unroll
         for (int k = 0; k < PPL; ++k)
@@ -689,8 +709,7 @@ __global__ __launch_bounds__(64 * MAX_WPG) void kmc_steps_kernel(KParams kp)
             for (int k = 0; k < PPL; ++k)
                 if (k == pk) W.pstat[k] += 1;
         }
-        step_uniforms(seed, stream, ctr, u);
-        mlog = -log(u[0]);
+        uj += 1;
         const int site = readlane_i(sv, pl);
         done += 1;
         int nr_before[PPL];
