@@ -442,21 +442,23 @@ __device__ int run_proc(Wave<PPL, BITS> &W, int proc, int site, bool merged)
     return e ? e : a;
 }
 
-// lane l receives v from lane l-1 (DPP wave_shr:1); lane 0 keeps `old`
-__device__ __forceinline__ double wave_shr1(double v, double old)
+// lane l receives v from lane l-1 (DPP wave_shr:1); lane 0 receives 0
+__device__ __forceinline__ double wave_shr1z(double v)
 {
-    const long long b = __double_as_longlong(v), o = __double_as_longlong(old);
-    const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)b, 0x138, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(b >> 32), 0x138,
-                                               0xF, 0xF, false);
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, 0x138, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), 0x138, 0xF, 0xF, true);
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
+
 // accum_rates (base.mpy:603-623) as a strictly left-to-right FP64 sum; the
-// partial sum of process p ends on lane p%64, slot p/64.  Pass `it` shifts
-// the partial sums one lane up and adds each lane's own term on lanes >= it:
-// lane it then holds the exact sequential prefix ((c0 + c1) + c2) + ...,
-// the same rounding as the reference's loop.  Prefixes below the first
+// partial sum of process p ends on lane p%64, slot p/64.  Each pass shifts
+// the partial sums one lane up (lane 0 receives the sum of the slots
+// before) and adds every lane's own term: after pass i lane i holds the
+// exact sequential prefix ((c0 + c1) + c2) + ..., the same rounding as the
+// reference's loop, and a lane that holds its final prefix keeps it in
+// later passes (its left neighbour is final too).  Prefixes below the first
 // process whose term changed (`first`, per slot) are kept from the last
 // step, so only the passes from there on run.  Returns the total.
 template <int PPL>
@@ -474,13 +476,11 @@ __device__ __forceinline__ double accumulate(const double (&c)[PPL], double (&ac
         }
         const int f = dirty ? 0 : first[k];
         if (f < lim) {
-            double a = (lane >= f) ? c[k] : acc[k];
-            if (f == 0 && k > 0 && lane == 0)
-                a = carry + c[k];
-            for (int it = max(f, 1); it < lim; ++it) {
-                const double sh = wave_shr1(a, a);
-                a = (lane >= it) ? sh + c[k] : a;
-            }
+            // lane 0's term carries the sum of the slots before (shifted-in 0)
+            const double ce = (lane == 0) ? carry + c[k] : c[k];
+            double a = (lane >= f) ? ce : acc[k];
+            for (int it = max(f, 1); it < lim; ++it)
+                a = wave_shr1z(a) + ce;
             acc[k] = a;
             dirty = true;
         }
