@@ -142,12 +142,15 @@ __device__ __forceinline__ int site0(const Dims &d, int x, int y, int z,
     return d.lin(xx, yy, zz) + n - 1;
 }
 
-// site0 from a base cell: `in` (base cell interior, wave-uniform) skips the
-// wrap, lb is the base cell's first site d.lin(x, y, z)
-__device__ __forceinline__ int site_from(const Dims &d, bool in, int lb, int x, int y, int z,
-                                         int dx, int dy, int dz, int n)
+// site0 from a base cell: IN (base cell interior) skips the wrap, lb is
+// the base cell's first site d.lin(x, y, z).  A template parameter, so that
+// the caller branches once instead of computing both forms.
+template <bool IN>
+__device__ __forceinline__ int site_from(const Dims &d, int lb, int x, int y, int z, int dx, int dy, int dz, int n)
 {
-    return in ? lb + d.off(dx, dy, dz, n) : site0(d, x, y, z, dx, dy, dz, n);
+    if (IN)
+        return lb + d.off(dx, dy, dz, n);
+    return site0(d, x, y, z, dx, dy, dz, n);
 }
 
 // a / d for 0 <= a < 2^31 with m = ceil(2^32 / d) (d > 1): one multiply-high

@@ -72,8 +72,8 @@ __device__ __forceinline__ T slot_get(const T (&a)[PPL], int k)
 }
 
 // nli_<group>(cell): walk the case tree (kmos/io/__init__.py:1985-2057).
-template <int PPL, int BITS>
-__device__ int nli_eval(const Wave<PPL, BITS> &W, int group, bool in, int cx, int cy, int cz)
+template <int PPL, int BITS, bool IN>
+__device__ int nli_eval(const Wave<PPL, BITS> &W, int group, int cx, int cy, int cz)
 {
     const int *prog = W.prog;
     const int lc = W.d.lin(cx, cy, cz);
@@ -84,8 +84,8 @@ __device__ int nli_eval(const Wave<PPL, BITS> &W, int group, bool in, int cx, in
         if (nc == -1)
             return nd[2];
         const int c = nd[0];
-        const int sp = W.lat.get(site_from(W.d, in, lc, cx, cy, cz, unpack_x(c), unpack_y(c), unpack_z(c),
-                                           (c >> 24) & 0xFF));
+        const int sp = W.lat.get(site_from<IN>(W.d, lc, cx, cy, cz, unpack_x(c), unpack_y(c), unpack_z(c),
+                                               (c >> 24) & 0xFF));
         int next = -1;
         for (int k = 0; k < nc; ++k) {
             const int *e = nd + 2 + 3 * k;
@@ -240,22 +240,20 @@ __device__ __forceinline__ int apply_queues(Wave<PPL, BITS> &W)
     return __ballot(err & 1) ? ST_DEL_ABSENT : ST_OK;
 }
 
-template <int PPL, int BITS>
-__device__ __forceinline__ int cond_species(const Wave<PPL, BITS> &W, int cw, bool in, int lb, int bx, int by,
-                                            int bz)
+template <int PPL, int BITS, bool IN>
+__device__ __forceinline__ int cond_species(const Wave<PPL, BITS> &W, int cw, int lb, int bx, int by, int bz)
 {
-    return W.lat.get(site_from(W.d, in, lb, bx, by, bz, (cw & 63) - 32, ((cw >> 6) & 63) - 32,
-                               ((cw >> 12) & 63) - 32, (cw >> 18) & 63));
+    return W.lat.get(site_from<IN>(W.d, lb, bx, by, bz, (cw & 63) - 32, ((cw >> 6) & 63) - 32,
+                                   ((cw >> 12) & 63) - 32, (cw >> 18) & 63));
 }
 
 // Evaluate op `op` (its first four condition words in cw) at base cell
 // (bx, by, bz) on the current lattice and append it to its owner's queue.
 // Ops on one process sit in different rounds, so issuing the rounds in
 // order keeps the reference order within every queue.
-template <int PPL, int BITS>
+template <int PPL, int BITS, bool IN>
 __device__ __forceinline__ void eval_post(Wave<PPL, BITS> &W, bool valid, int2 op, const int (&cw)[4],
-                                          const int *cd, bool as_add, int nrounds, bool in, int lb, int bx,
-                                          int by, int bz)
+                                          const int *cd, bool as_add, int nrounds, int lb, int bx, int by, int bz)
 {
     int proc = 0, code = 0, s = 0, rnd = 0;
     if (valid) {
@@ -267,7 +265,7 @@ __device__ __forceinline__ void eval_post(Wave<PPL, BITS> &W, bool valid, int2 o
         const int arg = (int)((unsigned)op.x >> 23);
         const int dx = sext7(op.y & 0x7F), dy = sext7((op.y >> 7) & 0x7F), dz = sext7((op.y >> 14) & 0x7F);
         if (kind == OP_DEL_IF || kind == OP_ADD_IF) {
-            s = site_from(W.d, in, lb, bx, by, bz, dx, dy, dz, n);
+            s = site_from<IN>(W.d, lb, bx, by, bz, dx, dy, dz, n);
             if (kind == OP_DEL_IF) {
                 proc = arg;
                 code = MB_DEL_IF;
@@ -276,15 +274,15 @@ __device__ __forceinline__ void eval_post(Wave<PPL, BITS> &W, bool valid, int2 o
                 int q[4];
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
-                    q[c] = cond_species<PPL, BITS>(W, (c < nc) ? cw[c] : (32 | 32 << 6 | 32 << 12 | 1 << 18),
-                                                   in, lb, bx, by, bz);
+                    q[c] = cond_species<PPL, BITS, IN>(W, (c < nc) ? cw[c] : (32 | 32 << 6 | 32 << 12 | 1 << 18),
+                                                       lb, bx, by, bz);
                 bool ok = true;
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
                     ok &= (c >= nc) || (q[c] == ((cw[c] >> 24) & 0xFF));
                 for (int c = 4; c < nc; ++c) {
                     const int w = cd[c];
-                    ok &= cond_species<PPL, BITS>(W, w, in, lb, bx, by, bz) == ((w >> 24) & 0xFF);
+                    ok &= cond_species<PPL, BITS, IN>(W, w, lb, bx, by, bz) == ((w >> 24) & 0xFF);
                 }
                 if (ok) {
                     proc = arg;
@@ -296,7 +294,7 @@ __device__ __forceinline__ void eval_post(Wave<PPL, BITS> &W, bool valid, int2 o
             const int cy = W.d.dim >= 2 ? wrap1(by + dy, W.d.Y) : 0;
             const int cz = W.d.dim >= 3 ? wrap1(bz + dz, W.d.Z) : 0;
             s = W.d.lin(cx, cy, cz);
-            proc = nli_eval<PPL, BITS>(W, arg, in, cx, cy, cz);
+            proc = nli_eval<PPL, BITS, IN>(W, arg, cx, cy, cz);
             code = (kind == OP_DEL_NLI) ? MB_DEL_STRICT : MB_ADD;
         }
     }
@@ -347,8 +345,8 @@ __device__ __forceinline__ void load_op(const int *prog, int ooff, int cbase, in
 // the writes (they do not depend on the lattice).  Wave-uniform call.  On
 // a species mismatch the writes before the failing one are done and
 // nothing is posted.
-template <int PPL, int BITS>
-__device__ __forceinline__ int post_segment(Wave<PPL, BITS> &W, int seg, int bx, int by, int bz)
+template <int PPL, int BITS, bool IN>
+__device__ __forceinline__ int post_segment_t(Wave<PPL, BITS> &W, int seg, int bx, int by, int bz)
 {
     const int *prog = W.prog;
     const int *sg = prog + W.off_seg + SEG_W * seg;
@@ -360,7 +358,6 @@ __device__ __forceinline__ int post_segment(Wave<PPL, BITS> &W, int seg, int bx,
     int cw[4];
     const int *cd;
     load_op(prog, ooff, cbase, W.lane, W.lane < nops, op, cw, cd);
-    const bool in = W.d.interior(bx, by, bz);
     const int lb = W.d.lin(bx, by, bz);
 
     if (nw > 0) {
@@ -371,8 +368,8 @@ __device__ __forceinline__ int post_segment(Wave<PPL, BITS> &W, int seg, int bx,
             const int w = w0i + W.lane;
             if (w < nw) {
                 const int w0 = prog[woff + WR_W * w], w1 = prog[woff + WR_W * w + 1];
-                s = site_from(W.d, in, lb, bx, by, bz, unpack_x(w0), unpack_y(w0), unpack_z(w0),
-                              (w0 >> 24) & 0xFF);
+                s = site_from<IN>(W.d, lb, bx, by, bz, unpack_x(w0), unpack_y(w0), unpack_z(w0),
+                                  (w0 >> 24) & 0xFF);
                 cur = W.lat.get(s);
                 nsp = w1 >> 16;
                 bad = cur < 0 || !((w1 >> cur) & 1);
@@ -389,14 +386,23 @@ __device__ __forceinline__ int post_segment(Wave<PPL, BITS> &W, int seg, int bx,
         wave_sync();
     }
 
-    eval_post<PPL, BITS>(W, W.lane < nops, op, cw, cd, as_add, nrounds, in, lb, bx, by, bz);
+    eval_post<PPL, BITS, IN>(W, W.lane < nops, op, cw, cd, as_add, nrounds, lb, bx, by, bz);
     for (int base = 64; base < nops; base += 64) {
         const int j = base + W.lane;
         load_op(prog, ooff, cbase, j, j < nops, op, cw, cd);
-        eval_post<PPL, BITS>(W, j < nops, op, cw, cd, as_add, nrounds, in, lb, bx, by, bz);
+        eval_post<PPL, BITS, IN>(W, j < nops, op, cw, cd, as_add, nrounds, lb, bx, by, bz);
     }
     wave_sync();
     return ST_OK;
+}
+
+// base cells away from the faces take the path without wrapping
+template <int PPL, int BITS>
+__device__ __forceinline__ int post_segment(Wave<PPL, BITS> &W, int seg, int bx, int by, int bz)
+{
+    if (W.d.interior(bx, by, bz))
+        return post_segment_t<PPL, BITS, true>(W, seg, bx, by, bz);
+    return post_segment_t<PPL, BITS, false>(W, seg, bx, by, bz);
 }
 
 // Execute one segment (a touchup) at base cell (bx, by, bz).
