@@ -154,6 +154,18 @@ int kmcg_get_counters(kmcg_batch *b, int32_t r0, int32_t n, int64_t *procstat, d
 #define KMCG_SAMPLE_WIDTH(S, spuck, P) (4 + (S) * (spuck) + 2 * (P))
 int kmcg_sample(kmcg_batch *b, const int32_t *replicas, int32_t n, double *out);
 
+/* kmcg_run followed, on the same stream, by kmcg_sample of the same
+ * replicas: one launch and one wait per sampling interval.  The records
+ * (KMCG_SAMPLE_WIDTH doubles per replica, list order) and the budgets are
+ * written to `samples` / `budget` when kmcg_sync (or any other call on the
+ * batch) waits for the run; both must stay valid until then. */
+int kmcg_run_sampled(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps,
+                     int64_t *budget, double *samples);
+
+/* *done = 1 when the batch's stream has no queued work left (kmcg_sync would
+ * not wait), else 0; never waits. */
+int kmcg_query(kmcg_batch *b, int32_t *done);
+
 #ifdef __cplusplus
 }
 #endif

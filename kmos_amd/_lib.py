@@ -50,6 +50,8 @@ SIGNATURES = [
     ("kmcg_get_occupation", _i32, [_vp, _i32, _i32, _vp]),
     ("kmcg_get_counters", _i32, [_vp, _i32, _i32, _vp, _vp]),
     ("kmcg_sample", _i32, [_vp, _vp, _i32, _vp]),
+    ("kmcg_run_sampled", _i32, [_vp, _vp, _i32, _i64, _vp, _vp]),
+    ("kmcg_query", _i32, [_vp, _vp]),
 ]
 
 _lib = None

@@ -1104,6 +1104,9 @@ struct kmcg_batch {
     int *h_ids = nullptr;
     long long *h_budget = nullptr;
     int64_t *user_budget = nullptr;    // where the budgets go back at the sync
+    double *h_samples = nullptr;        // pinned staging of sample records
+    double *user_samples = nullptr;     // where the records go at the sync
+    size_t n_samples = 0;
     int n_last = 0;
     float last_ms = 0.f;
     bool timed_run_pending = false;   // ev0/ev1 bracket a launch not yet read
@@ -1668,6 +1671,10 @@ int settle(kmcg_batch *b)
         memcpy(b->user_budget, b->h_budget, b->n_last * sizeof(long long));
         b->user_budget = nullptr;
     }
+    if (b->user_samples) {
+        memcpy(b->user_samples, b->h_samples, b->n_samples * sizeof(double));
+        b->user_samples = nullptr;
+    }
     return KMCG_OK;
 }
 
@@ -1793,6 +1800,8 @@ void kmcg_batch_free(kmcg_batch *b)
         (void)hipHostFree(b->h_ids);
     if (b->h_budget)
         (void)hipHostFree(b->h_budget);
+    if (b->h_samples)
+        (void)hipHostFree(b->h_samples);
     if (b->ev0)
         (void)hipEventDestroy(b->ev0);
     if (b->ev1)
@@ -1875,7 +1884,9 @@ int kmcg_batch_create(const kmcg_model *m, int32_t n, const int32_t *sizes, int3
     BALLOC(b->occ, (size_t)n * m->S * m->spuck);
     BALLOC(b->samples, (size_t)n * (4 + m->S * m->spuck + 2 * P));
     if (hipHostMalloc((void **)&b->h_ids, n * sizeof(int), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&b->h_budget, n * sizeof(long long), hipHostMallocDefault) != hipSuccess)
+        hipHostMalloc((void **)&b->h_budget, n * sizeof(long long), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_samples, (size_t)n * (4 + m->S * m->spuck + 2 * P) * sizeof(double),
+                      hipHostMallocDefault) != hipSuccess)
         return bail(fail(KMCG_ENOMEM, "pinned host allocation failed"));
 #undef BALLOC
     auto H2D = [&](void *d, const void *h, size_t bytes) {
@@ -2052,6 +2063,47 @@ int kmcg_run(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps, i
         b->user_budget = budget;
         b->n_last = n;
     }
+    return KMCG_OK;
+}
+
+int kmcg_run_sampled(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps, int64_t *budget,
+                     double *samples)
+{
+    if (!samples)
+        return fail(KMCG_EINVAL, "null sample buffer");
+    if (b && b->m->S * b->m->spuck > 8 * 32)
+        return fail(KMCG_EUNSUPPORTED, "too many species x sites for occupation");
+    int rc = kmcg_run(b, replicas, n, steps, budget);
+    if (rc)
+        return rc;
+    // the run left the replica list in rep_ids; the records follow it on the
+    // same stream
+    const int width = 4 + b->m->S * b->m->spuck + 2 * b->m->P;
+    KParams kp = make_params(b);
+    if (b->bits == 2)
+        hipLaunchKernelGGL((kmc_sample_kernel<2>), dim3(n), dim3(256), 0, b->stream, kp, b->samples, width);
+    else
+        hipLaunchKernelGGL((kmc_sample_kernel<8>), dim3(n), dim3(256), 0, b->stream, kp, b->samples, width);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(b->h_samples, b->samples, (size_t)n * width * sizeof(double), hipMemcpyDeviceToHost,
+                          b->stream));
+    b->user_samples = samples;
+    b->n_samples = (size_t)n * width;
+    return KMCG_OK;
+}
+
+int kmcg_query(kmcg_batch *b, int32_t *done)
+{
+    if (!b || !done)
+        return fail(KMCG_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(b->device));
+    const hipError_t e = hipStreamQuery(b->stream);
+    if (e == hipErrorNotReady) {
+        *done = 0;
+        return KMCG_OK;
+    }
+    HIPCHK(e);
+    *done = 1;
     return KMCG_OK;
 }
 

@@ -137,6 +137,12 @@ class Batch:
             self.sync()
         return bud
 
+    def idle(self) -> bool:
+        """True when no queued work is left on the batch's stream."""
+        d = ctypes.c_int32(0)
+        _lib.check(_lib.load().kmcg_query(self.h, ctypes.byref(d)))
+        return bool(d.value)
+
     def sync(self):
         _lib.check(_lib.load().kmcg_sync(self.h))
 
@@ -219,15 +225,8 @@ class Batch:
         _lib.check(_lib.load().kmcg_get_occupation(self.h, r0, n, _ptr(out)))
         return out
 
-    def sample(self, replicas):
-        """One record per replica: dict of arrays kmc_time[n], kmc_time_step[n],
-        kmc_step[n], status[n], occupation[n, S, spuck], integ_rates[n, P],
-        procstat[n, P] (one device gather, one copy)."""
-        ids = np.ascontiguousarray(np.atleast_1d(replicas), dtype=np.int32)
+    def _records(self, out):
         S, N, P = self.S, self.spuck, self.P
-        w = 4 + S * N + 2 * P
-        out = np.zeros((ids.size, w), np.float64)
-        _lib.check(_lib.load().kmcg_sample(self.h, _ptr(ids), ids.size, _ptr(out)))
         return {
             "kmc_time": out[:, 0].copy(),
             "kmc_time_step": out[:, 1].copy(),
@@ -237,6 +236,29 @@ class Batch:
             "integ_rates": out[:, 4 + S * N:4 + S * N + P].copy(),
             "procstat": out[:, 4 + S * N + P:].astype(np.int64),
         }
+
+    def sample(self, replicas):
+        """One record per replica: dict of arrays kmc_time[n], kmc_time_step[n],
+        kmc_step[n], status[n], occupation[n, S, spuck], integ_rates[n, P],
+        procstat[n, P] (one device gather, one copy)."""
+        ids = np.ascontiguousarray(np.atleast_1d(replicas), dtype=np.int32)
+        out = np.zeros((ids.size, 4 + self.S * self.spuck + 2 * self.P), np.float64)
+        _lib.check(_lib.load().kmcg_sample(self.h, _ptr(ids), ids.size, _ptr(out)))
+        return self._records(out)
+
+    def run_sampled(self, steps: int, replicas, budget=None):
+        """run() then sample() of the same replicas, queued together and not
+        waited for: returns a function that waits and gives (budget, records)."""
+        ids = np.ascontiguousarray(np.atleast_1d(replicas), dtype=np.int32)
+        bud = None if budget is None else np.array(budget, dtype=np.int64)
+        out = np.zeros((ids.size, 4 + self.S * self.spuck + 2 * self.P), np.float64)
+        self._ids, self._bud, self._out = ids, bud, out   # written at the next sync
+        _lib.check(_lib.load().kmcg_run_sampled(self.h, _ptr(ids), ids.size, int(steps), _ptr(bud), _ptr(out)))
+
+        def wait():
+            self.sync()
+            return bud, self._records(out)
+        return wait
 
     def counters(self, r0: int = 0, n: Optional[int] = None):
         n = self.n - r0 if n is None else n

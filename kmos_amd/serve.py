@@ -5,16 +5,16 @@ constants, a seed, a step budget and a sampling interval (the quantities a
 ``KMC_Model`` user sets before ``get_std_sampled_data``, kmos/run/__init__.py:
 865-966).  The server keeps every admitted trajectory resident in HBM as a
 *slot* of a *page* (an engine ``Batch`` of equal-sized replicas of one
-model) and advances all admitted slots together, one sampling interval per
-tick:
+model) and advances every page by one sampling interval at a time, each page at
+its own pace:
 
-  tick:  admit arrived requests into free slots (new pages while the HBM
+  loop:  admit arrived requests into free slots (new pages while the HBM
          budget allows), initialise them on the device
-         -> launch every page's run for its active slots (each page has its
-            own stream, so the pages' kernels overlap on the GPU)
-         -> wait, gather one sample record per active slot (one device
-            gather and one copy per page)
-         -> retire finished slots, whose requests complete now
+         -> every idle page with active slots queues its next interval: one
+            run and one device gather of a sample record per active slot
+            (each page has its own stream, so pages overlap on the GPU)
+         -> pages whose interval finished hand over their records; finished
+            slots retire and their requests complete now
 
 Requests that arrive while others run join at the next tick, and a slot
 freed by a finished request is reused at once: this is continuous batching,
@@ -113,10 +113,12 @@ class Server:
     def _page_slots(self, key) -> int:
         return int(max(1, min(self.max_slots, self.page_bytes // max(1, self._slot_bytes(key)))))
 
-    def _try_admit(self, req: Request, now: float) -> bool:
+    def _try_admit(self, req: Request, now: float, busy=()) -> bool:
         key = (req.model, tuple(int(v) for v in req.size))
         pages = self.pages.setdefault(key, [])
-        page = next((p for p in pages if p.free), None)
+        page = next((p for p in pages if p.free and id(p) not in busy), None)
+        if page is None and any(p.free for p in pages):
+            return False        # a page with room is mid-interval: join it next
         if page is None:
             slots = self._page_slots(key)
             need = self.models[req.model].footprint([list(key[1])] * slots)
@@ -161,10 +163,12 @@ class Server:
         pending = sorted(requests, key=lambda r: r.arrival)
         waiting: List[Request] = []
         results: Dict[int, Result] = {}
+        inflight: Dict[int, tuple] = {}      # id(page) -> (page, ids, wait)
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0  # noqa: E731
         i = 0
-        while i < len(pending) or waiting or any(p.active for ps in self.pages.values() for p in ps):
+        while i < len(pending) or waiting or inflight or \
+                any(p.active for ps in self.pages.values() for p in ps):
             t = now()
             while i < len(pending) and (not realtime or pending[i].arrival <= t):
                 waiting.append(pending[i])
@@ -173,38 +177,46 @@ class Server:
             # pool has no room while others still fit
             still = []
             for req in waiting:
-                if not self._try_admit(req, now() if realtime else 0.0):
+                if not self._try_admit(req, now() if realtime else 0.0, inflight):
                     still.append(req)
-            if still and len(still) == len(waiting):
+            if still and len(still) == len(waiting) and not inflight:
                 self._release_idle_pages()
                 still = [r for r in still if not self._try_admit(r, now() if realtime else 0.0)]
             waiting = still
-            live = [p for ps in self.pages.values() for p in ps if p.active]
-            if not live:
+            # every page with active slots and no run in flight starts its
+            # next sampling interval; pages progress independently and
+            # overlap on the device (one stream each)
+            for ps in self.pages.values():
+                for p in ps:
+                    if p.active and id(p) not in inflight:
+                        ids = np.array(sorted(p.active), np.int32)
+                        budget = np.array([min(p.active[s].req.sample_every, p.active[s].left) for s in ids],
+                                          np.int64)
+                        wait = p.batch.run_sampled(int(budget.max()), ids, budget)
+                        inflight[id(p)] = (p, ids, wait)
+                        self.stats["ticks"] += 1
+            active = sum(len(p.active) for ps in self.pages.values() for p in ps)
+            self.stats["peak_active"] = max(self.stats["peak_active"], active)
+            if not inflight:
                 if i < len(pending) and realtime:
                     time.sleep(max(0.0, min(pending[i].arrival - now(), 0.01)))
                 elif waiting:
                     raise RuntimeError("request %d does not fit the HBM budget" % waiting[0].rid)
                 continue
-            # launch every page, then collect (pages overlap on the device)
-            launched = []
-            for p in live:
-                ids = np.array(sorted(p.active), np.int32)
-                budget = np.array([min(p.active[s].req.sample_every, p.active[s].left) for s in ids], np.int64)
-                p.batch.run(int(budget.max()), replicas=ids, budget=budget, sync=False)
-                launched.append((p, ids, budget.copy()))
-            self.stats["ticks"] += 1
-            self.stats["peak_active"] = max(self.stats["peak_active"], sum(len(p.active) for p in live))
-            for p, ids, asked in launched:
-                p.batch.sync()
-                rec = p.batch.sample(ids)
+            finished = [k for k, (p, _, _) in inflight.items() if p.batch.idle()]
+            if not finished:
+                time.sleep(50e-6)
+                continue
+            for k in finished:
+                p, ids, wait = inflight.pop(k)
+                _, rec = wait()
                 tdone = now()
                 for j, s in enumerate(ids):
                     f = p.active[s]
                     ran = int(rec["kmc_step"][j]) - (f.req.steps - f.left)
                     f.left -= ran
                     f.status = int(rec["status"][j])
-                    f.records.append({k: v[j] for k, v in rec.items()})
+                    f.records.append({key: v[j] for key, v in rec.items()})
                     if f.left <= 0 or f.status != 0:
                         results[f.req.rid] = _finish(f, tdone)
                         del p.active[s]

@@ -172,3 +172,28 @@ def test_sample_records_and_async_budgets(engine, oracle):
         o = oracle_replica(oracle, cm, sizes[r], r + 1, 0, rates[r])
         o.run([100, 700, 700][r])
         compare_state(b2, r, o, where="async batch r%d" % r)
+
+
+def test_run_sampled_equals_run_then_sample(engine):
+    """kmcg_run_sampled gives the records kmcg_run + kmcg_sample give, and
+    kmcg_query reports the stream idle after the sync."""
+    pt = W.zgb_project()
+    cm = compile_model(pt, "local_smart")
+    m = engine.Model(cm)
+    rng = np.random.default_rng(9)
+    rates = 10.0 ** rng.uniform(-1, 3, (4, cm.nr_of_proc))
+    a = engine.Batch(m, [[16, 16, 1]] * 4)
+    b = engine.Batch(m, [[16, 16, 1]] * 4)
+    for x in (a, b):
+        x.init(np.arange(4, dtype=np.uint32) + 5)
+        x.set_rates(rates)
+    ids = np.array([3, 1], np.int32)
+    wait = a.run_sampled(400, ids, budget=[400, 250])
+    bud_a, rec_a = wait()
+    assert a.idle()
+    bud_b = b.run(400, replicas=ids, budget=[400, 250])
+    rec_b = b.sample(ids)
+    np.testing.assert_array_equal(bud_a, bud_b)
+    for k in rec_a:
+        np.testing.assert_array_equal(rec_a[k], rec_b[k], err_msg=k)
+    assert list(rec_a["kmc_step"]) == [400, 250]
