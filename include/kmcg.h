@@ -166,6 +166,18 @@ int kmcg_run_sampled(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t 
  * not wait), else 0; never waits. */
 int kmcg_query(kmcg_batch *b, int32_t *done);
 
+/* kmcg_run that also records, inside the step loop, one sample record
+ * (KMCG_SAMPLE_WIDTH doubles, the layout of kmcg_sample) after every
+ * `sample_every` steps of each listed replica and one for its final state:
+ * record j of replicas[i] at records[(i * max_records + j) * width], for
+ * j < ceil(executed / sample_every) (one record if it executed no step).
+ * steps <= sample_every * max_records.  This is what
+ * KMC_Model.get_std_sampled_data reads per sample (kmos/run/__init__.py:
+ * 865-966), without a host round trip per sample.  `records` and `budget`
+ * are written when kmcg_sync (or any other call on the batch) waits. */
+int kmcg_run_recorded(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps,
+                      int64_t *budget, int32_t sample_every, int32_t max_records, double *records);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,7 @@ SIGNATURES = [
     ("kmcg_sample", _i32, [_vp, _vp, _i32, _vp]),
     ("kmcg_run_sampled", _i32, [_vp, _vp, _i32, _i64, _vp, _vp]),
     ("kmcg_query", _i32, [_vp, _vp]),
+    ("kmcg_run_recorded", _i32, [_vp, _vp, _i32, _i64, _vp, _i32, _i32, _vp]),
 ]
 
 _lib = None

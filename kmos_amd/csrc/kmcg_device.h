@@ -119,6 +119,8 @@ struct KParams {
     int mrows;                     // mailbox rows per wave
     int merged;                    // 1: the segments of a step share one apply
     int off_seg, off_proc, off_touch, off_nli;
+    double *srec;                  // in-loop sample records (or null)
+    int sample_every, srec_cap, srec_width;
     int mode;                      // misc kernel mode
     int arg0, arg1, arg2, arg3;    // misc kernel arguments
 };

@@ -48,6 +48,8 @@ struct Wave {
     int *mbox;         // queues [mrows][64*PPL]: entry j of process p at j*64*PPL + p
     int *cnt;          // [64*PPL] queue lengths
     int *win;          // [64][WIN] list-tail window of each lane (replay)
+    int *occ;          // [S][spuck] species counts, kept while `track`
+    bool track;
     int *avail;        // sites[P][V+1] then back[P][V+1]
     long long V1;
     int P, lane, mstride, mrows;
@@ -376,8 +378,14 @@ __device__ __forceinline__ int post_segment_t(Wave<PPL, BITS> &W, int seg, int b
             }
             const unsigned long long bm = __ballot(w < nw && bad);
             const int nok = bm ? __ffsll((long long)bm) - 1 : 64;
-            if (w < nw && W.lane < nok)
+            if (w < nw && W.lane < nok) {
                 W.lat.xset(s, cur, nsp);
+                if (W.track) {
+                    const int t = prog[woff + WR_W * w] >> 24 & 0xFF;   // site type n
+                    atomicAdd(&W.occ[cur * W.d.spuck + t - 1], -1);
+                    atomicAdd(&W.occ[nsp * W.d.spuck + t - 1], 1);
+                }
+            }
             if (bm) {
                 wave_sync();
                 return ST_MISMATCH;
@@ -577,6 +585,8 @@ __device__ __forceinline__ int wave_setup(Wave<PPL, BITS> &W, const KParams &kp,
     W.mrows = kp.mrows;
     W.cnt = W.mbox + kp.mrows * W.mstride;
     W.win = W.cnt + W.mstride;
+    W.occ = W.win + 64 * WIN;
+    W.track = false;
     for (int i = W.lane; i < W.mstride; i += 64)
         W.cnt[i] = 0;
 #pragma unroll
@@ -619,6 +629,36 @@ __device__ __forceinline__ void wave_store(Wave<PPL, BITS> &W, const KParams &kp
     }
 }
 
+// One sample record (the layout of kmc_sample_kernel) from the wave's
+// registers and its species counts.
+template <int PPL, int BITS>
+__device__ __forceinline__ void write_record(const Wave<PPL, BITS> &W, const KParams &kp, int slot, long long idx,
+                                             double t, double dt, long long ks, int st)
+{
+    if (idx < 0 || idx >= kp.srec_cap)
+        return;
+    wave_sync();
+    double *rec = kp.srec + ((long long)slot * kp.srec_cap + idx) * kp.srec_width;
+    const int no = kp.S * W.d.spuck, P = W.P;
+    const double cells = (double)W.d.X * W.d.Y * W.d.Z;
+    for (int i = W.lane; i < no; i += 64)
+        rec[4 + i] = (double)W.occ[i] / cells;
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        const int p = W.lane + 64 * k;
+        if (p < P) {
+            rec[4 + no + p] = W.integ[k];
+            rec[4 + no + P + p] = (double)W.pstat[k];
+        }
+    }
+    if (W.lane == 0) {
+        rec[0] = t;
+        rec[1] = dt;
+        rec[2] = (double)ks;
+        rec[3] = (double)st;
+    }
+}
+
 // do_kmc_steps for one replica per wave (proclist_generic_subroutines.mpy:1-44)
 template <int PPL, int BITS, bool LDS_LAT>
 __global__ __launch_bounds__(64 * MAX_WPG) void kmc_steps_kernel(KParams kp)
@@ -630,19 +670,32 @@ __global__ __launch_bounds__(64 * MAX_WPG) void kmc_steps_kernel(KParams kp)
         return;
     const int slot = blockIdx.x * kp.wpg + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = W.lane, P = W.P;
-    if (kp.status[r] != ST_OK) {
-        if (lane == 0)
-            kp.executed[slot] = 0;
-        return;
-    }
+    // a replica that has stopped takes no step (and still gives its record)
+    const int st0 = kp.status[r];
     long long n = kp.nsteps;
     if (kp.budget)
         n = min(n, kp.budget[slot]);
+    if (st0 != ST_OK)
+        n = 0;
+    if (kp.srec) {
+        // species counts per site type, then kept up to date by the writes
+        W.track = true;
+        const int no = kp.S * W.d.spuck;
+        for (int i = lane; i < no; i += 64)
+            W.occ[i] = 0;
+        wave_sync();
+        for (int s = lane; s < W.d.V; s += 64) {
+            const int q = W.lat.get(s);
+            if (q >= 0)
+                atomicAdd(&W.occ[q * W.d.spuck + (s - divm(s, W.ms) * W.d.spuck)], 1);
+        }
+        wave_sync();
+    }
     double t = kp.ktime[r], dt_last = kp.kdt[r];
     long long ks = kp.kstep[r];
     unsigned long long ctr = kp.rngctr[r];
     const unsigned seed = kp.seed[r], stream = kp.stream[r];
-    int st = ST_OK;
+    int st = st0;
     long long done = 0;
     // accum_rates stay in registers across steps; `first` is the lowest
     // lane per slot whose term changed (64 = none), everything at the start
@@ -730,7 +783,11 @@ __global__ __launch_bounds__(64 * MAX_WPG) void kmc_steps_kernel(KParams kp)
             const unsigned long long ch = __ballot(W.nr[k] != nr_before[k]);
             first[k] = ch ? __ffsll((long long)ch) - 1 : 64;
         }
+        if (W.track && done % kp.sample_every == 0)
+            write_record<PPL, BITS>(W, kp, slot, done / kp.sample_every - 1, t, dt_last, ks, ST_OK);
     }
+    if (W.track)
+        write_record<PPL, BITS>(W, kp, slot, done == 0 ? 0 : (done - 1) / kp.sample_every, t, dt_last, ks, st);
     wave_store<PPL, BITS, LDS_LAT>(W, kp, r, true);
     if (lane == 0) {
         kp.ktime[r] = t;
@@ -1104,6 +1161,11 @@ struct kmcg_batch {
     int *h_ids = nullptr;
     long long *h_budget = nullptr;
     int64_t *user_budget = nullptr;    // where the budgets go back at the sync
+    double *srec = nullptr;             // in-loop records on the device
+    double *h_srec = nullptr;           // and their pinned staging
+    size_t srec_len = 0;                // doubles in each
+    double *user_srec = nullptr;
+    size_t n_srec = 0;
     double *h_samples = nullptr;        // pinned staging of sample records
     double *user_samples = nullptr;     // where the records go at the sync
     size_t n_samples = 0;
@@ -1578,7 +1640,8 @@ Shape launch_shape(kmcg_batch *b, const int *ids, int n, KParams &kp, int max_wp
     long long maxlb = 0;
     for (int i = 0; i < n; ++i)
         maxlb = std::max(maxlb, lattice_bytes(b->V[ids[i]], b->bits));
-    const size_t mbox = ((size_t)(b->m->mrows + 1) * 64 * ppl_of(b->m->P) + 64 * WIN) * sizeof(int);
+    const size_t occ = ((size_t)b->m->S * b->m->spuck + 3) & ~(size_t)3;
+    const size_t mbox = ((size_t)(b->m->mrows + 1) * 64 * ppl_of(b->m->P) + 64 * WIN + occ) * sizeof(int);
     const size_t prog = (size_t)kp.prog_lds;
     Shape s;
     s.lds_lat = b->lds_mode == 0 ? false : (prog + (size_t)maxlb + mbox <= (size_t)b->max_lds);
@@ -1674,6 +1737,10 @@ int settle(kmcg_batch *b)
     if (b->user_samples) {
         memcpy(b->user_samples, b->h_samples, b->n_samples * sizeof(double));
         b->user_samples = nullptr;
+    }
+    if (b->user_srec) {
+        memcpy(b->user_srec, b->h_srec, b->n_srec * sizeof(double));
+        b->user_srec = nullptr;
     }
     return KMCG_OK;
 }
@@ -1802,6 +1869,10 @@ void kmcg_batch_free(kmcg_batch *b)
         (void)hipHostFree(b->h_budget);
     if (b->h_samples)
         (void)hipHostFree(b->h_samples);
+    if (b->h_srec)
+        (void)hipHostFree(b->h_srec);
+    if (b->srec)
+        (void)hipFree(b->srec);
     if (b->ev0)
         (void)hipEventDestroy(b->ev0);
     if (b->ev1)
@@ -2034,7 +2105,8 @@ int kmcg_set_config(kmcg_batch *b, int32_t r0, int32_t n, const int8_t *config)
     return run_misc(b, ids, MODE_ADJUST, 0, 0, 0, 0);
 }
 
-int kmcg_run(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps, int64_t *budget)
+static int run_impl(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps, int64_t *budget,
+                    int sample_every, int max_records, double *records)
 {
     if (!b || !replicas || n < 1 || n > b->n || steps < 0)
         return fail(KMCG_EINVAL, "bad run arguments");
@@ -2054,6 +2126,27 @@ int kmcg_run(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps, i
     const Shape s = launch_shape(b, replicas, n, kp, b->max_wpg);
     kp.nsteps = steps;
     kp.budget = budget ? b->budget : nullptr;
+    const int width = 4 + b->m->S * b->m->spuck + 2 * b->m->P;
+    if (records) {
+        const size_t need = (size_t)n * max_records * width;
+        if (need > b->srec_len) {
+            if (b->srec)
+                HIPCHK(hipFree(b->srec));
+            if (b->h_srec)
+                HIPCHK(hipHostFree(b->h_srec));
+            b->srec = nullptr;
+            b->h_srec = nullptr;
+            b->srec_len = 0;
+            if (hipMalloc((void **)&b->srec, need * sizeof(double)) != hipSuccess ||
+                hipHostMalloc((void **)&b->h_srec, need * sizeof(double), hipHostMallocDefault) != hipSuccess)
+                return fail(KMCG_ENOMEM, "record buffers");
+            b->srec_len = need;
+        }
+        kp.srec = b->srec;
+        kp.sample_every = sample_every;
+        kp.srec_cap = max_records;
+        kp.srec_width = width;
+    }
     HIPCHK(hipEventRecord(b->ev0, b->stream));
     HIPCHK(launch(b, true, s, kp));
     HIPCHK(hipEventRecord(b->ev1, b->stream));
@@ -2063,8 +2156,31 @@ int kmcg_run(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps, i
         b->user_budget = budget;
         b->n_last = n;
     }
+    if (records) {
+        const size_t cnt = (size_t)n * max_records * width;
+        HIPCHK(hipMemcpyAsync(b->h_srec, b->srec, cnt * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+        b->user_srec = records;
+        b->n_srec = cnt;
+    }
     return KMCG_OK;
 }
+
+int kmcg_run(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps, int64_t *budget)
+{
+    return run_impl(b, replicas, n, steps, budget, 0, 0, nullptr);
+}
+
+int kmcg_run_recorded(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps, int64_t *budget,
+                      int32_t sample_every, int32_t max_records, double *records)
+{
+    if (!b || !records || sample_every < 1 || max_records < 1 ||
+        steps > (int64_t)sample_every * max_records)
+        return fail(KMCG_EINVAL, "bad record arguments (steps must fit max_records intervals)");
+    if (b->m->S * b->m->spuck > 8 * 32)
+        return fail(KMCG_EUNSUPPORTED, "too many species x sites for occupation");
+    return run_impl(b, replicas, n, steps, budget, sample_every, max_records, records);
+}
+
 
 int kmcg_run_sampled(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps, int64_t *budget,
                      double *samples)

@@ -260,6 +260,29 @@ class Batch:
             return bud, self._records(out)
         return wait
 
+    def run_recorded(self, steps: int, replicas, budget, sample_every: int, max_records: int):
+        """run() that records a sample every `sample_every` steps inside the
+        kernel; not waited for: returns a function that waits and gives
+        (budget, [records of replica i: dict of arrays over its samples])."""
+        ids = np.ascontiguousarray(np.atleast_1d(replicas), dtype=np.int32)
+        bud = np.array(budget, dtype=np.int64)
+        before = bud.copy()
+        w = 4 + self.S * self.spuck + 2 * self.P
+        out = np.zeros((ids.size, max_records, w), np.float64)
+        self._ids, self._bud, self._rec = ids, bud, out   # written at the next sync
+        _lib.check(_lib.load().kmcg_run_recorded(self.h, _ptr(ids), ids.size, int(steps), _ptr(bud),
+                                                 int(sample_every), int(max_records), _ptr(out)))
+
+        def wait():
+            self.sync()
+            done = before - bud
+            recs = []
+            for i in range(ids.size):
+                k = max(1, -(-int(done[i]) // sample_every))
+                recs.append(self._records(out[i, :k]))
+            return bud, recs
+        return wait
+
     def counters(self, r0: int = 0, n: Optional[int] = None):
         n = self.n - r0 if n is None else n
         ps = np.zeros((n, self.P), np.int64)

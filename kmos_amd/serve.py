@@ -68,12 +68,28 @@ class Result:
 
 
 class _Page:
-    def __init__(self, model: engine.Model, size, slots: int, device: int):
-        self.batch = engine.Batch(model, [list(size)] * slots, device=device)
-        self.slots = slots
-        self.free: List[int] = list(range(slots - 1, -1, -1))
+    """One engine batch of one model: `per_size` slots of each lattice size,
+    so that a single launch advances every admitted trajectory of the model
+    whatever its size (one stream per model keeps the pages' kernels on
+    separate hardware queues)."""
+
+    def __init__(self, model: engine.Model, sizes, per_size: int, device: int):
+        layout = [list(sz) for sz in sizes for _ in range(per_size)]
+        self.batch = engine.Batch(model, layout, device=device)
+        self.slots = len(layout)
+        self.free: Dict[tuple, List[int]] = {}
+        for i, sz in enumerate(layout):
+            self.free.setdefault(tuple(sz), []).append(i)
+        for v in self.free.values():
+            v.reverse()
         self.active: Dict[int, "_Flight"] = {}
-        self.bytes = model.footprint([list(size)] * slots)
+        self.bytes = model.footprint(layout)
+
+    def has_room(self, size) -> bool:
+        return bool(self.free.get(size))
+
+    def n_free(self) -> int:
+        return sum(len(v) for v in self.free.values())
 
 
 class _Flight:
@@ -90,7 +106,7 @@ class Server:
     key to a CompiledModel (default: the quickstart, ZGB and RuO2 models)."""
 
     def __init__(self, models: Optional[Dict[str, CompiledModel]] = None, hbm_budget: int = 64 << 30,
-                 page_bytes: int = 256 << 20, max_slots_per_page: int = 256, device: int = 0):
+                 sizes=None, slots_per_size: int = 64, device: int = 0, intervals_per_launch: int = 4):
         if models is None:
             models = {"quickstart": compile_model(W.quickstart_project(), "local_smart"),
                       "zgb": compile_model(W.zgb_project(), "local_smart"),
@@ -98,42 +114,39 @@ class Server:
         self.compiled = models
         self.models = {k: engine.Model(cm) for k, cm in models.items()}
         self.hbm_budget = int(hbm_budget)
-        self.page_bytes = int(page_bytes)
-        self.max_slots = int(max_slots_per_page)
+        # the lattice sizes a page holds slots for (others get pages of their own)
+        self.sizes = [tuple(int(v) for v in (list(sz) + [1, 1])[:3]) for sz in
+                      (sizes if sizes is not None else [(n, n, 1) for n in SIZES])]
+        self.per_size = int(slots_per_size)
+        # sampling intervals one launch covers (records made in the kernel):
+        # more means fewer host round trips, fewer means quicker admission
+        self.intervals_per_launch = int(intervals_per_launch)
         self.device = device
-        self.pages: Dict[tuple, List[_Page]] = {}
+        self.pages: Dict[str, List[_Page]] = {}
         self.bytes_in_use = 0
         self.stats = {"ticks": 0, "pages": 0, "peak_active": 0, "peak_bytes": 0}
 
     # ------------------------------------------------------------------
-    def _slot_bytes(self, key) -> int:
-        model, size = key
-        return self.models[model].footprint([list(size)])
-
-    def _page_slots(self, key) -> int:
-        return int(max(1, min(self.max_slots, self.page_bytes // max(1, self._slot_bytes(key)))))
-
     def _try_admit(self, req: Request, now: float, busy=()) -> bool:
-        key = (req.model, tuple(int(v) for v in req.size))
-        pages = self.pages.setdefault(key, [])
-        page = next((p for p in pages if p.free and id(p) not in busy), None)
-        if page is None and any(p.free for p in pages):
+        size = tuple(int(v) for v in (list(req.size) + [1, 1])[:3])
+        pages = self.pages.setdefault(req.model, [])
+        page = next((p for p in pages if p.has_room(size) and id(p) not in busy), None)
+        if page is None and any(p.has_room(size) for p in pages):
             return False        # a page with room is mid-interval: join it next
         if page is None:
-            slots = self._page_slots(key)
-            need = self.models[req.model].footprint([list(key[1])] * slots)
-            if self.bytes_in_use + need > self.hbm_budget:
-                # a smaller page if that fits the remaining budget
-                one = self._slot_bytes(key)
-                slots = int((self.hbm_budget - self.bytes_in_use) // max(1, one))
-                if slots < 1:
-                    return False
-            page = _Page(self.models[req.model], key[1], slots, self.device)
+            sizes = self.sizes if size in self.sizes else [size]
+            per = self.per_size
+            m = self.models[req.model]
+            while per >= 1 and self.bytes_in_use + m.footprint([list(z) for z in sizes] * per) > self.hbm_budget:
+                per //= 2
+            if per < 1:
+                return False
+            page = _Page(m, sizes, per, self.device)
             pages.append(page)
             self.bytes_in_use += page.bytes
             self.stats["pages"] += 1
             self.stats["peak_bytes"] = max(self.stats["peak_bytes"], self.bytes_in_use)
-        slot = page.free.pop()
+        slot = page.free[size].pop()
         b = page.batch
         P = self.compiled[req.model].nr_of_proc
         rates = np.asarray(req.rates, np.float64).reshape(1, P)
@@ -147,7 +160,7 @@ class Server:
         for key, pages in self.pages.items():
             keep = []
             for p in pages:
-                if not p.active and len(p.free) == p.slots:
+                if not p.active and p.n_free() == p.slots:
                     p.batch.close()
                     self.bytes_in_use -= p.bytes
                 else:
@@ -190,10 +203,18 @@ class Server:
                 for p in ps:
                     if p.active and id(p) not in inflight:
                         ids = np.array(sorted(p.active), np.int32)
-                        budget = np.array([min(p.active[s].req.sample_every, p.active[s].left) for s in ids],
-                                          np.int64)
-                        wait = p.batch.run_sampled(int(budget.max()), ids, budget)
-                        inflight[id(p)] = (p, ids, wait)
+                        every = p.active[int(ids[0])].req.sample_every
+                        if all(p.active[int(s)].req.sample_every == every for s in ids):
+                            # several intervals per launch, sampled inside the kernel
+                            nint = self.intervals_per_launch
+                            budget = np.array([min(nint * every, p.active[s].left) for s in ids], np.int64)
+                            wait = p.batch.run_recorded(int(budget.max()), ids, budget, every, nint)
+                            inflight[id(p)] = (p, ids, wait, True)
+                        else:
+                            budget = np.array([min(p.active[s].req.sample_every, p.active[s].left) for s in ids],
+                                              np.int64)
+                            wait = p.batch.run_sampled(int(budget.max()), ids, budget)
+                            inflight[id(p)] = (p, ids, wait, False)
                         self.stats["ticks"] += 1
             active = sum(len(p.active) for ps in self.pages.values() for p in ps)
             self.stats["peak_active"] = max(self.stats["peak_active"], active)
@@ -203,30 +224,35 @@ class Server:
                 elif waiting:
                     raise RuntimeError("request %d does not fit the HBM budget" % waiting[0].rid)
                 continue
-            finished = [k for k, (p, _, _) in inflight.items() if p.batch.idle()]
+            finished = [k for k, v in inflight.items() if v[0].batch.idle()]
             if not finished:
                 time.sleep(50e-6)
                 continue
             for k in finished:
-                p, ids, wait = inflight.pop(k)
+                p, ids, wait, recorded = inflight.pop(k)
                 _, rec = wait()
                 tdone = now()
                 for j, s in enumerate(ids):
                     f = p.active[s]
-                    ran = int(rec["kmc_step"][j]) - (f.req.steps - f.left)
+                    rj = rec[j] if recorded else {key: v[j:j + 1] for key, v in rec.items()}
+                    nrec = len(rj["kmc_step"])
+                    ran = int(rj["kmc_step"][-1]) - (f.req.steps - f.left)
                     f.left -= ran
-                    f.status = int(rec["status"][j])
-                    f.records.append({key: v[j] for key, v in rec.items()})
+                    f.status = int(rj["status"][-1])
+                    if ran > 0 or not f.records:
+                        for q in range(nrec):
+                            f.records.append({key: v[q] for key, v in rj.items()})
                     if f.left <= 0 or f.status != 0:
                         results[f.req.rid] = _finish(f, tdone)
                         del p.active[s]
-                        p.free.append(int(s))
+                        p.free[tuple(int(v) for v in p.batch.sizes[s])].append(int(s))
         return [results[r.rid] for r in requests]
 
     def run_alone(self, req: Request) -> Result:
         """The same request served on its own (a fresh page, nothing else in
         flight)."""
-        solo = Server(self.compiled, hbm_budget=self.hbm_budget, page_bytes=1, device=self.device)
+        solo = Server(self.compiled, hbm_budget=self.hbm_budget, sizes=[], slots_per_size=1,
+                      device=self.device, intervals_per_launch=self.intervals_per_launch)
         try:
             r = dataclasses.replace(req, arrival=0.0)
             return solo.serve([r], realtime=False)[0]

@@ -197,3 +197,38 @@ def test_run_sampled_equals_run_then_sample(engine):
     for k in rec_a:
         np.testing.assert_array_equal(rec_a[k], rec_b[k], err_msg=k)
     assert list(rec_a["kmc_step"]) == [400, 250]
+
+
+@pytest.mark.parametrize("factory,backend", [(W.ruo2_project, "local_smart"), (W.pairwise_project, "lat_int")])
+def test_run_recorded_matches_oracle(engine, oracle, factory, backend):
+    """Records written inside the step loop every 1000 steps equal the
+    oracle's state at those steps; the last record is the final state."""
+    pt = factory()
+    cm = compile_model(pt, backend)
+    rng = np.random.default_rng(21)
+    sizes = [(12, 12), (20, 9), (8, 8)]
+    rates = 10.0 ** rng.uniform(-1, 4, (len(sizes), cm.nr_of_proc))
+    m = engine.Model(cm)
+    b = engine.Batch(m, [list(s) + [1] for s in sizes])
+    b.init(np.arange(len(sizes), dtype=np.uint32) + 2)
+    b.set_rates(rates)
+    configs = [_random_config(pt, int(v), rng) for v in b.volumes]
+    b.set_config(configs)
+    budgets = [2500, 1700, 0]
+    wait = b.run_recorded(2500, [0, 1, 2], budgets, 1000, 3)
+    bud, recs = wait()
+    np.testing.assert_array_equal(bud, [0, 0, 0])
+    for r, s in enumerate(sizes):
+        o = oracle_replica(oracle, cm, s, r + 2, 0, rates[r], configs[r])
+        marks = [1000, 2000, 2500] if budgets[r] == 2500 else [1000, 1700] if budgets[r] == 1700 else [0]
+        assert len(recs[r]["kmc_step"]) == len(marks)
+        done = 0
+        for j, mk in enumerate(marks):
+            o.run(mk - done)
+            done = mk
+            assert recs[r]["kmc_step"][j] == o.kmc_step == mk
+            np.testing.assert_array_equal(recs[r]["procstat"][j], o.procstat())
+            np.testing.assert_array_equal(recs[r]["occupation"][j], o.occupation())
+            np.testing.assert_allclose(recs[r]["kmc_time"][j], o.kmc_time, rtol=1e-12, atol=0)
+            np.testing.assert_allclose(recs[r]["integ_rates"][j], o.integ_rates(), rtol=1e-9, atol=0)
+        compare_state(b, r, o, where="after recorded run r%d" % r)

@@ -17,11 +17,10 @@ def _small_stream():
 
 def test_same_alone_and_in_the_mix(engine):
     reqs = _small_stream()
-    srv = serve.Server(hbm_budget=0)
-    # room for a handful of pages only: requests wait for slots and reuse them
-    one = srv.models["ruo2"].footprint([[64, 64, 1]])
-    srv.hbm_budget = 12 * one
-    srv.page_bytes = 3 * one
+    sizes = [(20, 20, 1), (64, 64, 1)]
+    srv = serve.Server(hbm_budget=0, sizes=sizes, slots_per_size=4, intervals_per_launch=3)
+    # room for a handful of small pages only: requests wait for slots and reuse them
+    srv.hbm_budget = 2 * srv.models["ruo2"].footprint([list(z) for z in sizes] * 4)
     print("serving", len(reqs), "requests", flush=True)
     res = srv.serve(reqs, realtime=True)
     assert srv.stats["peak_bytes"] <= srv.hbm_budget
