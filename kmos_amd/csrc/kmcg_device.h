@@ -39,6 +39,9 @@ namespace kmcg {
 // nli     (1 word per group): node offset; node: pack3 | n << 24, n_cases,
 //                    (species_mask, kind, target) x n_cases  (kind 0 child, 1 leaf)
 //         leaf node: 0, -1, value
+// nli table (1 word per group, -1 = none): K | S << 8, K coordinates
+//         (pack3 | n << 24), then S^K bytes: the variant for the species
+//         index sum_k species_k * S^k
 constexpr int SEG_W = 6, WR_W = 2, OP_W = 2, IT_W = 2;
 
 __device__ __forceinline__ int sext7(int v) { return (v ^ 0x40) - 0x40; }
@@ -118,7 +121,7 @@ struct KParams {
     int moff;                      // largest offset component of the program
     int mrows;                     // mailbox rows per wave
     int merged;                    // 1: the segments of a step share one apply
-    int off_seg, off_proc, off_touch, off_nli;
+    int off_seg, off_proc, off_touch, off_nli, off_nlitab;
     double *srec;                  // in-loop sample records (or null)
     int sample_every, srec_cap, srec_width;
     int mode;                      // misc kernel mode

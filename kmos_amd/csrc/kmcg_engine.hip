@@ -53,7 +53,7 @@ struct Wave {
     int *avail;        // sites[P][V+1] then back[P][V+1]
     long long V1;
     int P, lane, mstride, mrows;
-    int off_seg, off_proc, off_touch, off_nli;
+    int off_seg, off_proc, off_touch, off_nli, off_nlitab;
     int nr[PPL];
     double rate[PPL];
     double integ[PPL];
@@ -79,6 +79,28 @@ __device__ int nli_eval(const Wave<PPL, BITS> &W, int group, int cx, int cy, int
 {
     const int *prog = W.prog;
     const int lc = W.d.lin(cx, cy, cz);
+    const int th = prog[W.off_nlitab + group];
+    if (th >= 0) {
+        // tabulated: read the K coordinates, their species, one table byte
+        const int hdr = prog[th], K = hdr & 0xFF, S = (hdr >> 8) & 0xFF;
+        int cw[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            cw[k] = prog[th + 1 + k];
+        int sp[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int c = (k < K) ? cw[k] : (1 << 24);
+            sp[k] = W.lat.get(site_from<IN>(W.d, lc, cx, cy, cz, unpack_x(c), unpack_y(c), unpack_z(c),
+                                            (c >> 24) & 0xFF));
+        }
+        int idx = 0;
+#pragma unroll
+        for (int k = 7; k >= 0; --k)
+            idx = (k < K) ? idx * S + sp[k] : idx;
+        const unsigned char *tab = (const unsigned char *)(prog + th + 1 + K);
+        return tab[idx];
+    }
     int pos = prog[W.off_nli + group];
     for (;;) {
         const int *nd = prog + pos;
@@ -555,6 +577,7 @@ __device__ __forceinline__ int wave_setup(Wave<PPL, BITS> &W, const KParams &kp,
     W.off_proc = kp.off_proc;
     W.off_touch = kp.off_touch;
     W.off_nli = kp.off_nli;
+    W.off_nlitab = kp.off_nlitab;
     const int4 dd0 = kp.dims[r];
     int4 dd;
     dd.x = __builtin_amdgcn_readfirstlane(dd0.x);
@@ -1121,7 +1144,7 @@ struct kmcg_model {
     int rstep = 1;       // mailbox rows of one executed process (all its segments)
     int mrows = 1;       // mailbox rows per wave
     int merged = 0;      // 1: one apply per step
-    int off_seg = 0, off_proc = 0, off_touch = 0, off_nli = 0, off_init = 0;
+    int off_seg = 0, off_proc = 0, off_touch = 0, off_nli = 0, off_init = 0, off_nlitab = 0;
 };
 
 struct kmcg_batch {
@@ -1554,6 +1577,89 @@ int build_prog(kmcg_model *m)
             }
             m->moff = mo;
         }
+        // nli trees over few coordinates become lookup tables: the case
+        // tree is walked once here for every species combination of its
+        // coordinates (first matching case, as on the device), so the device
+        // reads the cluster at once and looks the variant up
+        m->off_nlitab = (int)pg.size();
+        pg.resize(pg.size() + nli_nodes.size(), -1);
+        for (size_t g = 0; g < nli_nodes.size(); ++g) {
+            const std::vector<int> &src = nli_nodes[g];
+            std::vector<std::vector<int>> coords;    // dx, dy, dz, n
+            std::vector<int> node_at;
+            size_t pos = 0;
+            while (pos + 5 <= src.size()) {
+                node_at.push_back((int)pos);
+                const int nc = src[pos + 4];
+                if (nc == -1) {
+                    pos += 6;
+                    continue;
+                }
+                std::vector<int> c = {src[pos], src[pos + 1], src[pos + 2], src[pos + 3]};
+                if (std::find(coords.begin(), coords.end(), c) == coords.end())
+                    coords.push_back(c);
+                pos += 5 + 3 * (size_t)nc;
+            }
+            const int K = (int)coords.size(), S = m->S;
+            long long entries = 1;
+            for (int k = 0; k < K && entries <= 4096; ++k)
+                entries *= S;
+            if (K < 1 || K > 8 || entries > 4096)
+                continue;
+            std::vector<unsigned char> tab((size_t)entries, 0);
+            for (long long e = 0; e < entries; ++e) {
+                std::vector<int> sp(K);
+                long long r = e;
+                for (int k = 0; k < K; ++k) {
+                    sp[k] = (int)(r % S);
+                    r /= S;
+                }
+                size_t at = 0;
+                int val = 0;
+                for (int guard = 0; guard < 4096; ++guard) {
+                    const int *nd = &src[at];
+                    if (nd[4] == -1) {
+                        val = nd[5];
+                        break;
+                    }
+                    std::vector<int> c = {nd[0], nd[1], nd[2], nd[3]};
+                    const int q = sp[std::find(coords.begin(), coords.end(), c) - coords.begin()];
+                    int next = -1, leaf = -1;
+                    for (int k = 0; k < nd[4]; ++k) {
+                        if ((nd[5 + 3 * k] >> q) & 1) {
+                            if (nd[6 + 3 * k] == 1)
+                                leaf = nd[7 + 3 * k];
+                            else
+                                next = nd[7 + 3 * k];
+                            break;
+                        }
+                    }
+                    if (leaf >= 0) {
+                        val = leaf;
+                        break;
+                    }
+                    if (next < 0) {
+                        val = 0;
+                        break;
+                    }
+                    at = (size_t)next;
+                }
+                if (val < 0 || val > 255)
+                    return fail(KMCG_EUNSUPPORTED, "nli value does not fit a byte");
+                tab[(size_t)e] = (unsigned char)val;
+            }
+            pg[m->off_nlitab + g] = (int)pg.size();
+            pg.push_back(K | (S << 8));
+            for (const auto &c : coords)
+                pg.push_back(pack3(c[0], c[1], c[2]) | (c[3] << 24));
+            for (size_t e = 0; e < tab.size(); e += 4) {
+                unsigned w = 0;
+                for (size_t q = 0; q < 4 && e + q < tab.size(); ++q)
+                    w |= (unsigned)tab[e + q] << (8 * q);
+                pg.push_back((int)w);
+            }
+            pg.insert(pg.end(), 8, 0);            // coordinate reads go 8 words ahead
+        }
         // per process: anchor site type, its touchup addition op (local_smart)
         // or its group (lat_int), for the parallel initialisation
         m->off_init = (int)pg.size();
@@ -1622,6 +1728,7 @@ KParams make_params(kmcg_batch *b)
     kp.off_proc = m->off_proc;
     kp.off_touch = m->off_touch;
     kp.off_nli = m->off_nli;
+    kp.off_nlitab = m->off_nlitab;
     kp.prog_words = (int)m->prog.size();
     kp.prog_lds = (int)((m->prog.size() * sizeof(int) + 15) & ~(size_t)15);
     return kp;
