@@ -362,6 +362,31 @@ __device__ __forceinline__ void load_op(const int *prog, int ooff, int cbase, in
         cw[c] = (c < nc) ? cw[c] : 0;
 }
 
+// The static words a segment needs before it can run: its header and, per
+// lane, the first op word and its first four condition words.  Read ahead
+// of time (while the step waits for its site, or while the previous segment
+// runs) to take them off the critical path.
+struct SegPre {
+    int nw, woff, nops, nrounds, ooff, cbase;
+    bool as_add;
+    int2 op;
+    int cw[4];
+    const int *cd;
+};
+
+__device__ __forceinline__ void seg_prefetch(const int *prog, int off_seg, int seg, int lane, SegPre &q)
+{
+    const int *sg = prog + off_seg + SEG_W * seg;
+    q.nw = sg[0];
+    q.woff = sg[1];
+    q.nops = sg[2] & 0xFFFF;
+    q.nrounds = (sg[2] >> 16) & 0xFF;
+    q.as_add = (sg[2] >> 24) & 1;
+    q.ooff = sg[3];
+    q.cbase = sg[4];
+    load_op(prog, q.ooff, q.cbase, lane, lane < q.nops, q.op, q.cw, q.cd);
+}
+
 // Post one segment at base cell (bx, by, bz) into the process queues: the
 // lattice writes with the replace_species check (base.mpy:1187-1231), done
 // by one lane per write, then the ops evaluated lane-parallel on the new
@@ -370,18 +395,15 @@ __device__ __forceinline__ void load_op(const int *prog, int ooff, int cbase, in
 // a species mismatch the writes before the failing one are done and
 // nothing is posted.
 template <int PPL, int BITS, bool IN>
-__device__ __forceinline__ int post_segment_t(Wave<PPL, BITS> &W, int seg, int bx, int by, int bz)
+__device__ __forceinline__ int post_segment_t(Wave<PPL, BITS> &W, const SegPre &q, int bx, int by, int bz)
 {
     const int *prog = W.prog;
-    const int *sg = prog + W.off_seg + SEG_W * seg;
-    const int nw = sg[0], woff = sg[1], nops = sg[2] & 0xFFFF, nrounds = (sg[2] >> 16) & 0xFF;
-    const bool as_add = (sg[2] >> 24) & 1;
-    const int ooff = sg[3], cbase = sg[4];
-
-    int2 op;
-    int cw[4];
-    const int *cd;
-    load_op(prog, ooff, cbase, W.lane, W.lane < nops, op, cw, cd);
+    const int nw = q.nw, woff = q.woff, nops = q.nops, nrounds = q.nrounds;
+    const bool as_add = q.as_add;
+    const int ooff = q.ooff, cbase = q.cbase;
+    int2 op = q.op;
+    int cw[4] = {q.cw[0], q.cw[1], q.cw[2], q.cw[3]};
+    const int *cd = q.cd;
     const int lb = W.d.lin(bx, by, bz);
 
     if (nw > 0) {
@@ -428,27 +450,46 @@ __device__ __forceinline__ int post_segment_t(Wave<PPL, BITS> &W, int seg, int b
 
 // base cells away from the faces take the path without wrapping
 template <int PPL, int BITS>
-__device__ __forceinline__ int post_segment(Wave<PPL, BITS> &W, int seg, int bx, int by, int bz)
+__device__ __forceinline__ int post_segment(Wave<PPL, BITS> &W, const SegPre &q, int bx, int by, int bz)
 {
     if (W.d.interior(bx, by, bz))
-        return post_segment_t<PPL, BITS, true>(W, seg, bx, by, bz);
-    return post_segment_t<PPL, BITS, false>(W, seg, bx, by, bz);
+        return post_segment_t<PPL, BITS, true>(W, q, bx, by, bz);
+    return post_segment_t<PPL, BITS, false>(W, q, bx, by, bz);
 }
 
 // Execute one segment (a touchup) at base cell (bx, by, bz).
 template <int PPL, int BITS>
 __device__ int run_segment(Wave<PPL, BITS> &W, int seg, int bx, int by, int bz)
 {
-    const int e = post_segment<PPL, BITS>(W, seg, bx, by, bz);
+    SegPre q;
+    seg_prefetch(W.prog, W.off_seg, seg, W.lane, q);
+    const int e = post_segment<PPL, BITS>(W, q, bx, by, bz);
     const int a = apply_queues<PPL, BITS>(W);
     return e ? e : a;
 }
 
+// The process's items and its first segment's static words, read ahead.
+struct ProcPre {
+    int nit;
+    const int *it;
+    SegPre seg;
+};
+
+__device__ __forceinline__ void proc_prefetch(const int *prog, int off_proc, int off_seg, int proc, int lane,
+                                              ProcPre &q)
+{
+    const int *pp = prog + off_proc + 2 * (proc - 1);
+    q.nit = pp[0];
+    q.it = prog + pp[1];
+    seg_prefetch(prog, off_seg, q.nit > 0 ? q.it[0] : 0, lane, q.seg);
+}
+
 // run_proc_nr(proc, site); procstat is counted by the caller.  With
 // `merged`, all segments of the process are queued and applied once, else
-// each segment is applied before the next is posted.
+// each segment is applied before the next is posted.  The first segment's
+// static words come read ahead in `pre`.
 template <int PPL, int BITS>
-__device__ int run_proc(Wave<PPL, BITS> &W, int proc, int site, bool merged)
+__device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, bool merged)
 {
     const int s0 = site - 1;
     const int cell = divm(s0, W.ms);
@@ -456,16 +497,18 @@ __device__ int run_proc(Wave<PPL, BITS> &W, int proc, int site, bool merged)
     const int x = cell - yz * W.d.X;
     const int z = divm(yz, W.my);
     const int y = yz - z * W.d.Y;
-    const int *pp = W.prog + W.off_proc + 2 * (proc - 1);
-    const int nit = pp[0];
-    const int *it = W.prog + pp[1];
+    const int nit = pre.nit;
+    const int *it = pre.it;
+    SegPre cur = pre.seg;
     int e = ST_OK;
     for (int i = 0; i < nit; ++i, it += IT_W) {
         const int o = it[1];
         const int bx = wrap1(x + unpack_x(o), W.d.X);
         const int by = W.d.dim >= 2 ? wrap1(y + unpack_y(o), W.d.Y) : 0;
         const int bz = W.d.dim >= 3 ? wrap1(z + unpack_z(o), W.d.Z) : 0;
-        e = post_segment<PPL, BITS>(W, it[0], bx, by, bz);
+        if (i > 0)
+            seg_prefetch(W.prog, W.off_seg, it[0], W.lane, cur);
+        e = post_segment<PPL, BITS>(W, cur, bx, by, bz);
         if (e)
             break;
         if (!merged) {
@@ -684,7 +727,7 @@ __device__ __forceinline__ void write_record(const Wave<PPL, BITS> &W, const KPa
 
 // do_kmc_steps for one replica per wave (proclist_generic_subroutines.mpy:1-44)
 template <int PPL, int BITS, bool LDS_LAT>
-__global__ __launch_bounds__(64 * MAX_WPG) void kmc_steps_kernel(KParams kp)
+__global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4))) void kmc_steps_kernel(KParams kp)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     Wave<PPL, BITS> W;
@@ -791,6 +834,9 @@ __global__ __launch_bounds__(64 * MAX_WPG) void kmc_steps_kernel(KParams kp)
             for (int k = 0; k < PPL; ++k)
                 if (k == pk) W.pstat[k] += 1;
         }
+        // the process's program is read while its site is on the way
+        ProcPre pre;
+        proc_prefetch(W.prog, W.off_proc, W.off_seg, pick + 1, lane, pre);
         uj += 1;
         const int site = readlane_i(sv, pl);
         done += 1;
@@ -798,7 +844,7 @@ __global__ __launch_bounds__(64 * MAX_WPG) void kmc_steps_kernel(KParams kp)
 #pragma unroll
         for (int k = 0; k < PPL; ++k)
             nr_before[k] = W.nr[k];
-        st = run_proc<PPL, BITS>(W, pick + 1, site, kp.merged);
+        st = run_proc<PPL, BITS>(W, pre, site, kp.merged);
         if (st)
             break;
 #pragma unroll
@@ -864,7 +910,9 @@ __global__ __launch_bounds__(64 * MAX_WPG) void kmc_misc_kernel(KParams kp)
             for (int k = 0; k < PPL; ++k)
                 if (k == pk) W.pstat[k] += 1;
         }
-        st = run_proc<PPL, BITS>(W, proc, site, kp.merged);
+        ProcPre pre;
+        proc_prefetch(W.prog, W.off_proc, W.off_seg, proc, lane, pre);
+        st = run_proc<PPL, BITS>(W, pre, site, kp.merged);
         changed = true;
     } else if (kp.mode == MODE_NEXT_STEP) {
         double c[PPL], acc[PPL];
