@@ -50,6 +50,7 @@ struct Wave {
     int *win;          // [64][WIN] list-tail window of each lane (replay)
     int *occ;          // [S][spuck] species counts, kept while `track`
     bool track;
+    bool local_smart;
     int *avail;        // sites[P][V+1] then back[P][V+1]
     long long V1;
     int P, lane, mstride, mrows;
@@ -448,6 +449,112 @@ __device__ __forceinline__ int post_segment_t(Wave<PPL, BITS> &W, const SegPre &
     return ST_OK;
 }
 
+// Two segments of one local_smart process posted in one pass (the common
+// two-action process, e.g. a hop or a pair reaction).  Both writes are done
+// first, then the ops of both segments are evaluated together, one op per
+// lane; the first segment's ops must see the lattice before the second
+// write, so their reads of the second write's site return its old species.
+// Returns -1 (and changes nothing) when the pair does not qualify or a
+// write would fail its species check: the caller then runs the segments
+// one by one, which also reproduces the reference's behaviour on a
+// mismatch.
+template <int PPL, int BITS, bool IN>
+__device__ __forceinline__ int post_pair(Wave<PPL, BITS> &W, const SegPre &q1, const SegPre &q2, int bx1, int by1,
+                                         int bz1, int bx2, int by2, int bz2)
+{
+    const int *prog = W.prog;
+    const int n1 = q1.nops, n2 = q2.nops;
+    const int lb1 = W.d.lin(bx1, by1, bz1), lb2 = W.d.lin(bx2, by2, bz2);
+    // the two writes, lanes 0 and 1
+    const bool second = W.lane == 1;
+    const int bx = second ? bx2 : bx1, by = second ? by2 : by1, bz = second ? bz2 : bz1;
+    const int lbw = second ? lb2 : lb1;
+    const int woff = second ? q2.woff : q1.woff;
+    int s = 0, cur = 0, nsp = 0, t = 1;
+    bool bad = false;
+    if (W.lane < 2) {
+        const int w0 = prog[woff], w1 = prog[woff + 1];
+        t = (w0 >> 24) & 0xFF;
+        s = site_from<IN>(W.d, lbw, bx, by, bz, unpack_x(w0), unpack_y(w0), unpack_z(w0), t);
+        cur = W.lat.get(s);
+        nsp = w1 >> 16;
+        bad = cur < 0 || !((w1 >> cur) & 1);
+    }
+    const int sA = readlane_i(s, 0), sB = readlane_i(s, 1), oldB = readlane_i(cur, 1);
+    if (__ballot(W.lane < 2 && bad) || sA == sB)
+        return -1;
+    if (W.lane < 2) {
+        W.lat.xset(s, cur, nsp);
+        if (W.track) {
+            atomicAdd(&W.occ[cur * W.d.spuck + t - 1], -1);
+            atomicAdd(&W.occ[nsp * W.d.spuck + t - 1], 1);
+        }
+    }
+    wave_sync();
+    // the ops of both segments, one per lane
+    const int j = W.lane;
+    const bool valid = j < n1 + n2, first = j < n1;
+    const int jj = first ? j : j - n1;
+    const int ooff = first ? q1.ooff : q2.ooff, cbase = first ? q1.cbase : q2.cbase;
+    const int ex = first ? bx1 : bx2, ey = first ? by1 : by2, ez = first ? bz1 : bz2;
+    const int lb = first ? lb1 : lb2;
+    int2 op;
+    int cw[4];
+    const int *cd;
+    load_op(prog, ooff, cbase, jj, valid, op, cw, cd);
+    int proc = 0, code = 0, so = 0, rnd = 0;
+    if (valid) {
+        const int kind = op.x & 7;
+        rnd = ((op.x >> 3) & 0xFF) + (first ? 0 : q1.nrounds);
+        const int n = (op.x >> 11) & 0x3F;
+        const int arg = (int)((unsigned)op.x >> 23);
+        const int dx = sext7(op.y & 0x7F), dy = sext7((op.y >> 7) & 0x7F), dz = sext7((op.y >> 14) & 0x7F);
+        so = site_from<IN>(W.d, lb, ex, ey, ez, dx, dy, dz, n);
+        if (kind == OP_DEL_IF) {
+            proc = arg;
+            code = MB_DEL_IF;
+        } else {
+            const int nc = (op.x >> 17) & 0x3F;
+            int q[4], cs[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int w = (c < nc) ? cw[c] : (32 | 32 << 6 | 32 << 12 | 1 << 18);
+                cs[c] = site_from<IN>(W.d, lb, ex, ey, ez, (w & 63) - 32, ((w >> 6) & 63) - 32,
+                                      ((w >> 12) & 63) - 32, (w >> 18) & 63);
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                q[c] = W.lat.get(cs[c]);
+            bool ok = true;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int sp = (first && cs[c] == sB) ? oldB : q[c];
+                ok &= (c >= nc) || (sp == ((cw[c] >> 24) & 0xFF));
+            }
+            for (int c = 4; c < nc; ++c) {
+                const int w = cd[c];
+                const int sc = site_from<IN>(W.d, lb, ex, ey, ez, (w & 63) - 32, ((w >> 6) & 63) - 32,
+                                             ((w >> 12) & 63) - 32, (w >> 18) & 63);
+                const int sp = (first && sc == sB) ? oldB : W.lat.get(sc);
+                ok &= sp == ((w >> 24) & 0xFF);
+            }
+            if (ok) {
+                proc = arg;
+                code = MB_ADD;
+            }
+        }
+    }
+    const int nrt = q1.nrounds + q2.nrounds;
+    int slot = 0;
+    for (int it = 0; it < nrt; ++it)
+        if (proc > 0 && rnd == it)
+            slot = atomicAdd(&W.cnt[proc - 1], 1);
+    if (proc > 0)
+        W.mbox[slot * W.mstride + proc - 1] = ((so + 1) << 2) | code;
+    wave_sync();
+    return ST_OK;
+}
+
 // base cells away from the faces take the path without wrapping
 template <int PPL, int BITS>
 __device__ __forceinline__ int post_segment(Wave<PPL, BITS> &W, const SegPre &q, int bx, int by, int bz)
@@ -501,6 +608,23 @@ __device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, bool m
     const int *it = pre.it;
     SegPre cur = pre.seg;
     int e = ST_OK;
+    if (merged && W.local_smart && nit == 2 && cur.nw == 1 && !cur.as_add) {
+        SegPre q2;
+        seg_prefetch(W.prog, W.off_seg, it[IT_W], W.lane, q2);
+        if (q2.nw == 1 && !q2.as_add && cur.nops + q2.nops <= 64) {
+            const int o1 = it[1], o2 = it[IT_W + 1];
+            const int bx1 = wrap1(x + unpack_x(o1), W.d.X);
+            const int by1 = W.d.dim >= 2 ? wrap1(y + unpack_y(o1), W.d.Y) : 0;
+            const int bz1 = W.d.dim >= 3 ? wrap1(z + unpack_z(o1), W.d.Z) : 0;
+            const int bx2 = wrap1(x + unpack_x(o2), W.d.X);
+            const int by2 = W.d.dim >= 2 ? wrap1(y + unpack_y(o2), W.d.Y) : 0;
+            const int bz2 = W.d.dim >= 3 ? wrap1(z + unpack_z(o2), W.d.Z) : 0;
+            // near the faces the segments run one by one
+            if (W.d.interior(bx1, by1, bz1) && W.d.interior(bx2, by2, bz2) &&
+                post_pair<PPL, BITS, true>(W, cur, q2, bx1, by1, bz1, bx2, by2, bz2) >= 0)
+                return apply_queues<PPL, BITS>(W);
+        }
+    }
     for (int i = 0; i < nit; ++i, it += IT_W) {
         const int o = it[1];
         const int bx = wrap1(x + unpack_x(o), W.d.X);
@@ -653,6 +777,7 @@ __device__ __forceinline__ int wave_setup(Wave<PPL, BITS> &W, const KParams &kp,
     W.win = W.cnt + W.mstride;
     W.occ = W.win + 64 * WIN;
     W.track = false;
+    W.local_smart = kp.backend == 0;
     for (int i = W.lane; i < W.mstride; i += 64)
         W.cnt[i] = 0;
 #pragma unroll
