@@ -142,29 +142,51 @@ __device__ int nli_eval(const Wave<PPL, BITS> &W, int group, int cx, int cy, int
 constexpr int RBLK = 4;
 constexpr int WIN = 2 * RBLK;
 
+// 32-bit byte offsets into one replica's avail tables, from the uniform
+// table base: one VALU instruction per address
+struct Rows {
+    char *base;        // the replica's avail tables (uniform)
+    unsigned so, bo;   // byte offsets of this lane's sites[p] and back[p]
+    __device__ __forceinline__ int &s(int i) const { return *(int *)(base + (so + ((unsigned)i << 2))); }
+    __device__ __forceinline__ int &b(int i) const { return *(int *)(base + (bo + ((unsigned)i << 2))); }
+};
+
 // w: this lane's window in LDS, slot q at w[q - (nr0 - RBLK + 1)]
 template <int PPL, int BITS>
-__device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, int *sp, int *bp, int *w,
+__device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, const Rows &L, int *w,
                                              const int (&v)[RBLK], int m, int &err)
 {
     const int nr0 = nr;
+    // per entry: deletion?, site; pairwise equal sites, computed once
+    bool isdel[RBLK];
+    int st[RBLK];
+#pragma unroll
+    for (int i = 0; i < RBLK; ++i) {
+        isdel[i] = i < m && (v[i] & 3) != MB_ADD;
+        st[i] = v[i] >> 2;
+    }
     // all reads are issued unconditionally (index 0 = the unused slot 0
     // when not needed) so that they form one batch with a single wait
     int bv[RBLK];
     int nd = 0;
 #pragma unroll
     for (int i = 0; i < RBLK; ++i) {
-        const bool del = i < m && (v[i] & 3) != MB_ADD;
-        bv[i] = bp[del ? (v[i] >> 2) : 0];
-        nd += del ? 1 : 0;
+        bv[i] = L.b(isdel[i] ? st[i] : 0);
+        nd += isdel[i] ? 1 : 0;
     }
     int tv[RBLK];
 #pragma unroll
     for (int j = 0; j < RBLK; ++j)
-        tv[j] = sp[(j < nd && nr0 - j >= 1) ? nr0 - j : 0];
+        tv[j] = L.s((j < nd && nr0 - j >= 1) ? nr0 - j : 0);
+    bool same[RBLK][RBLK];     // same[i][j], i < j: entry j deletes entry i's site
 #pragma unroll
     for (int i = 0; i < RBLK; ++i)
-        bv[i] = (i < m && (v[i] & 3) != MB_ADD) ? bv[i] : 0;
+#pragma unroll
+        for (int j = i + 1; j < RBLK; ++j)
+            same[i][j] = isdel[j] && st[j] == st[i];
+#pragma unroll
+    for (int i = 0; i < RBLK; ++i)
+        bv[i] = isdel[i] ? bv[i] : 0;
     // slots above nr0 are only read after an addition of this block wrote
     // them, slots below nr0-nd+1 are never read (their entries here are
     // written anyway, so that no read above is conditional)
@@ -178,42 +200,42 @@ __device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, int *s
         if (!__ballot(i < m))
             break;
         if (i < m) {
-            const int e = v[i];
-            const int site = e >> 2;
-            if ((e & 3) == MB_ADD) {
+            const int site = st[i];
+            if (!isdel[i]) {
                 if (cur + 1 > W.d.V) {      // only a double add can get here
                     err |= 2;
                 } else {
                     cur += 1;
-                    sp[cur] = site;
-                    bp[site] = cur;
+                    L.s(cur) = site;
+                    L.b(site) = cur;
                     w[cur - base] = site;
 #pragma unroll
                     for (int j = i + 1; j < RBLK; ++j)
-                        bv[j] = ((v[j] & 3) != MB_ADD && (v[j] >> 2) == site) ? cur : bv[j];
+                        bv[j] = same[i][j] ? cur : bv[j];
                 }
             } else {
                 const int mpos = bv[i];
                 if (mpos == 0) {            // not registered: can_do false
-                    if ((e & 3) == MB_DEL_STRICT)
+                    if ((v[i] & 3) == MB_DEL_STRICT)
                         err |= 1;
                 } else {
                     const int last = cur;
                     if (mpos < last) {
                         const int moved = w[last - base];
-                        sp[mpos] = moved;
-                        bp[moved] = mpos;
+                        L.s(mpos) = moved;
+                        L.b(moved) = mpos;
                         if (mpos >= base)
                             w[mpos - base] = moved;
+                        // bv of additions is unused, so no deletion test here
 #pragma unroll
                         for (int j = i + 1; j < RBLK; ++j)
-                            bv[j] = ((v[j] & 3) != MB_ADD && (v[j] >> 2) == moved) ? mpos : bv[j];
+                            bv[j] = (st[j] == moved) ? mpos : bv[j];
                     }
-                    sp[last] = 0;
-                    bp[site] = 0;
+                    L.s(last) = 0;
+                    L.b(site) = 0;
 #pragma unroll
                     for (int j = i + 1; j < RBLK; ++j)
-                        bv[j] = ((v[j] & 3) != MB_ADD && (v[j] >> 2) == site) ? 0 : bv[j];
+                        bv[j] = same[i][j] ? 0 : bv[j];
                     cur = last - 1;
                 }
             }
@@ -254,7 +276,11 @@ __device__ __forceinline__ int apply_queues(Wave<PPL, BITS> &W)
                 v[i] = (i < m) ? v[i] : 0;
             if (m > 0) {
                 int nrk = W.nr[k];
-                replay_block<PPL, BITS>(W, nrk, W.sites(p), W.back(p), W.win + WIN * W.lane, v, m, err);
+                Rows L;
+                L.base = (char *)W.avail;
+                L.so = (unsigned)((long long)p * W.V1) << 2;
+                L.bo = (unsigned)((long long)(W.P + p) * W.V1) << 2;
+                replay_block<PPL, BITS>(W, nrk, L, W.win + WIN * W.lane, v, m, err);
                 W.nr[k] = nrk;
             }
         }
@@ -2108,6 +2134,9 @@ static int sizes_ok(const kmcg_model *m, int32_t n, const int32_t *sizes)
                                          + std::to_string(m->maxoff) + " cells per dimension");
         if ((long long)X * Y * Z * m->spuck >= (1LL << 24))
             return fail(KMCG_EUNSUPPORTED, "more than 2^24 sites per replica");
+        // the replay addresses one replica's avail tables with 32-bit byte offsets
+        if (8LL * m->P * ((long long)X * Y * Z * m->spuck + 1) >= (1LL << 32))
+            return fail(KMCG_EUNSUPPORTED, "avail tables of one replica exceed 4 GiB");
     }
     return KMCG_OK;
 }
