@@ -232,3 +232,36 @@ def test_run_recorded_matches_oracle(engine, oracle, factory, backend):
             np.testing.assert_allclose(recs[r]["kmc_time"][j], o.kmc_time, rtol=1e-12, atol=0)
             np.testing.assert_allclose(recs[r]["integ_rates"][j], o.integ_rates(), rtol=1e-9, atol=0)
         compare_state(b, r, o, where="after recorded run r%d" % r)
+
+
+@pytest.mark.parametrize("proc_name,second", [("CO_adsorption", False), ("O2_adsorption1", True)])
+def test_species_mismatch_matches_oracle(engine, oracle, proc_name, second):
+    """lattice.replace_species leaves the database stale; running a process
+    whose action then finds another species stops the replica with status
+    'species mismatch' (the reference's `stop`), after exactly the work the
+    reference did before it: for a two-action process whose second site was
+    changed, the first action is complete."""
+    pt = W.zgb_project()
+    cm = compile_model(pt, "local_smart")
+    sid = pt.species_ids()
+    m = engine.Model(cm)
+    b = engine.Batch(m, [(8, 8, 1)])
+    b.init(np.array([3], np.uint32))
+    rates = np.array([0.5, 1e10, 1e10, 1e10, 1e10, 0.25, 0.25])
+    b.set_rates(rates)
+    o = oracle_replica(oracle, cm, (8, 8), 3, 0, rates)
+    b.run(20)
+    o.run(20)
+    proc = [p.name for p in pt.processes].index(proc_name) + 1
+    # an empty site whose right neighbour (x + 1) is empty too
+    lat = o.lattice().reshape(8, 8)
+    ys, xs = np.nonzero((lat == sid["empty"]) & (np.roll(lat, -1, axis=1) == sid["empty"]))
+    x, y = int(xs[0]), int(ys[0])
+    site = y * 8 + x + 1
+    target = y * 8 + (x + 1) % 8 + 1 if second else site
+    b.replace_species(0, target, sid["empty"], sid["CO"])
+    oracle.lib().orc_replace_species(o.h, target, sid["empty"], sid["CO"])
+    b.run_proc_nr(0, proc, site)
+    oracle.lib().orc_run_proc_nr(o.h, proc, site)
+    assert o.status == 2
+    compare_state(b, 0, o, where="mismatch %s" % proc_name)
