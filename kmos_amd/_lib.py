@@ -47,6 +47,7 @@ SIGNATURES = [
     ("kmcg_get_accum_rates", _i32, [_vp, _i32, _vp]),
     ("kmcg_get_lattice", _i32, [_vp, _i32, _vp]),
     ("kmcg_get_avail", _i32, [_vp, _i32, _vp, _vp]),
+    ("kmcg_set_state", _i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("kmcg_get_occupation", _i32, [_vp, _i32, _i32, _vp]),
     ("kmcg_get_counters", _i32, [_vp, _i32, _i32, _vp, _vp]),
     ("kmcg_sample", _i32, [_vp, _vp, _i32, _vp]),

@@ -2748,6 +2748,64 @@ int kmcg_get_avail(kmcg_batch *b, int32_t r, int32_t *sites, int32_t *back)
     return KMCG_OK;
 }
 
+int kmcg_set_state(kmcg_batch *b, int32_t r, const double *time2, const int64_t *step2,
+                   const int8_t *lattice, const int32_t *nr_of_sites, const int64_t *procstat,
+                   const double *integ_rates, const int32_t *sites, const int32_t *back)
+{
+    int rc = check_replica(b, r);
+    if (rc)
+        return rc;
+    if (!time2 || !step2 || !lattice || !nr_of_sites || !procstat || !sites || !back)
+        return fail(KMCG_EINVAL, "set_state: null argument");
+    if (!std::isfinite(time2[0]) || !std::isfinite(time2[1]) || step2[0] < 0)
+        return fail(KMCG_EINVAL, "set_state: kmc_time, kmc_time_step must be finite, kmc_step >= 0");
+    const long long P = b->m->P, V = b->V[r], V1 = V + 1;
+    // every list index and site the kernels will follow is checked here:
+    // an inconsistent list would send add_proc/del_proc out of the row
+    std::vector<int> blk(2 * P * V1, 0);
+    for (long long p = 0; p < P; ++p) {
+        const long long nr = nr_of_sites[p];
+        if (nr < 0 || nr > V)
+            return fail(KMCG_EINVAL, "set_state: nr_of_sites out of range");
+        const int32_t *sp = sites + p * V, *bp = back + p * V;
+        long long held = 0;
+        for (long long i = 1; i <= V; ++i) {
+            const int s = sp[i - 1];
+            if (i <= nr) {
+                if (s < 1 || s > V || bp[s - 1] != i)
+                    return fail(KMCG_EINVAL, "set_state: avail_sites and avail_sites_back disagree");
+            } else if (s != 0) {
+                return fail(KMCG_EINVAL, "set_state: avail_sites holds entries beyond nr_of_sites");
+            }
+            held += bp[i - 1] != 0;
+            blk[p * V1 + i] = s;
+            blk[(P + p) * V1 + i] = bp[i - 1];
+        }
+        if (held != nr)     // a back entry pointing at a slot that does not point back
+            return fail(KMCG_EINVAL, "set_state: avail_sites_back holds sites not in the list");
+    }
+    for (long long p = 0; p < P; ++p)
+        if (integ_rates && !std::isfinite(integ_rates[p]))
+            return fail(KMCG_EINVAL, "set_state: integ_rates must be finite");
+    std::vector<unsigned char> packed;
+    if ((rc = pack_config(b, r, lattice, packed)))
+        return rc;
+    const std::vector<double> ig = integ_rates ? std::vector<double>(integ_rates, integ_rates + P)
+                                               : std::vector<double>(P, 0.0);
+    const int running = 0;
+    if ((rc = copy_in(b, b->lat_pool + b->lat_off[r], packed.data(), packed.size())) ||
+        (rc = copy_in(b, b->avail_pool + b->avail_off[r], blk.data(), blk.size())) ||
+        (rc = copy_in(b, b->nr + r * P, (const int *)nr_of_sites, (size_t)P)) ||
+        (rc = copy_in(b, b->pstat + r * P, (const long long *)procstat, (size_t)P)) ||
+        (rc = copy_in(b, b->integ + r * P, ig.data(), (size_t)P)) ||
+        (rc = copy_in(b, b->ktime + r, time2, 1)) || (rc = copy_in(b, b->kdt + r, time2 + 1, 1)) ||
+        (rc = copy_in(b, b->kstep + r, (const long long *)step2, 1)) ||
+        (rc = copy_in(b, b->rngctr + r, (const unsigned long long *)(step2 + 1), 1)) ||
+        (rc = copy_in(b, b->status + r, &running, 1)))
+        return rc;
+    return KMCG_OK;
+}
+
 int kmcg_get_occupation(kmcg_batch *b, int32_t r0, int32_t n, double *out)
 {
     if (!b || r0 < 0 || n < 1 || r0 + n > b->n || !out)

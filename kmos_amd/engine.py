@@ -219,6 +219,44 @@ class Batch:
         _lib.check(_lib.load().kmcg_get_avail(self.h, r, _ptr(sites), _ptr(back)))
         return sites, back
 
+    def get_state(self, r: int) -> dict:
+        """Every field of replica r that a .reload snapshot holds."""
+        t, dt, step, ctr, status = self.scalars(r)
+        sites, back = self.avail(r)
+        return {"kmc_time": t, "kmc_time_step": dt, "kmc_step": step, "rng_counter": ctr,
+                "status": status, "lattice": self.lattice(r), "nr_of_sites": self.nr_of_sites(r),
+                "procstat": self.procstat(r), "integ_rates": self.integ_rates(r),
+                "rates": self.get_rates(r), "avail_sites": sites, "avail_sites_back": back}
+
+    def set_state(self, r: int, state: dict):
+        """Overwrite replica r with a state as get_state returns it (integ_rates,
+        kmc_time_step and rng_counter optional: 0, 0 and kmc_step); rates are
+        applied when present."""
+        V, P = int(self.volumes[r]), self.P
+
+        def arr(key, dtype, shape):
+            a = np.ascontiguousarray(state[key], dtype=dtype)
+            if a.shape != shape:
+                raise ValueError("state[%r] has shape %s, expected %s" % (key, a.shape, shape))
+            return a
+        lat = arr("lattice", np.int8, (V,))
+        nr = arr("nr_of_sites", np.int32, (P,))
+        ps = arr("procstat", np.int64, (P,))
+        sites = arr("avail_sites", np.int32, (P, V))
+        back = arr("avail_sites_back", np.int32, (P, V))
+        ig = arr("integ_rates", np.float64, (P,)) if state.get("integ_rates") is not None else None
+        step = int(state["kmc_step"])
+        ctr = state.get("rng_counter")
+        t2 = np.array([state["kmc_time"], state.get("kmc_time_step", 0.0)], np.float64)
+        s2 = np.array([step, step if ctr is None else int(ctr)], np.int64)
+        rates = arr("rates", np.float64, (P,)) if state.get("rates") is not None else None
+        if rates is not None and not (np.all(np.isfinite(rates)) and np.all(rates >= 0.0)):
+            raise ValueError("state['rates'] must be finite and >= 0")
+        _lib.check(_lib.load().kmcg_set_state(self.h, r, _ptr(t2), _ptr(s2), _ptr(lat), _ptr(nr),
+                                              _ptr(ps), _ptr(ig), _ptr(sites), _ptr(back)))
+        if rates is not None:
+            self.set_rates(rates[None, :], r0=r)
+
     def occupation(self, r0: int = 0, n: Optional[int] = None) -> np.ndarray:
         n = self.n - r0 if n is None else n
         out = np.zeros((n, self.S, self.spuck), np.float64)

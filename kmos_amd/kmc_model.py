@@ -24,6 +24,7 @@ import numpy as np
 
 from .compiler import CompiledModel
 from .engine import Batch, Model
+from .snapshot import ReloadError, read_reload, write_reload
 
 
 class _Base:
@@ -91,6 +92,52 @@ class _Base:
 
     def get_null_species(self):
         return self.null_species
+
+    # snapshots ---------------------------------------------------------------
+    def get_system_name(self):
+        return self._o.system_name
+
+    def set_system_name(self, name):
+        self._o.system_name = str(name).strip()
+
+    def save_system(self):
+        """Write <system_name>.reload in the working directory (base.mpy:517-578)."""
+        o = self._o
+        st = o._b().get_state(0)
+        st.update(walltime=o._walltime, rng_seed=o.proclist.seed, rng_stream=0,
+                  system_size=o.lattice.system_size.copy())
+        write_reload(o.system_name + ".reload", st)
+
+    def reload_system(self, input_system_name):
+        """Allocate the system and restore it from <name>.reload
+        (base.mpy:363-514); False when the file does not exist.  Files
+        without the engine's rng records continue the random stream of the
+        seed last given to proclist.init (1 if none) from kmc_step; files
+        without system_size take lattice.system_size when it matches the
+        volume, else the square (2-D) or cube (3-D) lattice of that volume."""
+        import os
+        o = self._o
+        name = str(input_system_name).strip()
+        path = name + ".reload"
+        if not os.path.exists(path):
+            return False
+        st = read_reload(path)
+        cm = o.compiled
+        if st["nr_of_proc"] != cm.nr_of_proc:
+            raise ReloadError("%s holds %d processes, the model has %d" % (path, st["nr_of_proc"], cm.nr_of_proc))
+        size = o._reload_size(st, path)
+        seed = int(st.get("rng_seed", o.proclist.seed or 1))
+        o.system_name = name
+        o._allocate(size, seed, stream=int(st.get("rng_stream", 0)))
+        try:
+            o._rates[:] = st["rates"]
+            o.batch.set_state(0, st)
+        except Exception:
+            o._free()
+            raise
+        o.proclist.seed = seed
+        o._walltime = float(st.get("walltime", 0.0))
+        return True
 
     def is_allocated(self):
         return self._o.batch is not None
@@ -173,6 +220,7 @@ class _Proclist:
     def init(self, input_system_size, system_name="kmc_model", layer=0, seed_in=1, no_banner=True):
         """allocate_system + initialize_state (proclist_generic_subroutines.mpy:166-304)."""
         self._o._allocate(input_system_size, int(seed_in))
+        self._o.system_name = str(system_name).strip()
         self.seed = int(seed_in)
 
     def do_kmc_steps(self, n):
@@ -205,6 +253,7 @@ class KmcModule:
         self.batch = None
         self._rates = np.zeros(compiled.nr_of_proc)
         self._walltime = 0.0
+        self.system_name = "kmc_model"
         self.base = _Base(self)
         self.lattice = _Lattice(self)
         self.proclist = _Proclist(self)
@@ -214,7 +263,7 @@ class KmcModule:
             raise RuntimeError("kmc_model: system not allocated (call proclist.init first)")
         return self.batch
 
-    def _allocate(self, size: Sequence[int], seed: int):
+    def _allocate(self, size: Sequence[int], seed: int, stream: int = 0):
         if self.batch is not None:
             raise RuntimeError("kmc_model: tried to allocate the system twice, please deallocate first")
         dim = self.compiled.dimension
@@ -223,10 +272,25 @@ class KmcModule:
         for d in range(dim, 3):
             full[d] = 1
         self.batch = Batch(self.model, [full], device=self.device)
-        self.batch.init([seed & 0xFFFFFFFF], [0])
+        self.batch.init([seed & 0xFFFFFFFF], [stream & 0xFFFFFFFF])
         self.batch.set_rates(self._rates[None, :])
         self.lattice.system_size[:] = full
         self._walltime = 0.0
+
+    def _reload_size(self, st: dict, path: str):
+        spuck, dim = self.compiled.spuck, self.compiled.dimension
+        V = st["volume"]
+        if st.get("system_size") is not None:
+            size = [int(v) for v in st["system_size"]]
+        elif int(np.prod(self.lattice.system_size)) * spuck == V:
+            size = [int(v) for v in self.lattice.system_size]
+        else:
+            side = int(round((V / spuck) ** (1.0 / dim)))
+            size = [side if d < dim else 1 for d in range(3)]
+        if int(np.prod(size)) * spuck != V:
+            raise ReloadError("%s: volume %d is no lattice of this model's %d sites per cell; "
+                              "set lattice.system_size first" % (path, V, spuck))
+        return size
 
     def _free(self):
         if self.batch is not None:

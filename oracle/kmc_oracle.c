@@ -20,6 +20,7 @@
  *   lattice2nr / nr2lattice           kmos/fortran_src/lattice.mpy:146-210
  *   do_kmc_step(s), get_next_kmc_step proclist_generic_subroutines.mpy:1-110
  *   get_occupation                    proclist_generic_subroutines.mpy:113-158
+ *   reload_system (state overwrite)   base.mpy:363-514
  *   initialize_state                  proclist_generic_subroutines.mpy:236-304
  *   put/take/touchup routines         kmos/io/__init__.py:2219-2443 (as data)
  *   lat_int run_proc/nli/touchup_cell kmos/io/__init__.py:1732-2057 (as data)
@@ -603,6 +604,28 @@ void orc_get_avail(const orc_t *o, int32_t *sites, int32_t *back)
     for (int p = 0; p < o->P; ++p) {
         memcpy(sites + (int64_t)p * o->V, o->sites + (int64_t)p * (o->V + 1) + 1, sizeof(int32_t) * o->V);
         memcpy(back + (int64_t)p * o->V, o->back + (int64_t)p * (o->V + 1) + 1, sizeof(int32_t) * o->V);
+    }
+}
+/* reload_system (base.mpy:363-514): every field of the file overwrites the
+ * state as it stands, without checks; integ_rates stay at zero unless given
+ * (the file of save_system, base.mpy:517-578, has none). */
+void orc_set_state(orc_t *o, double kmc_time, double kmc_time_step, int64_t kmc_step,
+                   uint64_t rng_ctr, const int8_t *lattice, const int32_t *nr, const int64_t *procstat,
+                   const double *rates, const double *integ, const int32_t *sites, const int32_t *back)
+{
+    o->kmc_time = kmc_time;
+    o->kmc_time_step = kmc_time_step;
+    o->kmc_step = kmc_step;
+    o->rng_ctr = rng_ctr;
+    o->status = 0;
+    memcpy(o->lat + 1, lattice, o->V);
+    memcpy(o->nr, nr, sizeof(int32_t) * o->P);
+    memcpy(o->procstat, procstat, sizeof(int64_t) * o->P);
+    memcpy(o->rates, rates, sizeof(double) * o->P);
+    for (int p = 0; p < o->P; ++p) {
+        o->integ[p] = integ ? integ[p] : 0.0;
+        memcpy(o->sites + (int64_t)p * (o->V + 1) + 1, sites + (int64_t)p * o->V, sizeof(int32_t) * o->V);
+        memcpy(o->back + (int64_t)p * (o->V + 1) + 1, back + (int64_t)p * o->V, sizeof(int32_t) * o->V);
     }
 }
 int orc_get_species(const orc_t *o, int site) { return o->lat[site]; }

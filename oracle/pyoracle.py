@@ -66,6 +66,7 @@ def lib():
                      "orc_get_accum", "orc_get_occupation"):
             getattr(L, name).argtypes = [vp, vp]
         L.orc_get_avail.argtypes = [vp, vp, vp]
+        L.orc_set_state.argtypes = [vp, dbl, dbl, i64, u64, vp, vp, vp, vp, vp, vp, vp]
         L.orc_get_species.argtypes = [vp, i32]
         L.orc_get_species.restype = i32
         L.orc_replace_species.argtypes = [vp, i32, i32, i32]
@@ -80,7 +81,7 @@ def lib():
 
 
 def _ptr(a):
-    return a.ctypes.data_as(ctypes.c_void_p)
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
 
 
 def philox4x32_10(ctr, key):
@@ -214,3 +215,25 @@ class Oracle:
         back = np.zeros((self.P, self.V), np.int32)
         lib().orc_get_avail(self.h, _ptr(sites), _ptr(back))
         return sites, back
+
+    def get_state(self, rates):
+        """The fields of a .reload snapshot (rates: the ones last set)."""
+        sites, back = self.avail()
+        return {"kmc_time": self.kmc_time, "kmc_time_step": lib().orc_kmc_time_step(self.h),
+                "kmc_step": self.kmc_step, "rng_counter": self.rng_ctr, "lattice": self.lattice(),
+                "nr_of_sites": self.nr_of_sites(), "procstat": self.procstat(),
+                "integ_rates": self.integ_rates(), "rates": np.asarray(rates, np.float64),
+                "avail_sites": sites, "avail_sites_back": back}
+
+    def set_state(self, st):
+        """reload_system: overwrite the state (rng_counter defaults to kmc_step)."""
+        c = lambda k, t: np.ascontiguousarray(st[k], dtype=t)
+        ig = st.get("integ_rates")
+        ig = None if ig is None else np.ascontiguousarray(ig, np.float64)
+        keep = [c("lattice", np.int8), c("nr_of_sites", np.int32), c("procstat", np.int64),
+                c("rates", np.float64), ig, c("avail_sites", np.int32), c("avail_sites_back", np.int32)]
+        assert keep[0].size == self.V and keep[5].size == self.P * self.V
+        ctr = st.get("rng_counter")
+        lib().orc_set_state(self.h, float(st["kmc_time"]), float(st.get("kmc_time_step", 0.0)),
+                            int(st["kmc_step"]), int(st["kmc_step"] if ctr is None else ctr),
+                            *[_ptr(a) for a in keep])

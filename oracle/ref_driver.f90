@@ -8,10 +8,14 @@
 !   X Y Z
 !   seed stream
 !   nsteps mode            mode 0: per-step trace, 1: one do_kmc_steps call
-!                          (its wall time is printed on stdout)
+!                          (its wall time is printed on stdout), 2: trace,
+!                          then save_system (writes ref.reload in the
+!                          working directory), 3: deallocate_system and
+!                          reload_system('in') (reads in.reload), then trace
 !   has_config
 !   rates(1:nr_of_proc)
 !   config(1:volume)       species per site, only if has_config /= 0
+!   rng_step               mode 3 only: first step of the uniform stream
 ! Output (stream access, little endian), see tests/refbin.py:
 !   int32 nexec, [int32 proc(n), int32 site(n), real64 time(n)] (mode 0)
 !   real64 kmc_time, int64 kmc_step, int32 lattice(volume),
@@ -22,13 +26,13 @@
 ! do_kmc_step (kmos/fortran_src/proclist_generic_subroutines.mpy:47-80);
 ! mode 1 runs the reference's own do_kmc_steps so both can be compared.
 program ref_driver
-  use iso_c_binding, only: c_int32_t
+  use iso_c_binding, only: c_int32_t, c_int64_t
   use kind_values
   use base, only: set_rate_const, update_accum_rate, update_clocks, &
        update_integ_rate, determine_procsite, get_kmc_time, get_kmc_step, &
        get_nrofsites, get_procstat, get_integ_rate, avail_sites, &
        base_get_species => get_species, base_replace_species => replace_species, &
-       get_volume
+       get_volume, deallocate_system, reload_system, save_system
   use proclist, only: init, run_proc_nr, do_kmc_steps, nr_of_proc
   use lattice, only: default_layer
   implicit none
@@ -37,9 +41,15 @@ program ref_driver
       import :: c_int32_t
       integer(c_int32_t), value :: seed, stream
     end subroutine
+    subroutine kmcg_ref_rng_seek(step) bind(C, name="kmcg_ref_rng_seek")
+      import :: c_int64_t
+      integer(c_int64_t), value :: step
+    end subroutine
   end interface
   character(len=4096) :: inpath, outpath
-  character(len=400) :: sysname
+  character(len=200) :: sysname, loadname
+  logical :: reloaded
+  integer(kind=ilong) :: rng_step
   integer(kind=iint) :: nx, ny, nz, seed, stream, mode, has_config
   integer(kind=ilong) :: nsteps, i, kstep
   integer(kind=iint) :: p, s, volume, proc_nr, nr_site, sp, x, y, z, n
@@ -85,13 +95,22 @@ program ref_driver
       end do
     end do
   end if
+  rng_step = 0
+  if (mode == 3) then
+    read(u_in, *) rng_step
+    call deallocate_system()
+    loadname = 'in'
+    call reload_system(loadname, reloaded)
+    if (.not. reloaded) stop 'ref_driver: in.reload not found'
+  end if
   close(u_in)
   ! drawing starts after set-up, as in the engine
   call kmcg_ref_rng_set(int(seed, c_int32_t), int(stream, c_int32_t))
+  call kmcg_ref_rng_seek(int(rng_step, c_int64_t))
 
   open(newunit=u_out, file=trim(outpath), access='stream', form='unformatted', &
        status='replace', action='write')
-  if (mode == 0) then
+  if (mode /= 1) then
     allocate(tproc(nsteps), tsite(nsteps), times(nsteps))
     do i = 1, nsteps
       call kmcg_ref_uniform(ran_time)
@@ -110,6 +129,7 @@ program ref_driver
     nexec = int(nsteps, iint)
     write(u_out) nexec
     write(u_out) tproc, tsite, times
+    if (mode == 2) call save_system()
   else
     call system_clock(c0, crate)
     call do_kmc_steps(nsteps)

@@ -44,6 +44,9 @@ void kmcg_ref_rng_set(int32_t seed, int32_t stream)
     g_draw = 0;
 }
 
+/* continue the stream at the first draw of step `step` */
+void kmcg_ref_rng_seek(int64_t step) { g_draw = 3 * (uint64_t)step; }
+
 int64_t kmcg_ref_rng_draws(void) { return (int64_t)g_draw; }
 
 /* implicit-interface Fortran call: `call kmcg_ref_uniform(x)` */

@@ -15,12 +15,17 @@ def driver_path(name):
     return p if os.path.exists(p) else None
 
 
-def run_reference(name, size, seed, stream, nsteps, rates, P, config=None, mode=0):
+def run_reference(name, size, seed, stream, nsteps, rates, P, config=None, mode=0,
+                  reload_text=None, rng_step=0):
+    """mode 0 trace, 1 do_kmc_steps, 2 trace then save_system (the file's
+    text comes back as res["reload"]), 3 reload_system(reload_text), then
+    trace from step rng_step of the uniform stream."""
     exe = driver_path(name)
     if exe is None:
         raise FileNotFoundError("reference build %s missing (run oracle/build_ref.py)" % name)
     X, Y = size[0], size[1]
     Z = size[2] if len(size) > 2 else 1
+    res = {}
     with tempfile.TemporaryDirectory() as td:
         inp = os.path.join(td, "in.txt")
         out = os.path.join(td, "out.bin")
@@ -30,8 +35,15 @@ def run_reference(name, size, seed, stream, nsteps, rates, P, config=None, mode=
             fh.write(" ".join(repr(float(r)) for r in rates) + "\n")
             if config is not None:
                 fh.write(" ".join(str(int(c)) for c in config) + "\n")
-        subprocess.check_call([exe, inp, out])
+            if mode == 3:
+                fh.write("%d\n" % rng_step)
+        if mode == 3:
+            with open(os.path.join(td, "in.reload"), "w") as fh:
+                fh.write(reload_text)
+        subprocess.check_call([exe, inp, out], cwd=td)
         buf = open(out, "rb").read()
+        if mode == 2:
+            res["reload"] = open(os.path.join(td, "ref.reload")).read()
     off = 0
 
     def take(dtype, count):
@@ -40,9 +52,8 @@ def run_reference(name, size, seed, stream, nsteps, rates, P, config=None, mode=
         off += a.nbytes
         return a.copy()
 
-    res = {}
     n = int(take("<i4", 1)[0])
-    if mode == 0:
+    if mode != 1:
         res["proc"] = take("<i4", n)
         res["site"] = take("<i4", n)
         res["time"] = take("<f8", n)
