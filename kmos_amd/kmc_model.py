@@ -277,6 +277,12 @@ class KmcModule:
         self.lattice.system_size[:] = full
         self._walltime = 0.0
 
+    def set_rates(self, rates):
+        """base.set_rate_const for every process at once."""
+        self._rates[:] = rates
+        if self.batch is not None:
+            self.batch.set_rates(self._rates[None, :])
+
     def _reload_size(self, st: dict, path: str):
         spuck, dim = self.compiled.spuck, self.compiled.dimension
         V = st["volume"]

@@ -27,6 +27,88 @@ class _Expando:
     pass
 
 
+def parameters_repr(parameters) -> str:
+    """The text of Model_Parameters.__repr__ (kmos/run/__init__.py:1670-1692)."""
+    fixed = [n for n, p in parameters.items() if not p.get("adjustable", False)]
+    res = "# kMC model parameters (%i, fixed %i)\n" % (len(parameters), len(fixed))
+    res += "# --------------------\n"
+    for n in sorted(parameters):
+        res += "# %s = %s" % (n, parameters[n]["value"])
+        res += "  # *\n" if parameters[n].get("adjustable", False) else "\n"
+    res += "# --------------------\n"
+    if len(fixed) != len(parameters):
+        res += "# * adjustable parameters\n"
+    return res
+
+
+def rate_constants(settings, parameters, compiled) -> np.ndarray:
+    """set_rate_constants (kmos/run/__init__.py:2369-2437): every process's
+    rate constant from its expression, in process-number order."""
+    names = [n.lower() for n in compiled.process_names]
+    rates = np.zeros(len(names))
+    for proc in sorted(settings.rate_constants):
+        expr = settings.rate_constants[proc][0]
+        k = evaluate_rate_expression(expr, parameters)
+        if k < 0.0:
+            raise UserWarning("%s = %s: Negative rate-constants do no make sense" % (expr, k))
+        rates[names.index(proc.lower())] = k
+    return rates
+
+
+def tof_matrix(settings, compiled) -> np.ndarray:
+    """[tof, process] counts of each TOF (kmos/run/__init__.py:262-267)."""
+    tofs = compiled.project.tof_names()
+    names = [n.lower() for n in compiled.process_names]
+    m = np.zeros((len(tofs), compiled.nr_of_proc))
+    for process, tof_count in sorted(settings.tof_count.items()):
+        for tof, factor in tof_count.items():
+            m[tofs.index(tof), names.index(process.lower())] += factor
+    return m
+
+
+def std_header(settings, compiled) -> str:
+    """get_std_header (kmos/run/__init__.py:856-862): adjustable parameters,
+    TOFs, species_site occupations, then the clocks."""
+    p = settings.parameters
+    params = " ".join(n for n in sorted(p) if p[n].get("adjustable", False))
+    tofs = " ".join(compiled.project.tof_names())
+    occ = " ".join("%s_%s" % (sp, site) for sp in sorted(settings.representations)
+                   for site in settings.site_names)
+    return "#%s %s %s kmc_time simulated_time kmc_steps\n" % (params, tofs, occ)
+
+
+def adjustable_values(parameters) -> list:
+    """atoms.params: the adjustable parameters' values, sorted by name."""
+    return [float(parameters[p]["value"]) for p in sorted(parameters)
+            if parameters[p].get("adjustable", False)]
+
+
+def interval_tof(tof_matrix, d_procstat, d_integ, delta_t, ncells, tof_method):
+    """get_atoms' tof_data / tof_integ over one interval
+    (kmos/run/__init__.py:769-800)."""
+    if tof_method == "procrates":
+        return np.dot(tof_matrix, d_procstat / delta_t / ncells)
+    if tof_method == "integ":
+        return np.dot(tof_matrix, d_integ / delta_t / ncells)
+    raise NotImplementedError('tof_method="%s" not supported' % tof_method)
+
+
+def std_sampled_output(params, tofs, occs, delta_ts, step_ts, total_time, simulated_time,
+                       total_steps, header, output="str"):
+    """The averaging and formatting of get_std_sampled_data
+    (kmos/run/__init__.py:925-966): occupations weighted by each interval's
+    last kmc_time_step, TOFs by the interval's duration."""
+    occs_mean = np.average(occs, axis=0, weights=step_ts)
+    tof_mean = np.average(tofs, axis=0, weights=delta_ts)
+    outdata = tuple(list(params) + list(np.asarray(tof_mean).flatten()) + list(np.asarray(occs_mean).flatten())
+                    + [total_time, simulated_time, total_steps])
+    if output == "str":
+        return (" ".join(["%.5e"] * len(outdata)) + "\n") % outdata
+    if output == "dict":
+        return dict(zip(header[1:].split(), outdata))
+    raise UserWarning("Output format %s not defined. I only know 'str' and 'dict'" % output)
+
+
 class Model_Parameters:
     """model.parameters.<name> = value re-evaluates every rate constant
     (kmos/run/__init__.py:1643-1714)."""
@@ -52,10 +134,7 @@ class Model_Parameters:
                 if pattern is None or fnmatch.fnmatch(n, pattern)]
 
     def __repr__(self):
-        lines = ["# kMC model parameters (%i)" % len(self._model.settings_parameters)]
-        for n in sorted(self._model.settings_parameters):
-            lines.append("# %s = %s" % (n, self._model.settings_parameters[n]["value"]))
-        return "\n".join(lines) + "\n"
+        return parameters_repr(object.__getattribute__(self, "_model").settings_parameters)
 
 
 class Model_Rate_Constants:
@@ -124,23 +203,15 @@ class KMC_Model:
 
     def set_rate_constants(self):
         """set_rate_constants (kmos/run/__init__.py:2369-2437)."""
-        for proc in sorted(self.settings.rate_constants):
-            expr = self.settings.rate_constants[proc][0]
-            k = evaluate_rate_expression(expr, self.settings_parameters)
-            if k < 0.0:
-                raise UserWarning("%s = %s: Negative rate-constants do no make sense" % (expr, k))
-            self.base.set_rate_const(getattr(self.proclist, proc.lower()), k)
+        rates = rate_constants(self.settings, self.settings_parameters, self.compiled)
+        self._kmc.set_rates(rates)
 
     def reset(self):
         """kmos/run/__init__.py:243-339 without the ASE representations."""
         self.proclist.init(self.size, self.system_name, self.lattice.default_layer,
                            self.random_seed, not self.banner)
-        self.tofs = tofs = self.compiled.project.tof_names()
-        self.tof_matrix = np.zeros((len(tofs), self.proclist.nr_of_proc))
-        for process, tof_count in sorted(self.settings.tof_count.items()):
-            pnr = getattr(self.proclist, process.lower())
-            for tof, factor in tof_count.items():
-                self.tof_matrix[tofs.index(tof), pnr - 1] += factor
+        self.tofs = self.compiled.project.tof_names()
+        self.tof_matrix = tof_matrix(self.settings, self.compiled)
         self.procstat = np.zeros(self.proclist.nr_of_proc, dtype=np.int64)
         self.integ_rates = np.zeros(self.proclist.nr_of_proc)
         self.time = 0.0
@@ -169,8 +240,7 @@ class KMC_Model:
         t, dt, ks, _, _ = b.scalars(0)
         atoms.kmc_time = t
         atoms.kmc_step = ks
-        atoms.params = [float(self.settings_parameters[p]["value"]) for p in sorted(self.settings_parameters)
-                        if self.settings_parameters[p].get("adjustable", False)]
+        atoms.params = adjustable_values(self.settings_parameters)
         atoms.procstat = b.procstat(0).astype(np.float64)
         atoms.occupation = self.proclist.get_occupation()
         atoms.integ_rates = b.integ_rates(0)
@@ -185,8 +255,9 @@ class KMC_Model:
             atoms.tof_data = np.zeros_like(self.tof_matrix[:, 0])
             atoms.tof_integ = np.zeros_like(self.tof_matrix[:, 0])
         else:
-            atoms.tof_data = np.dot(self.tof_matrix, (atoms.procstat - self.procstat) / delta_t / size)
-            atoms.tof_integ = np.dot(self.tof_matrix, (atoms.integ_rates - self.integ_rates) / delta_t / size)
+            d_ps, d_ig = atoms.procstat - self.procstat, atoms.integ_rates - self.integ_rates
+            atoms.tof_data = interval_tof(self.tof_matrix, d_ps, d_ig, delta_t, size, "procrates")
+            atoms.tof_integ = interval_tof(self.tof_matrix, d_ps, d_ig, delta_t, size, "integ")
         atoms.delta_t = delta_t
         self.procstat[:] = atoms.procstat
         self.integ_rates[:] = atoms.integ_rates
@@ -208,8 +279,7 @@ class KMC_Model:
         return " ".join(self.tofs)
 
     def get_std_header(self):
-        return "#%s %s %s kmc_time simulated_time kmc_steps\n" % (
-            self.get_param_header(), self.get_tof_header(), self.get_occupation_header())
+        return std_header(self.settings, self.compiled)
 
     def get_std_sampled_data(self, samples, sample_size, tof_method="integ", output="str",
                              show_progress=False):
@@ -230,18 +300,9 @@ class KMC_Model:
                 tofs.append(atoms.tof_integ.flatten())
             else:
                 raise NotImplementedError('tof_method="%s" not supported' % tof_method)
-        occs_mean = np.average(occs, axis=0, weights=step_ts)
-        tof_mean = np.average(tofs, axis=0, weights=delta_ts)
-        total_time = self.base.get_kmc_time() - t0
-        simulated_time = self.base.get_kmc_time()
-        total_steps = self.base.get_kmc_step() - step0
-        outdata = tuple(atoms.params + list(tof_mean.flatten()) + list(occs_mean.flatten())
-                        + [total_time, simulated_time, total_steps])
-        if output == "str":
-            return (" ".join(["%.5e"] * len(outdata)) + "\n") % outdata
-        if output == "dict":
-            return dict(zip(self.get_std_header()[1:].split(), outdata))
-        raise UserWarning("Output format %s not defined. I only know 'str' and 'dict'" % output)
+        t_end = self.base.get_kmc_time()
+        return std_sampled_output(atoms.params, tofs, occs, delta_ts, step_ts, t_end - t0, t_end,
+                                  self.base.get_kmc_step() - step0, self.get_std_header(), output)
 
     # -------------------------------------------------------------------
     def _get_configuration(self):
