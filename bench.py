@@ -129,7 +129,7 @@ def _ref_worker(args):
     return steps, secs
 
 
-def reference_cpu_baseline(wl, per_core=4):
+def reference_cpu_baseline(wl, per_core=12):
     """The reference's generated Fortran (oracle/_ref, built from the
     reference's checked-in export), one process per host core, each running
     whole replicas of the workload through the reference's do_kmc_steps."""
@@ -228,6 +228,11 @@ def calibrate_copy(nbytes=1 << 30, reps=5):
     return 0
 
 
+def _progress(msg):
+    """One line per finished leg on stderr (stdout carries only the JSON line)."""
+    print("bench.py: %s" % msg, file=sys.stderr, flush=True)
+
+
 def make_batch(engine, model, wl, idx=None):
     idx = np.arange(wl.n) if idx is None else np.asarray(idx)
     b = engine.Batch(model, [list(s) for s in wl.sizes[idx]])
@@ -279,6 +284,7 @@ def main():
     model = engine.Model(cm)
     K = max(1, a.steps)
     b, wall, executed, kernel_ms, chunk = run_batched(engine, model, wl, K, a.warmup)
+    _progress("batched %s: %d launches timed" % (wl.name, K))
     t_after, k_after, st_after = b.scalars_all()
     value = executed / wall
     avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
@@ -335,13 +341,17 @@ def main():
                      "algorithmic_bytes_per_kmc_step": bytes_per_step,
                      "kernel": "kmc_steps_kernel", "avg_launch_ms": avg_kernel_s * 1e3},
     }
+    _progress("plain path and spot check done")
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, cm, a.cpu_seconds)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+        _progress("cpu baseline done")
     if not a.no_ruo2 and a.workload == "zgb":
         out["ruo2"] = ruo2_leg(engine, K, a.no_cpu)
+        _progress("ruo2 leg done")
     if not a.no_serve:
         out["serving"] = serving_leg(a.serve_rate, a.serve_requests)
+        _progress("serving leg done")
         out["p50_latency_s"] = out["serving"]["p50_latency_s"]
         out["p99_latency_s"] = out["serving"]["p99_latency_s"]
     print(json.dumps(out))

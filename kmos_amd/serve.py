@@ -4,16 +4,19 @@ A request is one kMC trajectory: a model, a lattice size, its rate
 constants, a seed, a step budget and a sampling interval (the quantities a
 ``KMC_Model`` user sets before ``get_std_sampled_data``, kmos/run/__init__.py:
 865-966).  The server keeps every admitted trajectory resident in HBM as a
-*slot* of a *page* (an engine ``Batch`` of equal-sized replicas of one
-model) and advances every page by one sampling interval at a time, each page at
-its own pace:
+*slot* of a *page* (an engine ``Batch`` of one model holding a number of
+slots of every lattice size) and advances every page a few sampling
+intervals at a time, each page at its own pace:
 
-  loop:  admit arrived requests into free slots (new pages while the HBM
-         budget allows), initialise them on the device
-         -> every idle page with active slots queues its next interval: one
-            run and one device gather of a sample record per active slot
-            (each page has its own stream, so pages overlap on the GPU)
-         -> pages whose interval finished hand over their records; finished
+  loop:  admit arrived requests into free slots of their size (new pages
+         while the HBM budget allows), initialise them on the device
+         -> every idle page with active slots queues its next launch: up to
+            `intervals_per_launch` intervals of every active slot, with the
+            sample records written by the kernel itself at each interval's
+            end (one interval and a record gather when the page's slots
+            sample at different intervals); each page has its own stream,
+            so pages overlap on the GPU
+         -> pages whose launch finished hand over their records; finished
             slots retire and their requests complete now
 
 Requests that arrive while others run join at the next tick, and a slot
