@@ -150,6 +150,34 @@ int kmcg_get_avail(kmcg_batch *b, int32_t r, int32_t *sites, int32_t *back);
  * every other slot and back entry 0; anything else is rejected with
  * KMCG_EINVAL before any byte reaches the device.  The replica's status
  * is reset to running; its seed and stream are kept. */
+/* Displacement tracking for mean squared displacements (kmos/run/acf.py
+ * initialize_msd / do_kmc_steps_displacement / get_displacement /
+ * get_trajectory / set_displacement_to_zero over base_acf.mpy:217-361,
+ * 825-891 and proclist_generic_subroutines_acf.mpy:143-219).
+ *
+ * kmcg_msd_init numbers the sites holding `trace_species` 1.. in site order
+ * (initialize_mean_squared_displacement) for replicas [r0, r0+n) and zeroes
+ * their displacements.  `hops` [P][9] is get_diff_sites_displacement as a
+ * table (kmos/io/acf.py:443-671): per process a flag (1: the process moves
+ * one particle), then the site it leaves and the site it reaches, each as
+ * a cell offset dx, dy, dz from the chosen site's cell and a site type n
+ * (1-based); `disp` [P][3] is each hop's displacement vector. */
+int kmcg_msd_init(kmcg_batch *b, int32_t r0, int32_t n, int32_t trace_species, const int32_t *hops,
+                  const double *disp);
+/* allocate_trajectory: record the next nr_of_steps steps of every replica
+ * run with traj_on (one event per step: particle moved, 0 if none, and its
+ * new site). */
+int kmcg_msd_trajectory(kmcg_batch *b, int64_t nr_of_steps);
+/* do_kmc_steps_displacement(n, traj_on): kmcg_run with tracking. */
+int kmcg_run_displacement(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps, int64_t *budget,
+                          int32_t traj_on);
+/* set_displacement_to_zero for replicas [r0, r0+n) */
+int kmcg_msd_zero(kmcg_batch *b, int32_t r0, int32_t n);
+/* nr_of_ions, site_arr[V+1], id_arr[V+1], displacement[V+1][3] (row 0 and
+ * rows past nr_of_ions included), events recorded and the events [2*n]
+ * (any pointer may be NULL) */
+int kmcg_msd_get(kmcg_batch *b, int32_t r, int32_t *nr_of_ions, int32_t *site_arr, int32_t *id_arr,
+                 double *displacement, int64_t *n_events, int32_t *events);
 int kmcg_set_state(kmcg_batch *b, int32_t r, const double *time2, const int64_t *step2,
                    const int8_t *lattice, const int32_t *nr_of_sites, const int64_t *procstat,
                    const double *integ_rates, const int32_t *sites, const int32_t *back);

@@ -48,6 +48,11 @@ SIGNATURES = [
     ("kmcg_get_lattice", _i32, [_vp, _i32, _vp]),
     ("kmcg_get_avail", _i32, [_vp, _i32, _vp, _vp]),
     ("kmcg_set_state", _i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("kmcg_msd_init", _i32, [_vp, _i32, _i32, _i32, _vp, _vp]),
+    ("kmcg_msd_trajectory", _i32, [_vp, _i64]),
+    ("kmcg_run_displacement", _i32, [_vp, _vp, _i32, _i64, _vp, _i32]),
+    ("kmcg_msd_zero", _i32, [_vp, _i32, _i32]),
+    ("kmcg_msd_get", _i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("kmcg_get_occupation", _i32, [_vp, _i32, _i32, _vp]),
     ("kmcg_get_counters", _i32, [_vp, _i32, _i32, _vp, _vp]),
     ("kmcg_sample", _i32, [_vp, _vp, _i32, _vp]),

@@ -1,0 +1,171 @@
+"""Mean squared displacement of a traced species (kmos/run/acf.py, MSD part).
+
+kmos records, per kMC step, which particle of a traced species a diffusion
+process moves and by how much (do_kmc_steps_displacement,
+kmos/fortran_src/proclist_generic_subroutines_acf.mpy:143-219, over the
+base_acf module, base_acf.mpy:217-361 and 825-891).  The site pair and the
+displacement of every process come from a generated routine,
+get_diff_sites_displacement (kmos/io/acf.py:443-671); here the same
+information is a table computed once on the host (``displacement_table``)
+and the engine applies it inside the step kernel (kmcg_run_displacement):
+
+    from kmos_amd import acf
+    acf.initialize_msd(model, "ion")
+    acf.allocate_trajectory(model, 1000)
+    acf.do_kmc_steps_displacement(model, 1000, traj_on=True)
+    acf.calc_msd(model), acf.get_displacement(model), acf.get_trajectory(model)
+
+``model`` is a kmos_amd.run.KMC_Model (replica 0 of its batch).  Only the
+local_smart backend's routine is restated.  A process is a *hop* when, in
+the generated routine, one action takes the particle away (the site becomes
+the default species) and one puts a particle down on a site that held the
+default species, with no action that leaves its species unchanged (kmos's
+"source"/"drain" processes) and no other species change; for such models
+(pure diffusion) the trajectory is the reference's bit for bit.  For any
+other process the reference routine leaves the site pair undefined
+(uninitialised variables); here it moves no particle.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .compiler import BACKEND_LOCAL_SMART, CompiledModel
+
+HOP_W = 9
+
+
+def _f32(v) -> float:
+    """A default-REAL literal of the generated lattice module (single
+    precision, widened to double on assignment)."""
+    return float(np.float32(v))
+
+
+def displacement_table(cm: CompiledModel):
+    """(hops int32[P][9], disp float64[P][3]) for kmcg_msd_init: per process
+    the hop flag, the left site (dx, dy, dz, n) and the reached site
+    relative to the chosen site's cell, and the displacement
+    matmul(unit_cell_size, cell difference + site position difference)."""
+    if cm.backend != BACKEND_LOCAL_SMART:
+        raise NotImplementedError("displacement tracking is restated for the local_smart backend only")
+    pt = cm.project
+    default = pt.default_species
+    cell = [[_f32(v) for v in row] for row in pt.cell]
+    pos = [[_f32(v) for v in s.pos] for layer in pt.layers for s in layer.sites]
+    P = cm.nr_of_proc
+    hops = np.zeros((P, HOP_W), np.int32)
+    disp = np.zeros((P, 3), np.float64)
+    for p, proc in enumerate(pt.processes):
+        ex = proc.executing_coord()
+
+        def previous(action):
+            return [c.species for c in proc.conditions if c.coord == action.coord][0]
+
+        source = any(a.species == previous(a) for a in proc.actions)
+        old = new = None
+        ok = True
+        for a in proc.actions:
+            prev = previous(a)
+            rel = (a.coord.offset[0] - ex.offset[0], a.coord.offset[1] - ex.offset[1],
+                   a.coord.offset[2] - ex.offset[2], pt.site_index(a.coord.layer, a.coord.name))
+            if a.species[:1] in ("^", "$"):
+                ok = False
+            elif a.species == default and a.species != prev:
+                if source:
+                    ok = False                      # drain_process
+                else:
+                    old = rel
+            elif prev != a.species:
+                if prev != default:
+                    ok = False                      # the routine would call take_<prev> here
+                if source:
+                    ok = False                      # source_process
+                else:
+                    new = rel
+        if not (ok and old is not None and new is not None):
+            continue
+        hops[p] = (1,) + old + new
+        v = [float(new[c] - old[c]) + (pos[new[3] - 1][c] - pos[old[3] - 1][c]) for c in range(3)]
+        for i in range(3):
+            acc = 0.0
+            for j in range(3):
+                acc = acc + cell[i][j] * v[j]
+            disp[p, i] = acc
+    return hops, disp
+
+
+# ---- kmos.run.acf functions (MSD part), on replica 0 of a KMC_Model -------
+
+def _batch(model):
+    return model._kmc._b()
+
+
+def initialize_msd(model, species):
+    """Number the sites holding `species` 1.. and zero the displacements."""
+    trace = getattr(model.proclist, species.lower())
+    hops, disp = displacement_table(model.compiled)
+    _batch(model).msd_init(int(trace), hops, disp, 0, 1)
+    model._msd_traj0 = None
+
+
+def allocate_trajectory(model, nr_of_steps):
+    """Record the positions of every tracked particle for the next
+    nr_of_steps steps run with traj_on."""
+    b = _batch(model)
+    b.msd_trajectory(int(nr_of_steps))
+    st = b.msd_state(0)
+    model._msd_traj0 = (int(nr_of_steps), st["site_arr"][: st["nr_of_ions"] + 1].copy())
+
+
+def do_kmc_steps_displacement(model, n, traj_on=False):
+    """proclist_acf.do_kmc_steps_displacement(n, traj_on); a float n is
+    truncated like the f2py call."""
+    _batch(model).run_displacement(int(n), [0], traj_on)
+    model._kmc._check_status()
+
+
+def _msd_state(model):
+    return _batch(model).msd_state(0)
+
+
+def get_displacement(model):
+    """[1, nr_of_ions, 3] displacement of each tracked particle."""
+    st = _msd_state(model)
+    return st["displacement"][1: st["nr_of_ions"] + 1][None].copy()
+
+
+def calc_msd(model):
+    """calc_mean_squared_disp (base_acf.mpy:892-915): the particles' summed
+    squared displacements, in particle order, over nr_of_ions."""
+    st = _msd_state(model)
+    n = st["nr_of_ions"]
+    msd = 0.0
+    for k in range(1, n + 1):
+        d = st["displacement"][k]
+        msd = msd + ((d[0] ** 2 + d[1] ** 2) + d[2] ** 2)
+    return msd / n
+
+
+def set_displacement_to_zero(model):
+    _batch(model).msd_zero(0, 1)
+
+
+def get_trajectory(model):
+    """[1, nr_of_ions, nr_of_steps + 1] site of every tracked particle after
+    each recorded step (allocate_trajectory's positions in column 0;
+    update_trajectory, base_acf.mpy:825-830); columns of steps not run yet
+    stay 0, as in the reference's zero-filled array."""
+    if getattr(model, "_msd_traj0", None) is None:
+        raise RuntimeError("allocate_trajectory first")
+    nsteps, first = model._msd_traj0
+    st = _msd_state(model)
+    n = st["nr_of_ions"]
+    traj = np.zeros((n + 1, nsteps + 1), np.int64)
+    traj[: len(first), 0] = first
+    cur = traj[:, 0].copy()
+    ev = st["events"]
+    for j in range(min(nsteps, len(ev))):
+        p, s = int(ev[j, 0]), int(ev[j, 1])
+        cur[p] = s
+        cur[0] = 0
+        traj[:, j + 1] = cur
+    return traj[1:][None]

@@ -126,7 +126,19 @@ struct KParams {
     int sample_every, srec_cap, srec_width;
     int mode;                      // misc kernel mode
     int arg0, arg1, arg2, arg3;    // misc kernel arguments
+    // displacement tracking (msd_hop null: off)
+    const int *msd_hop;            // [P][MSD_HOP_W]: hop?, from dx dy dz n, to dx dy dz n
+    const double *msd_d;           // [P][3] displacement of each hop
+    int *msd_id;                   // id_arr, [V+1] per replica at msd_off
+    int *msd_site;                 // site_arr, [V+1] per replica at msd_off
+    double *msd_disp;              // displacement, [V+1][3] per replica at 3 * msd_off
+    const long long *msd_off;      // per replica
+    int *msd_ev;                   // trajectory events (particle, site), [msd_evcap] per replica
+    long long *msd_evn;            // events recorded per replica
+    long long msd_evcap;           // 0: no trajectory
 };
+
+constexpr int MSD_HOP_W = 9;
 
 // ---- helpers --------------------------------------------------------------
 __device__ __forceinline__ int wrap1(int a, int n)

@@ -877,7 +877,55 @@ __device__ __forceinline__ void write_record(const Wave<PPL, BITS> &W, const KPa
 }
 
 // do_kmc_steps for one replica per wave (proclist_generic_subroutines.mpy:1-44)
-template <int PPL, int BITS, bool LDS_LAT>
+// Displacement tracking of one step (do_kmc_steps_displacement,
+// proclist_generic_subroutines_acf.mpy:143-219): get_diff_sites_displacement
+// for the chosen (process, site) from the hop table (the site pair and the
+// displacement kmos/io/acf.py:443-671 generates), then assign_particle_id,
+// update_displacement, update_id_arr and, when a trajectory is recorded,
+// the step's event (base_acf.mpy:825-891).  A process that is no hop moves
+// no particle.  Lane 0 does the work.
+template <int PPL, int BITS>
+__device__ __forceinline__ void msd_track(const Wave<PPL, BITS> &W, const KParams &kp, int r, int proc0, int site)
+{
+    if (W.lane != 0)
+        return;
+    const int *h = kp.msd_hop + MSD_HOP_W * proc0;
+    int p = 0, fin = 0;
+    if (h[0]) {
+        const int s0 = site - 1;
+        const int cell = divm(s0, W.ms);
+        const int yz = divm(cell, W.mx);
+        const int x = cell - yz * W.d.X;
+        const int z = divm(yz, W.my);
+        const int y = yz - z * W.d.Y;
+        const int init = site0(W.d, x, y, z, h[1], h[2], h[3], h[4]) + 1;
+        fin = site0(W.d, x, y, z, h[5], h[6], h[7], h[8]) + 1;
+        const long long o = kp.msd_off[r];
+        int *id = kp.msd_id + o;
+        int *sa = kp.msd_site + o;
+        p = id[init];
+        double *dp = kp.msd_disp + 3 * (o + p);
+        const double *d = kp.msd_d + 3 * proc0;
+        dp[0] = dp[0] + d[0];
+        dp[1] = dp[1] + d[1];
+        dp[2] = dp[2] + d[2];
+        id[init] = 0;
+        id[fin] = p;
+        sa[p] = fin;
+        sa[0] = 0;
+    }
+    if (kp.msd_evcap > 0) {
+        const long long k = kp.msd_evn[r];
+        if (k < kp.msd_evcap) {
+            int *ev = kp.msd_ev + 2 * ((long long)r * kp.msd_evcap + k);
+            ev[0] = p;
+            ev[1] = fin;
+            kp.msd_evn[r] = k + 1;
+        }
+    }
+}
+
+template <int PPL, int BITS, bool LDS_LAT, bool MSD>
 __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4))) void kmc_steps_kernel(KParams kp)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -991,6 +1039,8 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
         uj += 1;
         const int site = readlane_i(sv, pl);
         done += 1;
+        if (MSD)
+            msd_track<PPL, BITS>(W, kp, r, pick, site);
         int nr_before[PPL];
 #pragma unroll
         for (int k = 0; k < PPL; ++k)
@@ -1394,6 +1444,17 @@ struct kmcg_batch {
     int n_last = 0;
     float last_ms = 0.f;
     bool timed_run_pending = false;   // ev0/ev1 bracket a launch not yet read
+    // displacement tracking (kmcg_msd_*): allocated by the first kmcg_msd_init
+    int *msd_hop = nullptr;
+    double *msd_d = nullptr;
+    int *msd_id = nullptr, *msd_site = nullptr;
+    double *msd_disp = nullptr;
+    long long *msd_off_d = nullptr;
+    std::vector<long long> msd_off;
+    std::vector<int> msd_nions;         // tracked particles per replica (0: not initialised)
+    int *msd_ev = nullptr;
+    long long *msd_evn = nullptr;
+    long long msd_evcap = 0;
 };
 
 namespace {
@@ -1968,11 +2029,16 @@ template <int PPL, int BITS>
 hipError_t launch_t(bool steps, const Shape &s, hipStream_t st, const KParams &kp)
 {
     const dim3 grid(s.grid), block(64 * s.wpg);
-    if (steps) {
+    if (steps && kp.msd_hop) {
         if (s.lds_lat)
-            hipLaunchKernelGGL((kmc_steps_kernel<PPL, BITS, true>), grid, block, s.lds, st, kp);
+            hipLaunchKernelGGL((kmc_steps_kernel<PPL, BITS, true, true>), grid, block, s.lds, st, kp);
         else
-            hipLaunchKernelGGL((kmc_steps_kernel<PPL, BITS, false>), grid, block, s.lds, st, kp);
+            hipLaunchKernelGGL((kmc_steps_kernel<PPL, BITS, false, true>), grid, block, s.lds, st, kp);
+    } else if (steps) {
+        if (s.lds_lat)
+            hipLaunchKernelGGL((kmc_steps_kernel<PPL, BITS, true, false>), grid, block, s.lds, st, kp);
+        else
+            hipLaunchKernelGGL((kmc_steps_kernel<PPL, BITS, false, false>), grid, block, s.lds, st, kp);
     } else {
         if (s.lds_lat)
             hipLaunchKernelGGL((kmc_misc_kernel<PPL, BITS, true>), grid, block, s.lds, st, kp);
@@ -2168,7 +2234,8 @@ void kmcg_batch_free(kmcg_batch *b)
     void *ptrs[] = {b->prog, b->dims, b->lat_off_d, b->avail_off_d, b->seed, b->strm, b->lat_pool,
                     b->avail_pool, b->nr, b->rates, b->integ, b->pstat, b->ktime, b->kdt, b->kstep,
                     b->rngctr, b->status, b->executed, b->budget, b->rep_ids, b->out, b->occ,
-                    b->samples};
+                    b->samples, b->msd_hop, b->msd_d, b->msd_id, b->msd_site, b->msd_disp,
+                    b->msd_off_d, b->msd_ev, b->msd_evn};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -2414,8 +2481,9 @@ int kmcg_set_config(kmcg_batch *b, int32_t r0, int32_t n, const int8_t *config)
     return run_misc(b, ids, MODE_ADJUST, 0, 0, 0, 0);
 }
 
+// track: 0 no displacement tracking, 1 tracking, 2 tracking and trajectory
 static int run_impl(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps, int64_t *budget,
-                    int sample_every, int max_records, double *records)
+                    int sample_every, int max_records, double *records, int track = 0)
 {
     if (!b || !replicas || n < 1 || n > b->n || steps < 0)
         return fail(KMCG_EINVAL, "bad run arguments");
@@ -2435,6 +2503,17 @@ static int run_impl(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t s
     const Shape s = launch_shape(b, replicas, n, kp, b->max_wpg);
     kp.nsteps = steps;
     kp.budget = budget ? b->budget : nullptr;
+    if (track) {
+        kp.msd_hop = b->msd_hop;
+        kp.msd_d = b->msd_d;
+        kp.msd_id = b->msd_id;
+        kp.msd_site = b->msd_site;
+        kp.msd_disp = b->msd_disp;
+        kp.msd_off = b->msd_off_d;
+        kp.msd_ev = b->msd_ev;
+        kp.msd_evn = b->msd_evn;
+        kp.msd_evcap = track == 2 ? b->msd_evcap : 0;
+    }
     const int width = 4 + b->m->S * b->m->spuck + 2 * b->m->P;
     if (records) {
         const size_t need = (size_t)n * max_records * width;
@@ -2746,6 +2825,170 @@ int kmcg_get_avail(kmcg_batch *b, int32_t r, int32_t *sites, int32_t *back)
         std::memcpy(back + p * (V1 - 1), blk.data() + (P + p) * V1 + 1, (V1 - 1) * sizeof(int));
     }
     return KMCG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// displacement tracking (base_acf.mpy:217-361, 825-891)
+// ---------------------------------------------------------------------------
+
+int kmcg_msd_init(kmcg_batch *b, int32_t r0, int32_t n, int32_t trace_species, const int32_t *hops,
+                  const double *disp)
+{
+    if (!b || r0 < 0 || n < 1 || r0 + n > b->n || !hops || !disp)
+        return fail(KMCG_EINVAL, "bad msd_init arguments");
+    const kmcg_model *m = b->m;
+    const int P = m->P;
+    if (trace_species < 0 || trace_species >= m->S)
+        return fail(KMCG_EINVAL, "msd_init: unknown trace species");
+    // every hop's two sites must stay within one periodic image of any
+    // replica's lattice (site0 wraps once)
+    int minX = 1 << 30, minY = 1 << 30, minZ = 1 << 30;
+    for (int r = 0; r < b->n; ++r) {
+        minX = std::min(minX, b->X[r]);
+        minY = std::min(minY, b->Y[r]);
+        minZ = std::min(minZ, b->Z[r]);
+    }
+    for (int p = 0; p < P; ++p) {
+        const int32_t *h = hops + MSD_HOP_W * p;
+        if (h[0] != 0 && h[0] != 1)
+            return fail(KMCG_EINVAL, "msd_init: hop flag must be 0 or 1");
+        if (!h[0])
+            continue;
+        for (int e = 0; e < 2; ++e) {
+            const int32_t *q = h + 1 + 4 * e;
+            if (std::abs(q[0]) >= minX || (m->dim >= 2 && std::abs(q[1]) >= minY) ||
+                (m->dim >= 3 && std::abs(q[2]) >= minZ) || (m->dim < 2 && q[1] != 0) ||
+                (m->dim < 3 && q[2] != 0) || q[3] < 1 || q[3] > m->spuck)
+                return fail(KMCG_EINVAL, "msd_init: hop site outside the lattice's reach");
+        }
+        for (int c = 0; c < 3; ++c)
+            if (!std::isfinite(disp[3 * p + c]))
+                return fail(KMCG_EINVAL, "msd_init: displacements must be finite");
+    }
+    int rc = settle(b);
+    if (rc)
+        return rc;
+    if (!b->msd_hop) {
+        long long tot = 0;
+        b->msd_off.assign(b->n, 0);
+        for (int r = 0; r < b->n; ++r) {
+            b->msd_off[r] = tot;
+            tot += b->V[r] + 1;
+        }
+        b->msd_nions.assign(b->n, 0);
+        if (hipMalloc((void **)&b->msd_hop, sizeof(int) * MSD_HOP_W * P) != hipSuccess ||
+            hipMalloc((void **)&b->msd_d, sizeof(double) * 3 * P) != hipSuccess ||
+            hipMalloc((void **)&b->msd_id, sizeof(int) * tot) != hipSuccess ||
+            hipMalloc((void **)&b->msd_site, sizeof(int) * tot) != hipSuccess ||
+            hipMalloc((void **)&b->msd_disp, sizeof(double) * 3 * tot) != hipSuccess ||
+            hipMalloc((void **)&b->msd_off_d, sizeof(long long) * b->n) != hipSuccess ||
+            hipMalloc((void **)&b->msd_evn, sizeof(long long) * b->n) != hipSuccess)
+            return fail(KMCG_ENOMEM, "displacement tracking buffers");
+        HIPCHK(hipMemcpy(b->msd_off_d, b->msd_off.data(), sizeof(long long) * b->n, hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(b->msd_evn, 0, sizeof(long long) * b->n));
+        // replicas never initialised track particle 0 only: their ids must be 0
+        HIPCHK(hipMemset(b->msd_id, 0, sizeof(int) * tot));
+        HIPCHK(hipMemset(b->msd_site, 0, sizeof(int) * tot));
+        HIPCHK(hipMemset(b->msd_disp, 0, sizeof(double) * 3 * tot));
+    }
+    HIPCHK(hipMemcpy(b->msd_hop, hops, sizeof(int) * MSD_HOP_W * P, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->msd_d, disp, sizeof(double) * 3 * P, hipMemcpyHostToDevice));
+    // initialize_mean_squared_displacement: particles numbered 1.. in site order
+    for (int r = r0; r < r0 + n; ++r) {
+        const int V = b->V[r];
+        std::vector<int8_t> lat(V);
+        if ((rc = kmcg_get_lattice(b, r, lat.data())))
+            return rc;
+        std::vector<int> id(V + 1, 0), sa(V + 1, 0);
+        int k = 0;
+        for (int s = 1; s <= V; ++s)
+            if (lat[s - 1] == trace_species) {
+                ++k;
+                id[s] = k;
+                sa[k] = s;
+            }
+        b->msd_nions[r] = k;
+        const long long o = b->msd_off[r];
+        std::vector<double> zero(3 * (size_t)(V + 1), 0.0);
+        const long long none = 0;
+        if ((rc = copy_in(b, b->msd_id + o, id.data(), id.size())) ||
+            (rc = copy_in(b, b->msd_site + o, sa.data(), sa.size())) ||
+            (rc = copy_in(b, b->msd_disp + 3 * o, zero.data(), zero.size())) ||
+            (rc = copy_in(b, b->msd_evn + r, &none, 1)))
+            return rc;
+    }
+    return KMCG_OK;
+}
+
+int kmcg_msd_trajectory(kmcg_batch *b, int64_t nr_of_steps)
+{
+    if (!b || nr_of_steps < 0 || nr_of_steps > (1ll << 31))
+        return fail(KMCG_EINVAL, "bad trajectory length");
+    if (!b->msd_hop)
+        return fail(KMCG_EINVAL, "msd_trajectory before msd_init");
+    int rc = settle(b);
+    if (rc)
+        return rc;
+    if (b->msd_ev)
+        HIPCHK(hipFree(b->msd_ev));
+    b->msd_ev = nullptr;
+    b->msd_evcap = 0;
+    if (nr_of_steps > 0) {
+        if (hipMalloc((void **)&b->msd_ev, sizeof(int) * 2 * (size_t)nr_of_steps * b->n) != hipSuccess)
+            return fail(KMCG_ENOMEM, "trajectory buffer");
+        b->msd_evcap = nr_of_steps;
+    }
+    HIPCHK(hipMemset(b->msd_evn, 0, sizeof(long long) * b->n));
+    return KMCG_OK;
+}
+
+int kmcg_msd_zero(kmcg_batch *b, int32_t r0, int32_t n)
+{
+    if (!b || r0 < 0 || n < 1 || r0 + n > b->n)
+        return fail(KMCG_EINVAL, "bad replica range");
+    if (!b->msd_hop)
+        return fail(KMCG_EINVAL, "msd_zero before msd_init");
+    int rc = settle(b);
+    if (rc)
+        return rc;
+    const long long o0 = b->msd_off[r0];
+    const long long o1 = b->msd_off[r0 + n - 1] + b->V[r0 + n - 1] + 1;
+    HIPCHK(hipMemset(b->msd_disp + 3 * o0, 0, sizeof(double) * 3 * (size_t)(o1 - o0)));
+    return KMCG_OK;
+}
+
+int kmcg_msd_get(kmcg_batch *b, int32_t r, int32_t *nr_of_ions, int32_t *site_arr, int32_t *id_arr,
+                 double *displacement, int64_t *n_events, int32_t *events)
+{
+    int rc = check_replica(b, r);
+    if (rc)
+        return rc;
+    if (!b->msd_hop)
+        return fail(KMCG_EINVAL, "msd_get before msd_init");
+    const long long o = b->msd_off[r], V1 = b->V[r] + 1;
+    long long ne = 0;
+    if ((rc = copy_out(b, &ne, (const long long *)b->msd_evn + r, 1)))
+        return rc;
+    if (nr_of_ions)
+        *nr_of_ions = b->msd_nions[r];
+    if (n_events)
+        *n_events = ne;
+    if ((site_arr && (rc = copy_out(b, site_arr, (const int *)b->msd_site + o, (size_t)V1))) ||
+        (id_arr && (rc = copy_out(b, id_arr, (const int *)b->msd_id + o, (size_t)V1))) ||
+        (displacement && (rc = copy_out(b, displacement, (const double *)b->msd_disp + 3 * o, (size_t)(3 * V1)))))
+        return rc;
+    if (events && ne > 0 && b->msd_ev)
+        if ((rc = copy_out(b, events, (const int *)b->msd_ev + 2 * (size_t)r * b->msd_evcap, (size_t)(2 * ne))))
+            return rc;
+    return KMCG_OK;
+}
+
+int kmcg_run_displacement(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps, int64_t *budget,
+                          int32_t traj_on)
+{
+    if (!b || !b->msd_hop)
+        return fail(KMCG_EINVAL, "run_displacement before msd_init");
+    return run_impl(b, replicas, n, steps, budget, 0, 0, nullptr, traj_on ? 2 : 1);
 }
 
 int kmcg_set_state(kmcg_batch *b, int32_t r, const double *time2, const int64_t *step2,

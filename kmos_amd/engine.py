@@ -257,6 +257,50 @@ class Batch:
         if rates is not None:
             self.set_rates(rates[None, :], r0=r)
 
+    # ---- displacement tracking ----------------------------------------------
+    def msd_init(self, trace_species: int, hops, disp, r0: int = 0, n: Optional[int] = None):
+        """Number the traced species' sites of replicas [r0, r0+n) and zero
+        their displacements; hops/disp from acf.displacement_table."""
+        n = self.n - r0 if n is None else n
+        h = np.ascontiguousarray(hops, dtype=np.int32)
+        d = np.ascontiguousarray(disp, dtype=np.float64)
+        if h.shape != (self.P, 9) or d.shape != (self.P, 3):
+            raise ValueError("hop table must be [P][9] and [P][3]")
+        _lib.check(_lib.load().kmcg_msd_init(self.h, r0, n, int(trace_species), _ptr(h), _ptr(d)))
+
+    def msd_trajectory(self, nr_of_steps: int):
+        _lib.check(_lib.load().kmcg_msd_trajectory(self.h, int(nr_of_steps)))
+
+    def msd_zero(self, r0: int = 0, n: Optional[int] = None):
+        n = self.n - r0 if n is None else n
+        _lib.check(_lib.load().kmcg_msd_zero(self.h, r0, n))
+
+    def run_displacement(self, steps: int, replicas=None, traj_on: bool = False):
+        """do_kmc_steps_displacement: run() with displacement tracking."""
+        ids = np.arange(self.n, dtype=np.int32) if replicas is None else \
+            np.ascontiguousarray(np.atleast_1d(replicas), dtype=np.int32)
+        self._ids = ids
+        _lib.check(_lib.load().kmcg_run_displacement(self.h, _ptr(ids), ids.size, int(steps), None,
+                                                     1 if traj_on else 0))
+        self.sync()
+
+    def msd_state(self, r: int) -> dict:
+        """nr_of_ions, site_arr[V+1], id_arr[V+1], displacement[V+1][3] and the
+        recorded trajectory events [n][2] (particle, new site) of replica r."""
+        V1 = int(self.volumes[r]) + 1
+        ni = np.zeros(1, np.int32)
+        ne = np.zeros(1, np.int64)
+        sa = np.zeros(V1, np.int32)
+        ia = np.zeros(V1, np.int32)
+        dp = np.zeros((V1, 3), np.float64)
+        L = _lib.load()
+        _lib.check(L.kmcg_msd_get(self.h, r, _ptr(ni), _ptr(sa), _ptr(ia), _ptr(dp), _ptr(ne), None))
+        ev = np.zeros((max(int(ne[0]), 1), 2), np.int32)
+        if ne[0]:
+            _lib.check(L.kmcg_msd_get(self.h, r, None, None, None, None, None, _ptr(ev)))
+        return {"nr_of_ions": int(ni[0]), "site_arr": sa, "id_arr": ia, "displacement": dp,
+                "events": ev[: int(ne[0])]}
+
     def occupation(self, r0: int = 0, n: Optional[int] = None) -> np.ndarray:
         n = self.n - r0 if n is None else n
         out = np.zeros((n, self.S, self.spuck), np.float64)

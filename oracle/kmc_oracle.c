@@ -21,6 +21,8 @@
  *   do_kmc_step(s), get_next_kmc_step proclist_generic_subroutines.mpy:1-110
  *   get_occupation                    proclist_generic_subroutines.mpy:113-158
  *   reload_system (state overwrite)   base.mpy:363-514
+ *   displacement tracking (MSD)       base_acf.mpy:217-361,825-891,
+ *                                     proclist_generic_subroutines_acf.mpy:143-219
  *   initialize_state                  proclist_generic_subroutines.mpy:236-304
  *   put/take/touchup routines         kmos/io/__init__.py:2219-2443 (as data)
  *   lat_int run_proc/nli/touchup_cell kmos/io/__init__.py:1732-2057 (as data)
@@ -107,6 +109,14 @@ typedef struct {
     uint64_t rng_ctr;     /* steps' worth of uniforms drawn */
     int status;
     uint64_t touched;     /* avail entries read or written */
+    /* displacement tracking (base_acf.mpy:1-30 module variables) */
+    int32_t *msd_hops;    /* [P][9] hop table */
+    double *msd_d;        /* [P][3] */
+    int32_t *msd_id, *msd_site;  /* id_arr(0:V), site_arr(0:V) */
+    double *msd_disp;     /* displacement(0:V, 3) */
+    int msd_nions;
+    int32_t *msd_ev;      /* trajectory events (particle, site) */
+    int64_t msd_evcap, msd_evn;
 } orc_t;
 
 #define HDR(o, i) ((o)->blob[(i)])
@@ -359,6 +369,8 @@ void orc_free(orc_t *o)
     if (!o) return;
     free(o->blob); free(o->lat); free(o->sites); free(o->back); free(o->nr);
     free(o->rates); free(o->accum); free(o->integ); free(o->procstat);
+    free(o->msd_hops); free(o->msd_d); free(o->msd_id); free(o->msd_site); free(o->msd_disp);
+    free(o->msd_ev);
     free(o);
 }
 
@@ -550,6 +562,107 @@ int64_t orc_run(orc_t *o, int64_t n, int32_t *tr_proc, int32_t *tr_site, double 
         if (tr_proc) tr_proc[i] = p;
         if (tr_site) tr_site[i] = s;
         if (tr_time) tr_time[i] = o->kmc_time;
+        if (rc) { i += 1; break; }
+    }
+    return i;
+}
+
+/* ---------------- displacement tracking ---------------- */
+
+/* initialize_mean_squared_displacement (base_acf.mpy:314-361): particles
+ * numbered 1.. over the sites holding trace_species, in site order. */
+void orc_msd_init(orc_t *o, int trace_species, const int32_t *hops, const double *disp)
+{
+    free(o->msd_hops); free(o->msd_d); free(o->msd_id); free(o->msd_site); free(o->msd_disp);
+    o->msd_hops = malloc(sizeof(int32_t) * 9 * o->P);
+    o->msd_d = malloc(sizeof(double) * 3 * o->P);
+    memcpy(o->msd_hops, hops, sizeof(int32_t) * 9 * o->P);
+    memcpy(o->msd_d, disp, sizeof(double) * 3 * o->P);
+    o->msd_id = calloc(o->V + 1, sizeof(int32_t));
+    o->msd_site = calloc(o->V + 1, sizeof(int32_t));
+    o->msd_disp = calloc(3 * (size_t)(o->V + 1), sizeof(double));
+    int i = 1;
+    for (int k = 1; k <= o->V; ++k)
+        if (o->lat[k] == trace_species) {
+            o->msd_id[k] = i;
+            o->msd_site[i] = k;
+            i += 1;
+        }
+    o->msd_nions = i - 1;
+}
+
+/* allocate_trajectory (base_acf.mpy:427-458): events of the next n steps */
+void orc_msd_trajectory(orc_t *o, int64_t n)
+{
+    free(o->msd_ev);
+    o->msd_ev = n > 0 ? malloc(sizeof(int32_t) * 2 * (size_t)n) : NULL;
+    o->msd_evcap = n;
+    o->msd_evn = 0;
+}
+
+/* set_displacement_to_zero (base_acf.mpy:553-557) */
+void orc_msd_zero(orc_t *o)
+{
+    memset(o->msd_disp, 0, sizeof(double) * 3 * (size_t)(o->V + 1));
+}
+
+void orc_msd_get(const orc_t *o, int32_t *nions, int32_t *site_arr, int32_t *id_arr, double *disp,
+                 int64_t *nev, int32_t *ev)
+{
+    *nions = o->msd_nions;
+    memcpy(site_arr, o->msd_site, sizeof(int32_t) * (o->V + 1));
+    memcpy(id_arr, o->msd_id, sizeof(int32_t) * (o->V + 1));
+    memcpy(disp, o->msd_disp, sizeof(double) * 3 * (size_t)(o->V + 1));
+    *nev = o->msd_evn;
+    if (ev && o->msd_evn)
+        memcpy(ev, o->msd_ev, sizeof(int32_t) * 2 * (size_t)o->msd_evn);
+}
+
+/* do_kmc_steps_displacement (proclist_generic_subroutines_acf.mpy:143-219):
+ * the step of do_step, with get_diff_sites_displacement (the hop table),
+ * assign_particle_id, update_displacement and update_id_arr between
+ * determine_procsite and run_proc_nr, and update_trajectory after it
+ * (base_acf.mpy:825-891).  A process that is no hop moves no particle. */
+int64_t orc_run_displacement(orc_t *o, int64_t n, int traj_on)
+{
+    int64_t i;
+    if (o->status != ORC_OK || !o->msd_hops) return 0;
+    for (i = 0; i < n; ++i) {
+        double u[3];
+        update_accum_rate(o);
+        if (!(o->accum[o->P - 1] > 0.0)) {
+            o->status = ORC_ABSORBING;
+            break;
+        }
+        orc_step_uniforms(o->seed, o->stream, o->rng_ctr, u);
+        o->rng_ctr += 1;
+        o->kmc_time_step = -log(u[0]) / o->accum[o->P - 1];
+        o->kmc_time = o->kmc_time + o->kmc_time_step;
+        o->kmc_step += 1;
+        update_integ_rate(o);
+        int proc, site;
+        determine_procsite(o, u[1], u[2], &proc, &site);
+        const int32_t *h = o->msd_hops + 9 * (proc - 1);
+        int particle = 0, fin = 0;
+        if (h[0]) {
+            int c[4];
+            nr2lattice(o, site, c);
+            int init = lattice2nr(o, c[0] + h[1], c[1] + h[2], c[2] + h[3], h[4]);
+            fin = lattice2nr(o, c[0] + h[5], c[1] + h[6], c[2] + h[7], h[8]);
+            particle = o->msd_id[init];                                  /* assign_particle_id */
+            for (int d = 0; d < 3; ++d)                                  /* update_displacement */
+                o->msd_disp[3 * particle + d] = o->msd_disp[3 * particle + d] + o->msd_d[3 * (proc - 1) + d];
+            o->msd_id[init] = 0;                                         /* update_id_arr */
+            o->msd_id[fin] = particle;
+            o->msd_site[particle] = fin;
+            o->msd_site[0] = 0;
+        }
+        int rc = orc_run_proc_nr(o, proc, site);
+        if (traj_on && o->msd_evn < o->msd_evcap) {                      /* update_trajectory */
+            o->msd_ev[2 * o->msd_evn] = particle;
+            o->msd_ev[2 * o->msd_evn + 1] = fin;
+            o->msd_evn += 1;
+        }
         if (rc) { i += 1; break; }
     }
     return i;

@@ -67,6 +67,12 @@ def lib():
             getattr(L, name).argtypes = [vp, vp]
         L.orc_get_avail.argtypes = [vp, vp, vp]
         L.orc_set_state.argtypes = [vp, dbl, dbl, i64, u64, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_msd_init.argtypes = [vp, i32, vp, vp]
+        L.orc_msd_trajectory.argtypes = [vp, i64]
+        L.orc_msd_zero.argtypes = [vp]
+        L.orc_msd_get.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.orc_run_displacement.argtypes = [vp, i64, i32]
+        L.orc_run_displacement.restype = i64
         L.orc_get_species.argtypes = [vp, i32]
         L.orc_get_species.restype = i32
         L.orc_replace_species.argtypes = [vp, i32, i32, i32]
@@ -224,6 +230,33 @@ class Oracle:
                 "nr_of_sites": self.nr_of_sites(), "procstat": self.procstat(),
                 "integ_rates": self.integ_rates(), "rates": np.asarray(rates, np.float64),
                 "avail_sites": sites, "avail_sites_back": back}
+
+    # displacement tracking ------------------------------------------------
+    def msd_init(self, trace_species, hops, disp):
+        self._hops = np.ascontiguousarray(hops, np.int32)
+        self._disp = np.ascontiguousarray(disp, np.float64)
+        lib().orc_msd_init(self.h, int(trace_species), _ptr(self._hops), _ptr(self._disp))
+
+    def msd_trajectory(self, n):
+        lib().orc_msd_trajectory(self.h, int(n))
+
+    def msd_zero(self):
+        lib().orc_msd_zero(self.h)
+
+    def run_displacement(self, n, traj_on=False):
+        return lib().orc_run_displacement(self.h, int(n), 1 if traj_on else 0)
+
+    def msd_state(self):
+        ni = np.zeros(1, np.int32)
+        ne = np.zeros(1, np.int64)
+        sa = np.zeros(self.V + 1, np.int32)
+        ia = np.zeros(self.V + 1, np.int32)
+        dp = np.zeros((self.V + 1, 3), np.float64)
+        lib().orc_msd_get(self.h, _ptr(ni), _ptr(sa), _ptr(ia), _ptr(dp), _ptr(ne), None)
+        ev = np.zeros((max(int(ne[0]), 1), 2), np.int32)
+        lib().orc_msd_get(self.h, _ptr(ni), _ptr(sa), _ptr(ia), _ptr(dp), _ptr(ne), _ptr(ev))
+        return {"nr_of_ions": int(ni[0]), "site_arr": sa, "id_arr": ia, "displacement": dp,
+                "events": ev[: int(ne[0])]}
 
     def set_state(self, st):
         """reload_system: overwrite the state (rng_counter defaults to kmc_step)."""

@@ -1,0 +1,211 @@
+"""Displacement tracking (MSD) on the CPU: the hop table and the oracle.
+
+* the hop table of the 2d_grid model equals the site pairs of the
+  reference's generated get_diff_sites_displacement (tests/golden/
+  ref_hops_2d_grid.json, extracted from the reference's fixture by
+  extract_reference_hops.py), and its displacements are the site-to-site
+  vectors of the model's geometry;
+* on a pure-hop model with hops across cell borders, every tracked
+  particle's start position plus its accumulated displacement is its end
+  position up to whole lattice periods, particle ids and sites stay a
+  bijection onto the traced species' sites, and the recorded events replay
+  to the final positions;
+* processes that are no hop (adsorption, reaction) move no particle.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from kmos_amd.acf import displacement_table
+from kmos_amd.compiler import compile_model
+from kmos_amd.model import load_project, parse_ini
+from paritylib import oracle_replica
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+HOP_MODEL = """
+[Meta]
+author = test
+email = test@example.org
+model_name = hops
+model_dimension = 2
+debug = 0
+
+[SpeciesList]
+default_species = empty
+
+[Species empty]
+representation =
+color = #ffffff
+tags =
+
+[Species ion]
+representation = Atoms('F')
+color = #000000
+tags =
+
+[Lattice]
+cell_size = 3.0 0.0 0.0 0.0 2.0 0.0 0.0 0.0 10.0
+default_layer = g
+substrate_layer = g
+representation =
+
+[Layer g]
+color = #ffffff
+site a = (0.25, 0.25, 0.5); default_species;
+site b = (0.75, 0.625, 0.5); default_species;
+
+[Process a_b]
+rate_constant = 3.
+enabled = True
+conditions = ion@a + empty@b
+actions = empty@a + ion@b
+
+[Process b_a_right]
+rate_constant = 2.
+enabled = True
+conditions = ion@b + empty@a.(1, 0, 0)
+actions = empty@b + ion@a.(1, 0, 0)
+
+[Process a_a_up]
+rate_constant = 1.5
+enabled = True
+conditions = ion@a + empty@a.(0, 1, 0)
+actions = empty@a + ion@a.(0, 1, 0)
+
+[Process b_b_left]
+rate_constant = 1.
+enabled = True
+conditions = ion@b + empty@b.(-1, 0, 0)
+actions = empty@b + ion@b.(-1, 0, 0)
+
+[Process b_a_down]
+rate_constant = 0.7
+enabled = True
+conditions = ion@b + empty@a.(0, -1, 0)
+actions = ion@a.(0, -1, 0) + empty@b
+"""
+
+
+def hop_project(with_adsorption=False):
+    text = HOP_MODEL
+    if with_adsorption:
+        text += """
+[Process ion_adsorption_a]
+rate_constant = 0.5
+enabled = True
+conditions = empty@a
+actions = ion@a
+
+[Process ion_desorption_b]
+rate_constant = 0.5
+enabled = True
+conditions = ion@b
+actions = empty@b
+"""
+    return parse_ini(text)
+
+
+def test_hop_table_matches_reference_routine():
+    pt = load_project(os.path.join(GOLDEN, "2d_grid.xml"))
+    cm = compile_model(pt, "local_smart")
+    hops, disp = displacement_table(cm)
+    ref = json.load(open(os.path.join(GOLDEN, "ref_hops_2d_grid.json")))
+    assert sorted(ref) == sorted(cm.process_names)
+    for p, name in enumerate(cm.process_names):
+        ex = pt.processes[p].executing_coord()
+        for key, cols in (("init", slice(1, 5)), ("fin", slice(5, 9))):
+            dx, dy, dz, site = ref[name][key]
+            n = pt.site_index(ex.layer, ex.name) if site == "exec" else pt.site_index("default", site)
+            assert list(hops[p, cols]) == [dx, dy, dz, n], (name, key)
+        assert hops[p, 0] == 1
+    # site positions (fractions of the 3.5 x 3.5 cell): a_1 (.25,.25),
+    # a_2 (.75,.25), b_1 (.25,.75), b_2 (.75,.75)
+    expect = {"a_1_a_2": (1.75, 0, 0), "a_1_b_2": (1.75, 1.75, 0), "b_2_a_1": (-1.75, -1.75, 0),
+              "b_1_a_2": (1.75, -1.75, 0)}
+    for name, v in expect.items():
+        np.testing.assert_array_equal(disp[cm.process_names.index(name)], v)
+
+
+def test_hop_table_across_cells():
+    cm = compile_model(hop_project(), "local_smart")
+    hops, disp = displacement_table(cm)
+    row = dict(zip(cm.process_names, zip(hops.tolist(), disp.tolist())))
+    # b (0.75, 0.625) -> a of the cell to the right (1.25, 0.25): (+0.5, -0.375) x (3, 2);
+    # the executing coordinate (lowest sort_key) is a.(1,0,0), so b sits at -1
+    h, d = row["b_a_right"]
+    assert h == [1, -1, 0, 0, 2, 0, 0, 0, 1]
+    assert d == [1.5, -0.75, 0.0]
+    # the executing coordinate of b_a_down is its lowest-sort_key action, a.(0,-1,0)
+    h, d = row["b_a_down"]
+    assert h == [1, 0, 1, 0, 2, 0, 0, 0, 1]
+    assert d == [-1.5, -2.75, 0.0]
+    assert row["a_a_up"][1] == [0.0, 2.0, 0.0] and row["b_b_left"][1] == [-3.0, 0.0, 0.0]
+    hops2, _ = displacement_table(compile_model(hop_project(True), "local_smart"))
+    assert hops2[-2:, 0].tolist() == [0, 0]          # adsorption / desorption: no hop
+
+
+def _positions(cm, sites, size):
+    """Fractional positions (cell units) of 1-based sites."""
+    pos = np.array([s.pos for layer in cm.project.layers for s in layer.sites])
+    s0 = np.asarray(sites) - 1
+    n = s0 % cm.spuck
+    cell = s0 // cm.spuck
+    x, y = cell % size[0], cell // size[0]
+    return np.stack([x + pos[n, 0], y + pos[n, 1]], axis=1)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_oracle_tracking_invariants(oracle, seed):
+    cm = compile_model(hop_project(), "local_smart")
+    hops, disp = displacement_table(cm)
+    size = (7, 5)
+    g = np.random.default_rng(seed)
+    V = size[0] * size[1] * cm.spuck
+    ion = cm.species_names.index("ion")
+    config = np.where(g.random(V) < 0.3, ion, cm.species_names.index("empty")).astype(np.int8)
+    o = oracle_replica(oracle, cm, size, seed, 0, np.array([3, 2, 1.5, 1, 0.7]), config)
+    o.msd_init(ion, hops, disp)
+    st0 = o.msd_state()
+    n = st0["nr_of_ions"]
+    assert n == int((config == ion).sum())
+    start = st0["site_arr"][1:n + 1].copy()
+    o.msd_trajectory(3000)
+    assert o.run_displacement(3000, traj_on=True) == 3000
+    st = o.msd_state()
+    end = st["site_arr"][1:n + 1]
+    lat = o.lattice()
+    # ids <-> sites stay a bijection onto the ion sites
+    assert sorted(end) == sorted(np.flatnonzero(lat == ion) + 1)
+    assert all(st["id_arr"][s] == k + 1 for k, s in enumerate(end))
+    # start + displacement = end, up to whole periods of the 7 x 5 lattice
+    D = st["displacement"][1:n + 1, :2] / np.array([3.0, 2.0])
+    wrapped = _positions(cm, start, size) + D - _positions(cm, end, size)
+    periods = wrapped / np.array(size, float)
+    np.testing.assert_allclose(periods, np.round(periods), atol=1e-9)
+    assert np.abs(D).sum() > 0
+    # the events replay to the final positions
+    cur = np.concatenate([[0], start])
+    for p, s in st["events"]:
+        cur[p] = s
+        cur[0] = 0
+    np.testing.assert_array_equal(cur[1:], end)
+    assert len(st["events"]) == 3000
+
+
+def test_oracle_non_hops_move_no_particle(oracle):
+    cm = compile_model(hop_project(True), "local_smart")
+    hops, disp = displacement_table(cm)
+    o = oracle_replica(oracle, cm, (6, 6), 5, 0, np.array([3, 2, 1.5, 1, 0.7, 0.5, 0.5]))
+    ion = cm.species_names.index("ion")
+    o.msd_init(ion, hops, disp)
+    assert o.msd_state()["nr_of_ions"] == 0               # starts empty
+    o.msd_trajectory(500)
+    o.run_displacement(500, traj_on=True)
+    st = o.msd_state()
+    assert st["nr_of_ions"] == 0
+    # adsorbed ions were never numbered: every hop moves particle 0
+    assert set(st["events"][:, 0].tolist()) == {0}
+    assert not st["site_arr"].any() and not st["id_arr"].any()
