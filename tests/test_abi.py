@@ -81,3 +81,17 @@ def test_batch_create_without_gpu_fails_cleanly():
     sizes = np.array([[16, 16, 1]], np.int32)
     rc = _lib.load().kmcg_batch_create(h, 1, sizes.ctypes.data_as(ctypes.c_void_p), 0, ctypes.byref(b))
     assert rc == -1 and b"device" in _lib.load().kmcg_last_error()
+
+
+def test_many_process_models_load_or_are_rejected_cleanly():
+    """72 and 136 processes load as local_smart (two and four processes per
+    lane); as lat_int their op programs exceed 64 KiB and loading says so."""
+    from test_gpu_many_procs import DIRS4, DIRS8, many_process_project
+    for dirs, nproc in ((DIRS4, 72), (DIRS8, 136)):
+        pt = many_process_project(dirs)
+        cm = compile_model(pt, "local_smart")
+        rc, h, _ = _load(cm)
+        assert rc == 0 and cm.nr_of_proc == nproc, _lib.load().kmcg_last_error()
+        _lib.load().kmcg_model_free(h)
+        rc, _, _ = _load(compile_model(pt, "lat_int"))
+        assert rc == -4 and b"64 KiB" in _lib.load().kmcg_last_error()
