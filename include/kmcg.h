@@ -178,6 +178,37 @@ int kmcg_msd_zero(kmcg_batch *b, int32_t r0, int32_t n);
  * (any pointer may be NULL) */
 int kmcg_msd_get(kmcg_batch *b, int32_t r, int32_t *nr_of_ions, int32_t *site_arr, int32_t *id_arr,
                  double *displacement, int64_t *n_events, int32_t *events);
+/* Autocorrelation of a site property along the traced particles' paths
+ * (kmos/run/acf.py allocate_acf / set_property_acf / calc_product_property /
+ * initialize_acf / do_kmc_steps_acf / get_acf / set_acf_to_zero over
+ * base_acf.mpy:382-552,1053-1201 and proclist_generic_subroutines_acf.mpy:
+ * 1-98).
+ *
+ * kmcg_acf_alloc: allocate_tracing_arr(nr_of_types) and
+ * allocate_config_bin_acf(t_bin, t_f, safety, extending) for every replica
+ * (0 = the reference's default factor 2).  kmcg_acf_set_properties:
+ * property_acf per site for replicas [r0, r0+n) (V+1 values each, entry 0
+ * unused) and, when non-NULL, product_property [(T+1)][(T+1)]; products must
+ * be integers (|p| <= 2^20): sum(buffer_acf) is then exact as a running sum.
+ * kmcg_acf_init: initialize_acf(trace_species) with the hop table of
+ * kmcg_msd_init.  kmcg_run_acf: do_kmc_steps_acf(steps, traj_on).
+ * kmcg_acf_get: config_bin[extended bins], counter_write_in_bin,
+ * d4 = {sum(buffer_acf), t_o, wrap_count, t_bin}, i5 = {bin_index,
+ * nr_of_bins, extended bins, nr_of_ions_before_kmc_step_acf, nr_of_ions},
+ * buffer_acf[V+1], property_o[V+1] (any pointer may be NULL).  A step that
+ * would run past the last extended bin stops there (the reference writes
+ * past its array). */
+int kmcg_acf_alloc(kmcg_batch *b, int32_t nr_of_types, double t_bin, double t_f, int32_t safety_factor,
+                   int32_t extending_factor);
+int kmcg_acf_set_properties(kmcg_batch *b, int32_t r0, int32_t n, const int32_t *property_acf,
+                            const double *product_property);
+int kmcg_acf_init(kmcg_batch *b, int32_t r0, int32_t n, int32_t trace_species, const int32_t *hops);
+int kmcg_run_acf(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps, int64_t *budget,
+                 int32_t traj_on);
+/* set_acf_to_zero for replicas [r0, r0+n) */
+int kmcg_acf_zero(kmcg_batch *b, int32_t r0, int32_t n);
+int kmcg_acf_get(kmcg_batch *b, int32_t r, double *config_bin, int32_t *counter_write_in_bin, double *d4,
+                 int32_t *i5, double *buffer_acf, int32_t *property_o);
 int kmcg_set_state(kmcg_batch *b, int32_t r, const double *time2, const int64_t *step2,
                    const int8_t *lattice, const int32_t *nr_of_sites, const int64_t *procstat,
                    const double *integ_rates, const int32_t *sites, const int32_t *back);

@@ -53,6 +53,12 @@ SIGNATURES = [
     ("kmcg_run_displacement", _i32, [_vp, _vp, _i32, _i64, _vp, _i32]),
     ("kmcg_msd_zero", _i32, [_vp, _i32, _i32]),
     ("kmcg_msd_get", _i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("kmcg_acf_alloc", _i32, [_vp, _i32, _d, _d, _i32, _i32]),
+    ("kmcg_acf_set_properties", _i32, [_vp, _i32, _i32, _vp, _vp]),
+    ("kmcg_acf_init", _i32, [_vp, _i32, _i32, _i32, _vp]),
+    ("kmcg_run_acf", _i32, [_vp, _vp, _i32, _i64, _vp, _i32]),
+    ("kmcg_acf_zero", _i32, [_vp, _i32, _i32]),
+    ("kmcg_acf_get", _i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("kmcg_get_occupation", _i32, [_vp, _i32, _i32, _vp]),
     ("kmcg_get_counters", _i32, [_vp, _i32, _i32, _vp, _vp]),
     ("kmcg_sample", _i32, [_vp, _vp, _i32, _vp]),

@@ -1,4 +1,5 @@
-"""Mean squared displacement of a traced species (kmos/run/acf.py, MSD part).
+"""Particle tracking of a traced species (kmos/run/acf.py): mean squared
+displacements and the autocorrelation of a site property.
 
 kmos records, per kMC step, which particle of a traced species a diffusion
 process moves and by how much (do_kmc_steps_displacement,
@@ -14,6 +15,16 @@ and the engine applies it inside the step kernel (kmcg_run_displacement):
     acf.allocate_trajectory(model, 1000)
     acf.do_kmc_steps_displacement(model, 1000, traj_on=True)
     acf.calc_msd(model), acf.get_displacement(model), acf.get_trajectory(model)
+
+and the autocorrelation function (do_kmc_steps_acf, same file 1-98): every
+step adds, to the time bins since the last wrap, the step's duration times
+the mean over tracked particles of product_property(property at the
+particle's first site, property at its site now):
+
+    acf.allocate_acf(model, 2, t_bin=1e-3, t_f=1e-1)
+    acf.set_property_acf(model, "default_a_1", 1); acf.set_types_acf(model, 1.0)
+    acf.calc_product_property(model); acf.initialize_acf(model, "ion")
+    acf.do_kmc_steps_acf(model, 100000); acf.get_acf(model, normalization=True)
 
 ``model`` is a kmos_amd.run.KMC_Model (replica 0 of its batch).  Only the
 local_smart backend's routine is restated.  A process is a *hop* when, in
@@ -169,3 +180,117 @@ def get_trajectory(model):
         cur[0] = 0
         traj[:, j + 1] = cur
     return traj[1:][None]
+
+
+# ---- autocorrelation (kmos/run/acf.py:159-283) ------------------------------
+
+def allocate_acf(model, nr_of_types, t_bin, t_f, safety_factor=None, extending_factor=None):
+    """allocate_tracing_arr + allocate_config_bin_acf (base_acf.mpy:382-524)."""
+    b = _batch(model)
+    b.acf_alloc(int(nr_of_types), t_bin, t_f, int(safety_factor or 0), int(extending_factor or 0))
+    V = int(b.volumes[0])
+    model._acf = {"types": np.zeros(int(nr_of_types)), "property": np.zeros(V + 1, np.int32),
+                  "product": np.zeros((int(nr_of_types) + 1, int(nr_of_types) + 1))}
+
+
+def set_property_acf(model, layer_site_name, property_type):
+    """Every site of type `layer_site_name` gets the property type index."""
+    a = model._acf
+    n = getattr(model.lattice, layer_site_name.lower())
+    spuck = int(model.lattice.spuck)
+    V = len(a["property"]) - 1
+    for i in range(V):
+        if (i + 1) % spuck + spuck == n or (i + 1) % spuck == n:
+            a["property"][i + 1] = int(property_type)
+    _batch(model).acf_set_properties([a["property"]], None)
+
+
+def get_types_acf(model):
+    return model._acf["types"][None].copy()
+
+
+def set_types_acf(model, site_property):
+    """The value of the first type index not set yet."""
+    types = model._acf["types"]
+    for i in range(len(types)):
+        if types[i] == 0:
+            types[i] = site_property
+            break
+
+
+def calc_product_property(model):
+    """product_property(i, j) = types(i) * types(j) for every pair of types."""
+    a = model._acf
+    T = len(a["types"])
+    for i in range(T):
+        for j in range(T):
+            a["product"][i + 1, j + 1] = a["types"][i] * a["types"][j]
+    _batch(model).acf_set_properties(None, a["product"])
+
+
+def get_product_property(model):
+    return model._acf["product"][1:, 1:][None].copy()
+
+
+def initialize_acf(model, species):
+    """Number the traced particles and set their starting properties."""
+    trace = getattr(model.proclist, species.lower())
+    hops, _ = displacement_table(model.compiled)
+    _batch(model).acf_init(int(trace), hops, 0, 1)
+    model._msd_traj0 = None
+
+
+def do_kmc_steps_acf(model, n, traj_on=False):
+    _batch(model).run_acf(int(n), [0], traj_on)
+    model._kmc._check_status()
+
+
+def _acf_state(model):
+    return _batch(model).acf_state(0)
+
+
+def get_config_bin_acf(model):
+    return _acf_state(model)["config_bin"][None].copy()
+
+
+def get_counter_write_in_bin_acf(model):
+    return _acf_state(model)["counter_write_in_bin"][None].copy()
+
+
+def get_property_acf(model):
+    return model._acf["property"][1:][None].copy()
+
+
+def get_property_o(model):
+    st = _acf_state(model)
+    return st["property_o"][1: st["nr_of_ions"] + 1][None].copy()
+
+
+def get_buffer_acf(model):
+    st = _acf_state(model)
+    return st["buffer_acf"][1: st["nr_of_ions"] + 1][None].copy()
+
+
+def get_id_arr(model):
+    return _msd_state(model)["id_arr"][1:][None].copy()
+
+
+def get_site_arr(model):
+    st = _msd_state(model)
+    return st["site_arr"][1: st["nr_of_ions"] + 1][None].copy()
+
+
+def get_acf(model, normalization=False):
+    """calc_acf per bin (base_acf.mpy:917-935): config_bin over
+    wrap_count * t_bin; normalised to the first bin when asked."""
+    st = _acf_state(model)
+    acf = np.zeros(st["nr_of_bins"])
+    for i in range(st["nr_of_bins"]):
+        acf[i] = st["config_bin"][i] / (st["wrap_count"] * st["t_bin"])
+    if normalization:
+        acf = acf / acf[0]
+    return acf
+
+
+def set_acf_to_zero(model):
+    _batch(model).acf_zero(0, 1)

@@ -136,6 +136,18 @@ struct KParams {
     int *msd_ev;                   // trajectory events (particle, site), [msd_evcap] per replica
     long long *msd_evn;            // events recorded per replica
     long long msd_evcap;           // 0: no trajectory
+    // autocorrelation bins (acf_pa null: off), over the tracking arrays above
+    const int *acf_pa;             // property_acf, [V+1] per replica at msd_off
+    int *acf_po;                   // property_o, [V+1] per replica at msd_off
+    double *acf_buf;               // buffer_acf, [V+1] per replica at msd_off
+    const double *acf_pp;          // product_property [acf_nt1][acf_nt1]
+    int acf_nt1;                   // number of property types + 1
+    double *acf_bin;               // config_bin per replica at acf_boff
+    int *acf_cnt;                  // counter_write_in_bin per replica at acf_boff
+    const long long *acf_boff;
+    double *acf_d;                 // [8] per replica: sum(buffer_acf), t_o, wrap_count, t_bin
+    int *acf_i;                    // [8] per replica: bin_index, nr_of_bins, extended bins,
+                                   // nr_of_ions_before_kmc_step_acf, nr_of_ions
 };
 
 constexpr int MSD_HOP_W = 9;

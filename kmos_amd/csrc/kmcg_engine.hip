@@ -891,7 +891,26 @@ __device__ __forceinline__ void msd_track(const Wave<PPL, BITS> &W, const KParam
         return;
     const int *h = kp.msd_hop + MSD_HOP_W * proc0;
     int p = 0, fin = 0;
-    if (h[0]) {
+    if (h[0] && kp.acf_pa) {
+        // do_kmc_steps_acf: the sites move, the displacement does not
+        // (get_diff_sites_acf, assign_particle_id, update_id_arr)
+        const int s0 = site - 1;
+        const int cell = divm(s0, W.ms);
+        const int yz = divm(cell, W.mx);
+        const int x = cell - yz * W.d.X;
+        const int z = divm(yz, W.my);
+        const int y = yz - z * W.d.Y;
+        const int init = site0(W.d, x, y, z, h[1], h[2], h[3], h[4]) + 1;
+        fin = site0(W.d, x, y, z, h[5], h[6], h[7], h[8]) + 1;
+        const long long o = kp.msd_off[r];
+        int *id = kp.msd_id + o;
+        int *sa = kp.msd_site + o;
+        p = id[init];
+        id[init] = 0;
+        id[fin] = p;
+        sa[p] = fin;
+        sa[0] = 0;
+    } else if (h[0]) {
         const int s0 = site - 1;
         const int cell = divm(s0, W.ms);
         const int yz = divm(cell, W.mx);
@@ -914,6 +933,120 @@ __device__ __forceinline__ void msd_track(const Wave<PPL, BITS> &W, const KParam
         sa[p] = fin;
         sa[0] = 0;
     }
+    if (kp.acf_pa) {
+        kp.acf_i[8 * r + 5] = p;            // the particle update_buffer_acf takes
+        kp.acf_i[8 * r + 6] = fin;
+        return;                              // its trajectory event follows run_proc_nr
+    }
+    if (kp.msd_evcap > 0) {
+        const long long k = kp.msd_evn[r];
+        if (k < kp.msd_evcap) {
+            int *ev = kp.msd_ev + 2 * ((long long)r * kp.msd_evcap + k);
+            ev[0] = p;
+            ev[1] = fin;
+            kp.msd_evn[r] = k + 1;
+        }
+    }
+}
+
+// update_config_bin (base_acf.mpy:1098-1172): the step's time, weighted by
+// sum(buffer_acf) over nr_of_ions_before_kmc_step_acf, spread over the time
+// bins since t_o.  Lane 0.  A step longer than the extended bins reaches
+// stops at the last bin (the reference writes past its array there).
+__device__ __forceinline__ void acf_bins(const KParams &kp, int r, double t, double dt)
+{
+    double *d = kp.acf_d + 8 * r;
+    int *iv = kp.acf_i + 8 * r;
+    double *cb = kp.acf_bin + kp.acf_boff[r] - 1;     // 1-based like config_bin
+    int *cc = kp.acf_cnt + kp.acf_boff[r] - 1;
+    const double sum = d[0];
+    const double entry = (sum == 0.0) ? 0.0 : sum / (double)iv[3];
+    const double t_o = d[1], t_bin = d[3];
+    const int nb = iv[1], ext = iv[2];
+    int bin = iv[0];
+    if (t - t_o < (double)(bin + 1) * t_bin) {
+        cb[bin + 1] = cb[bin + 1] + entry * dt;
+        cc[bin + 1] += 1;
+    } else {
+        cb[bin + 1] = cb[bin + 1] + entry * ((double)(bin + 1) * t_bin - (t - t_o - dt));
+        cc[bin + 1] += 1;
+        bin += 1;
+        if (bin < nb) {
+            while ((double)(bin + 1) * t_bin < t - t_o && bin + 1 < ext) {
+                cb[bin + 1] = cb[bin + 1] + entry * t_bin;
+                cc[bin + 1] += 1;
+                bin += 1;
+            }
+            cb[bin + 1] = cb[bin + 1] + entry * (t - t_o - (double)bin * t_bin);
+            cc[bin + 1] += 1;
+        }
+    }
+    iv[0] = bin;
+}
+
+// update_property_and_buffer_acf (base_acf.mpy:1075-1096), whole wave: each
+// particle's property now and buffer_acf = product_property(p, p); the sum
+// of buffer_acf is exact in any order (the products are integers).
+template <int PPL, int BITS>
+__device__ __forceinline__ void acf_refresh(const Wave<PPL, BITS> &W, const KParams &kp, int r)
+{
+    const long long o = kp.msd_off[r];
+    const int *sa = kp.msd_site + o;
+    const int *pa = kp.acf_pa + o;
+    int *po = kp.acf_po + o;
+    double *buf = kp.acf_buf + o;
+    const int nt1 = kp.acf_nt1;
+    double part = 0.0;
+    for (int i = W.lane; i <= W.d.V; i += 64) {
+        if (i >= 1) {
+            const int q = pa[sa[i]];
+            po[i] = q;
+            buf[i] = kp.acf_pp[q * nt1 + q];
+        }
+        part += buf[i];
+    }
+    for (int sh = 32; sh >= 1; sh >>= 1)
+        part += __shfl_xor(part, sh);
+    if (W.lane == 0)
+        kp.acf_d[8 * r] = part;
+}
+
+// The ACF steps of do_kmc_steps_acf before run_proc_nr (update_config_bin,
+// then the tracking of msd_track, then update_after_wrap_acf,
+// base_acf.mpy:1174-1201).  Wave-uniform call.
+template <int PPL, int BITS>
+__device__ __forceinline__ void acf_before(const Wave<PPL, BITS> &W, const KParams &kp, int r, double t,
+                                           double dt, int proc0, int site)
+{
+    if (W.lane == 0)
+        acf_bins(kp, r, t, dt);
+    msd_track<PPL, BITS>(W, kp, r, proc0, site);
+    int wrap = 0;
+    if (W.lane == 0) {
+        int *iv = kp.acf_i + 8 * r;
+        if (iv[0] > iv[1] - 1) {
+            double *d = kp.acf_d + 8 * r;
+            d[1] = t;
+            iv[0] = 0;
+            d[2] = d[2] + 1.0;
+            iv[3] = iv[4];
+            wrap = 1;
+        }
+    }
+    if (__builtin_amdgcn_readfirstlane(wrap))
+        acf_refresh<PPL, BITS>(W, kp, r);
+}
+
+// update_buffer_acf (base_acf.mpy:1053-1073) after run_proc_nr, then the
+// step's trajectory event.  Lane 0.
+__device__ __forceinline__ void acf_after(const KParams &kp, int r)
+{
+    const int p = kp.acf_i[8 * r + 5], fin = kp.acf_i[8 * r + 6];
+    const long long o = kp.msd_off[r];
+    double *buf = kp.acf_buf + o;
+    const double nv = kp.acf_pp[kp.acf_po[o + p] * kp.acf_nt1 + kp.acf_pa[o + kp.msd_site[o + p]]];
+    kp.acf_d[8 * r] = kp.acf_d[8 * r] + (nv - buf[p]);
+    buf[p] = nv;
     if (kp.msd_evcap > 0) {
         const long long k = kp.msd_evn[r];
         if (k < kp.msd_evcap) {
@@ -1039,13 +1172,19 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
         uj += 1;
         const int site = readlane_i(sv, pl);
         done += 1;
-        if (MSD)
-            msd_track<PPL, BITS>(W, kp, r, pick, site);
+        if (MSD) {
+            if (kp.acf_pa)
+                acf_before<PPL, BITS>(W, kp, r, t, dt, pick, site);
+            else
+                msd_track<PPL, BITS>(W, kp, r, pick, site);
+        }
         int nr_before[PPL];
 #pragma unroll
         for (int k = 0; k < PPL; ++k)
             nr_before[k] = W.nr[k];
         st = run_proc<PPL, BITS>(W, pre, site, kp.merged);
+        if (MSD && kp.acf_pa && lane == 0)
+            acf_after(kp, r);
         if (st)
             break;
 #pragma unroll
@@ -1455,6 +1594,16 @@ struct kmcg_batch {
     int *msd_ev = nullptr;
     long long *msd_evn = nullptr;
     long long msd_evcap = 0;
+    // autocorrelation bins (kmcg_acf_*): allocated by kmcg_acf_alloc
+    int *acf_pa = nullptr, *acf_po = nullptr;
+    double *acf_buf = nullptr, *acf_pp = nullptr;
+    int acf_nt1 = 0;
+    double *acf_bin = nullptr;
+    int *acf_cnt = nullptr;
+    long long *acf_boff = nullptr;
+    double *acf_d = nullptr;
+    int *acf_i = nullptr;
+    int acf_ext = 0;
 };
 
 namespace {
@@ -2235,7 +2384,8 @@ void kmcg_batch_free(kmcg_batch *b)
                     b->avail_pool, b->nr, b->rates, b->integ, b->pstat, b->ktime, b->kdt, b->kstep,
                     b->rngctr, b->status, b->executed, b->budget, b->rep_ids, b->out, b->occ,
                     b->samples, b->msd_hop, b->msd_d, b->msd_id, b->msd_site, b->msd_disp,
-                    b->msd_off_d, b->msd_ev, b->msd_evn};
+                    b->msd_off_d, b->msd_ev, b->msd_evn, b->acf_pa, b->acf_po, b->acf_buf,
+                    b->acf_pp, b->acf_bin, b->acf_cnt, b->acf_boff, b->acf_d, b->acf_i};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -2481,7 +2631,7 @@ int kmcg_set_config(kmcg_batch *b, int32_t r0, int32_t n, const int8_t *config)
     return run_misc(b, ids, MODE_ADJUST, 0, 0, 0, 0);
 }
 
-// track: 0 no displacement tracking, 1 tracking, 2 tracking and trajectory
+// track: bit 0 displacement tracking, bit 1 trajectory, bit 2 ACF bins
 static int run_impl(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps, int64_t *budget,
                     int sample_every, int max_records, double *records, int track = 0)
 {
@@ -2503,6 +2653,18 @@ static int run_impl(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t s
     const Shape s = launch_shape(b, replicas, n, kp, b->max_wpg);
     kp.nsteps = steps;
     kp.budget = budget ? b->budget : nullptr;
+    if (track & 4) {
+        kp.acf_pa = b->acf_pa;
+        kp.acf_po = b->acf_po;
+        kp.acf_buf = b->acf_buf;
+        kp.acf_pp = b->acf_pp;
+        kp.acf_nt1 = b->acf_nt1;
+        kp.acf_bin = b->acf_bin;
+        kp.acf_cnt = b->acf_cnt;
+        kp.acf_boff = b->acf_boff;
+        kp.acf_d = b->acf_d;
+        kp.acf_i = b->acf_i;
+    }
     if (track) {
         kp.msd_hop = b->msd_hop;
         kp.msd_d = b->msd_d;
@@ -2512,7 +2674,7 @@ static int run_impl(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t s
         kp.msd_off = b->msd_off_d;
         kp.msd_ev = b->msd_ev;
         kp.msd_evn = b->msd_evn;
-        kp.msd_evcap = track == 2 ? b->msd_evcap : 0;
+        kp.msd_evcap = (track & 2) ? b->msd_evcap : 0;
     }
     const int width = 4 + b->m->S * b->m->spuck + 2 * b->m->P;
     if (records) {
@@ -2831,15 +2993,13 @@ int kmcg_get_avail(kmcg_batch *b, int32_t r, int32_t *sites, int32_t *back)
 // displacement tracking (base_acf.mpy:217-361, 825-891)
 // ---------------------------------------------------------------------------
 
-int kmcg_msd_init(kmcg_batch *b, int32_t r0, int32_t n, int32_t trace_species, const int32_t *hops,
-                  const double *disp)
+namespace {
+// the hop table's sites must stay within one periodic image of every
+// replica's lattice (site0 wraps once)
+int check_hops(kmcg_batch *b, const int32_t *hops, const double *disp)
 {
-    if (!b || r0 < 0 || n < 1 || r0 + n > b->n || !hops || !disp)
-        return fail(KMCG_EINVAL, "bad msd_init arguments");
     const kmcg_model *m = b->m;
     const int P = m->P;
-    if (trace_species < 0 || trace_species >= m->S)
-        return fail(KMCG_EINVAL, "msd_init: unknown trace species");
     // every hop's two sites must stay within one periodic image of any
     // replica's lattice (site0 wraps once)
     int minX = 1 << 30, minY = 1 << 30, minZ = 1 << 30;
@@ -2861,13 +3021,17 @@ int kmcg_msd_init(kmcg_batch *b, int32_t r0, int32_t n, int32_t trace_species, c
                 (m->dim < 3 && q[2] != 0) || q[3] < 1 || q[3] > m->spuck)
                 return fail(KMCG_EINVAL, "msd_init: hop site outside the lattice's reach");
         }
-        for (int c = 0; c < 3; ++c)
+        for (int c = 0; disp && c < 3; ++c)
             if (!std::isfinite(disp[3 * p + c]))
                 return fail(KMCG_EINVAL, "msd_init: displacements must be finite");
     }
-    int rc = settle(b);
-    if (rc)
-        return rc;
+    return KMCG_OK;
+}
+
+// the tracking pools of every replica, zero-filled (once per batch)
+int msd_alloc(kmcg_batch *b)
+{
+    const int P = b->m->P;
     if (!b->msd_hop) {
         long long tot = 0;
         b->msd_off.assign(b->n, 0);
@@ -2890,10 +3054,17 @@ int kmcg_msd_init(kmcg_batch *b, int32_t r0, int32_t n, int32_t trace_species, c
         HIPCHK(hipMemset(b->msd_id, 0, sizeof(int) * tot));
         HIPCHK(hipMemset(b->msd_site, 0, sizeof(int) * tot));
         HIPCHK(hipMemset(b->msd_disp, 0, sizeof(double) * 3 * tot));
+        HIPCHK(hipMemset(b->msd_d, 0, sizeof(double) * 3 * P));
     }
-    HIPCHK(hipMemcpy(b->msd_hop, hops, sizeof(int) * MSD_HOP_W * P, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(b->msd_d, disp, sizeof(double) * 3 * P, hipMemcpyHostToDevice));
-    // initialize_mean_squared_displacement: particles numbered 1.. in site order
+    return KMCG_OK;
+}
+
+// particles numbered 1.. over the sites holding trace_species, in site
+// order (calc_nr_of_ions + the numbering of initialize_acf /
+// initialize_mean_squared_displacement); displacements to zero when asked
+int msd_number(kmcg_batch *b, int r0, int n, int trace_species, bool zero_disp)
+{
+    int rc;
     for (int r = r0; r < r0 + n; ++r) {
         const int V = b->V[r];
         std::vector<int8_t> lat(V);
@@ -2913,11 +3084,27 @@ int kmcg_msd_init(kmcg_batch *b, int32_t r0, int32_t n, int32_t trace_species, c
         const long long none = 0;
         if ((rc = copy_in(b, b->msd_id + o, id.data(), id.size())) ||
             (rc = copy_in(b, b->msd_site + o, sa.data(), sa.size())) ||
-            (rc = copy_in(b, b->msd_disp + 3 * o, zero.data(), zero.size())) ||
+            (zero_disp && (rc = copy_in(b, b->msd_disp + 3 * o, zero.data(), zero.size()))) ||
             (rc = copy_in(b, b->msd_evn + r, &none, 1)))
             return rc;
     }
     return KMCG_OK;
+}
+} // namespace
+
+int kmcg_msd_init(kmcg_batch *b, int32_t r0, int32_t n, int32_t trace_species, const int32_t *hops,
+                  const double *disp)
+{
+    if (!b || r0 < 0 || n < 1 || r0 + n > b->n || !hops || !disp)
+        return fail(KMCG_EINVAL, "bad msd_init arguments");
+    if (trace_species < 0 || trace_species >= b->m->S)
+        return fail(KMCG_EINVAL, "msd_init: unknown trace species");
+    int rc = check_hops(b, hops, disp);
+    if (rc || (rc = settle(b)) || (rc = msd_alloc(b)))
+        return rc;
+    HIPCHK(hipMemcpy(b->msd_hop, hops, sizeof(int) * MSD_HOP_W * b->m->P, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->msd_d, disp, sizeof(double) * 3 * b->m->P, hipMemcpyHostToDevice));
+    return msd_number(b, r0, n, trace_species, true);
 }
 
 int kmcg_msd_trajectory(kmcg_batch *b, int64_t nr_of_steps)
@@ -2988,7 +3175,219 @@ int kmcg_run_displacement(kmcg_batch *b, const int32_t *replicas, int32_t n, int
 {
     if (!b || !b->msd_hop)
         return fail(KMCG_EINVAL, "run_displacement before msd_init");
-    return run_impl(b, replicas, n, steps, budget, 0, 0, nullptr, traj_on ? 2 : 1);
+    return run_impl(b, replicas, n, steps, budget, 0, 0, nullptr, traj_on ? 3 : 1);
+}
+
+// ---------------------------------------------------------------------------
+// autocorrelation bins (base_acf.mpy:382-552, 1053-1201)
+// ---------------------------------------------------------------------------
+
+namespace {
+// update_property_and_buffer_acf and sum(buffer_acf) of replica r, on the host
+int acf_refresh_host(kmcg_batch *b, int r)
+{
+    const long long o = b->msd_off[r], V1 = b->V[r] + 1;
+    std::vector<int> sa(V1), pa(V1), po(V1);
+    std::vector<double> buf(V1), pp((size_t)b->acf_nt1 * b->acf_nt1);
+    int rc;
+    if ((rc = copy_out(b, sa.data(), (const int *)b->msd_site + o, (size_t)V1)) ||
+        (rc = copy_out(b, pa.data(), (const int *)b->acf_pa + o, (size_t)V1)) ||
+        (rc = copy_out(b, po.data(), (const int *)b->acf_po + o, (size_t)V1)) ||
+        (rc = copy_out(b, buf.data(), (const double *)b->acf_buf + o, (size_t)V1)) ||
+        (rc = copy_out(b, pp.data(), (const double *)b->acf_pp, pp.size())))
+        return rc;
+    double sum = buf[0];
+    for (long long i = 1; i < V1; ++i) {
+        po[i] = pa[sa[i]];
+        buf[i] = pp[(size_t)po[i] * b->acf_nt1 + po[i]];
+        sum = sum + buf[i];
+    }
+    if ((rc = copy_in(b, b->acf_po + o, po.data(), (size_t)V1)) ||
+        (rc = copy_in(b, b->acf_buf + o, buf.data(), (size_t)V1)) ||
+        (rc = copy_in(b, b->acf_d + 8 * (long long)r, &sum, 1)))
+        return rc;
+    return KMCG_OK;
+}
+} // namespace
+
+int kmcg_acf_alloc(kmcg_batch *b, int32_t nr_of_types, double t_bin, double t_f, int32_t safety_factor,
+                   int32_t extending_factor)
+{
+    if (!b || nr_of_types < 1 || nr_of_types > 4096 || !(t_bin > 0.0) || !(t_f > 0.0) ||
+        !std::isfinite(t_bin) || !std::isfinite(t_f) || safety_factor < 0 || extending_factor < 0)
+        return fail(KMCG_EINVAL, "bad acf_alloc arguments");
+    const int safety = safety_factor == 0 ? 2 : safety_factor;
+    const int extend = extending_factor == 0 ? 2 : extending_factor;
+    const double tf = t_f * safety;
+    const double nbd = tf / t_bin + 1;
+    if (!(nbd < 1e7))
+        return fail(KMCG_EINVAL, "acf_alloc: too many bins");
+    const int nb = (int)nbd;
+    const long long ext = (long long)nb * extend;
+    if (ext * b->n > (1ll << 28))
+        return fail(KMCG_EINVAL, "acf_alloc: too many bins for this batch");
+    int rc = settle(b);
+    if (rc || (rc = msd_alloc(b)))
+        return rc;
+    void *old[] = {b->acf_pa, b->acf_po, b->acf_buf, b->acf_pp, b->acf_bin, b->acf_cnt, b->acf_boff, b->acf_d,
+                   b->acf_i};
+    for (void *p : old)
+        if (p)
+            HIPCHK(hipFree(p));
+    long long tot = 0;
+    for (int r = 0; r < b->n; ++r)
+        tot += b->V[r] + 1;
+    const int nt1 = nr_of_types + 1;
+    if (hipMalloc((void **)&b->acf_pa, sizeof(int) * tot) != hipSuccess ||
+        hipMalloc((void **)&b->acf_po, sizeof(int) * tot) != hipSuccess ||
+        hipMalloc((void **)&b->acf_buf, sizeof(double) * tot) != hipSuccess ||
+        hipMalloc((void **)&b->acf_pp, sizeof(double) * nt1 * nt1) != hipSuccess ||
+        hipMalloc((void **)&b->acf_bin, sizeof(double) * ext * b->n) != hipSuccess ||
+        hipMalloc((void **)&b->acf_cnt, sizeof(int) * ext * b->n) != hipSuccess ||
+        hipMalloc((void **)&b->acf_boff, sizeof(long long) * b->n) != hipSuccess ||
+        hipMalloc((void **)&b->acf_d, sizeof(double) * 8 * b->n) != hipSuccess ||
+        hipMalloc((void **)&b->acf_i, sizeof(int) * 8 * b->n) != hipSuccess)
+        return fail(KMCG_ENOMEM, "acf buffers");
+    b->acf_nt1 = nt1;
+    b->acf_ext = (int)ext;
+    HIPCHK(hipMemset(b->acf_pa, 0, sizeof(int) * tot));
+    HIPCHK(hipMemset(b->acf_po, 0, sizeof(int) * tot));
+    HIPCHK(hipMemset(b->acf_buf, 0, sizeof(double) * tot));
+    HIPCHK(hipMemset(b->acf_pp, 0, sizeof(double) * nt1 * nt1));
+    HIPCHK(hipMemset(b->acf_bin, 0, sizeof(double) * ext * b->n));
+    HIPCHK(hipMemset(b->acf_cnt, 0, sizeof(int) * ext * b->n));
+    std::vector<long long> boff(b->n);
+    std::vector<double> d(8 * (size_t)b->n, 0.0), kt(b->n);
+    std::vector<int> iv(8 * (size_t)b->n, 0);
+    HIPCHK(hipMemcpy(kt.data(), b->ktime, sizeof(double) * b->n, hipMemcpyDeviceToHost));
+    for (int r = 0; r < b->n; ++r) {
+        boff[r] = (long long)r * ext;
+        d[8 * r + 1] = kt[r];                 // t_o
+        d[8 * r + 3] = t_bin;
+        iv[8 * r + 1] = nb;
+        iv[8 * r + 2] = (int)ext;
+        iv[8 * r + 3] = b->msd_nions[r];
+        iv[8 * r + 4] = b->msd_nions[r];
+    }
+    HIPCHK(hipMemcpy(b->acf_boff, boff.data(), sizeof(long long) * b->n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->acf_d, d.data(), sizeof(double) * d.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->acf_i, iv.data(), sizeof(int) * iv.size(), hipMemcpyHostToDevice));
+    return KMCG_OK;
+}
+
+int kmcg_acf_set_properties(kmcg_batch *b, int32_t r0, int32_t n, const int32_t *property_acf,
+                            const double *product_property)
+{
+    if (!b || !b->acf_pa)
+        return fail(KMCG_EINVAL, "acf_set_properties before acf_alloc");
+    if (r0 < 0 || n < 0 || r0 + n > b->n || (n > 0 && !property_acf))
+        return fail(KMCG_EINVAL, "bad replica range");
+    const int nt1 = b->acf_nt1;
+    if (product_property) {
+        // sum(buffer_acf) is kept as a running sum: exact only for integers
+        for (int i = 0; i < nt1 * nt1; ++i) {
+            const double v = product_property[i];
+            if (!(std::fabs(v) <= 1048576.0) || v != std::floor(v))
+                return fail(KMCG_EUNSUPPORTED, "acf: product properties must be integers of magnitude <= 2^20");
+        }
+        int rc = copy_in(b, b->acf_pp, product_property, (size_t)nt1 * nt1);
+        if (rc)
+            return rc;
+    }
+    long long off = 0;
+    for (int r = r0; r < r0 + n; ++r) {
+        const long long V1 = b->V[r] + 1;
+        for (long long i = 0; i < V1; ++i)
+            if (property_acf[off + i] < 0 || property_acf[off + i] >= nt1)
+                return fail(KMCG_EINVAL, "acf: property type out of range");
+        int rc = copy_in(b, b->acf_pa + b->msd_off[r], property_acf + off, (size_t)V1);
+        if (rc)
+            return rc;
+        off += V1;
+    }
+    return KMCG_OK;
+}
+
+int kmcg_acf_init(kmcg_batch *b, int32_t r0, int32_t n, int32_t trace_species, const int32_t *hops)
+{
+    if (!b || !b->acf_pa)
+        return fail(KMCG_EINVAL, "acf_init before acf_alloc");
+    if (r0 < 0 || n < 1 || r0 + n > b->n)
+        return fail(KMCG_EINVAL, "bad replica range");
+    if (!hops || trace_species < 0 || trace_species >= b->m->S)
+        return fail(KMCG_EINVAL, "bad acf_init arguments");
+    // initialize_acf: the particle numbering (displacements untouched), then
+    // update_property_and_buffer_acf
+    int rc = check_hops(b, hops, nullptr);
+    if (rc || (rc = settle(b)))
+        return rc;
+    HIPCHK(hipMemcpy(b->msd_hop, hops, sizeof(int) * MSD_HOP_W * b->m->P, hipMemcpyHostToDevice));
+    if ((rc = msd_number(b, r0, n, trace_species, false)))
+        return rc;
+    for (int r = r0; r < r0 + n; ++r) {
+        const int nions = b->msd_nions[r];
+        int two[2] = {nions, nions};
+        if ((rc = copy_in(b, b->acf_i + 8 * (long long)r + 3, two, 2)) || (rc = acf_refresh_host(b, r)))
+            return rc;
+    }
+    return KMCG_OK;
+}
+
+int kmcg_acf_zero(kmcg_batch *b, int32_t r0, int32_t n)
+{
+    if (!b || !b->acf_pa)
+        return fail(KMCG_EINVAL, "acf_zero before acf_alloc");
+    if (r0 < 0 || n < 1 || r0 + n > b->n)
+        return fail(KMCG_EINVAL, "bad replica range");
+    int rc = settle(b);
+    if (rc)
+        return rc;
+    for (int r = r0; r < r0 + n; ++r) {
+        // set_acf_to_zero: nr_of_ions_before, property/buffer, bins, t_o
+        int iv[8];
+        double d[8], kt;
+        if ((rc = copy_out(b, iv, (const int *)b->acf_i + 8 * (long long)r, 8)) ||
+            (rc = copy_out(b, d, (const double *)b->acf_d + 8 * (long long)r, 8)) ||
+            (rc = copy_out(b, &kt, (const double *)b->ktime + r, 1)))
+            return rc;
+        iv[3] = iv[4];
+        iv[0] = 0;
+        d[1] = kt;
+        d[2] = 0.0;
+        if ((rc = copy_in(b, b->acf_i + 8 * (long long)r, iv, 8)) ||
+            (rc = copy_in(b, b->acf_d + 8 * (long long)r, d, 8)) || (rc = acf_refresh_host(b, r)))
+            return rc;
+        HIPCHK(hipMemset(b->acf_bin + (long long)r * b->acf_ext, 0, sizeof(double) * b->acf_ext));
+        HIPCHK(hipMemset(b->acf_cnt + (long long)r * b->acf_ext, 0, sizeof(int) * b->acf_ext));
+    }
+    return KMCG_OK;
+}
+
+int kmcg_run_acf(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps, int64_t *budget,
+                 int32_t traj_on)
+{
+    if (!b || !b->acf_pa)
+        return fail(KMCG_EINVAL, "run_acf before acf_alloc");
+    return run_impl(b, replicas, n, steps, budget, 0, 0, nullptr, traj_on ? 7 : 5);
+}
+
+int kmcg_acf_get(kmcg_batch *b, int32_t r, double *config_bin, int32_t *counter_write_in_bin, double *d4,
+                 int32_t *i5, double *buffer_acf, int32_t *property_o)
+{
+    int rc = check_replica(b, r);
+    if (rc)
+        return rc;
+    if (!b->acf_pa)
+        return fail(KMCG_EINVAL, "acf_get before acf_alloc");
+    const long long o = b->msd_off[r], V1 = b->V[r] + 1, e = b->acf_ext;
+    if ((config_bin && (rc = copy_out(b, config_bin, (const double *)b->acf_bin + r * e, (size_t)e))) ||
+        (counter_write_in_bin && (rc = copy_out(b, counter_write_in_bin, (const int *)b->acf_cnt + r * e, (size_t)e))) ||
+        (d4 && (rc = copy_out(b, d4, (const double *)b->acf_d + 8 * (long long)r, 4))) ||
+        (i5 && (rc = copy_out(b, i5, (const int *)b->acf_i + 8 * (long long)r, 5))) ||
+        (buffer_acf && (rc = copy_out(b, buffer_acf, (const double *)b->acf_buf + o, (size_t)V1))) ||
+        (property_o && (rc = copy_out(b, property_o, (const int *)b->acf_po + o, (size_t)V1))))
+        return rc;
+    return KMCG_OK;
 }
 
 int kmcg_set_state(kmcg_batch *b, int32_t r, const double *time2, const int64_t *step2,

@@ -301,6 +301,58 @@ class Batch:
         return {"nr_of_ions": int(ni[0]), "site_arr": sa, "id_arr": ia, "displacement": dp,
                 "events": ev[: int(ne[0])]}
 
+    # ---- autocorrelation bins -----------------------------------------------
+    def acf_alloc(self, nr_of_types: int, t_bin: float, t_f: float, safety_factor: int = 0,
+                  extending_factor: int = 0):
+        _lib.check(_lib.load().kmcg_acf_alloc(self.h, int(nr_of_types), float(t_bin), float(t_f),
+                                              int(safety_factor), int(extending_factor)))
+
+    def acf_set_properties(self, property_acf=None, product_property=None, r0: int = 0):
+        """property_acf: list of per-replica arrays [V+1] (from r0), or None;
+        product_property: [(T+1), (T+1)] or None."""
+        pa = None
+        n = 0
+        if property_acf is not None:
+            pa = np.ascontiguousarray(np.concatenate([np.asarray(a, np.int32) for a in property_acf]))
+            n = len(property_acf)
+        pp = None if product_property is None else np.ascontiguousarray(product_property, np.float64)
+        _lib.check(_lib.load().kmcg_acf_set_properties(self.h, r0, n, _ptr(pa), _ptr(pp)))
+
+    def acf_init(self, trace_species: int, hops, r0: int = 0, n: Optional[int] = None):
+        n = self.n - r0 if n is None else n
+        h = np.ascontiguousarray(hops, dtype=np.int32)
+        _lib.check(_lib.load().kmcg_acf_init(self.h, r0, n, int(trace_species), _ptr(h)))
+
+    def run_acf(self, steps: int, replicas=None, traj_on: bool = False):
+        """do_kmc_steps_acf: run() with the ACF bins and particle tracking."""
+        ids = np.arange(self.n, dtype=np.int32) if replicas is None else \
+            np.ascontiguousarray(np.atleast_1d(replicas), dtype=np.int32)
+        self._ids = ids
+        _lib.check(_lib.load().kmcg_run_acf(self.h, _ptr(ids), ids.size, int(steps), None,
+                                            1 if traj_on else 0))
+        self.sync()
+
+    def acf_zero(self, r0: int = 0, n: Optional[int] = None):
+        n = self.n - r0 if n is None else n
+        _lib.check(_lib.load().kmcg_acf_zero(self.h, r0, n))
+
+    def acf_state(self, r: int) -> dict:
+        i5 = np.zeros(5, np.int32)
+        d4 = np.zeros(4, np.float64)
+        L = _lib.load()
+        _lib.check(L.kmcg_acf_get(self.h, r, None, None, _ptr(d4), _ptr(i5), None, None))
+        ext = int(i5[2])
+        V1 = int(self.volumes[r]) + 1
+        cb = np.zeros(ext, np.float64)
+        cc = np.zeros(ext, np.int32)
+        buf = np.zeros(V1, np.float64)
+        po = np.zeros(V1, np.int32)
+        _lib.check(L.kmcg_acf_get(self.h, r, _ptr(cb), _ptr(cc), None, None, _ptr(buf), _ptr(po)))
+        return {"config_bin": cb, "counter_write_in_bin": cc, "sum_buffer": float(d4[0]), "t_o": float(d4[1]),
+                "wrap_count": float(d4[2]), "t_bin": float(d4[3]), "bin_index": int(i5[0]),
+                "nr_of_bins": int(i5[1]), "extended_nr_of_bins": ext, "nr_of_ions_before": int(i5[3]),
+                "nr_of_ions": int(i5[4]), "buffer_acf": buf, "property_o": po}
+
     def occupation(self, r0: int = 0, n: Optional[int] = None) -> np.ndarray:
         n = self.n - r0 if n is None else n
         out = np.zeros((n, self.S, self.spuck), np.float64)

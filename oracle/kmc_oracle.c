@@ -23,6 +23,8 @@
  *   reload_system (state overwrite)   base.mpy:363-514
  *   displacement tracking (MSD)       base_acf.mpy:217-361,825-891,
  *                                     proclist_generic_subroutines_acf.mpy:143-219
+ *   autocorrelation bins (ACF)        base_acf.mpy:382-552,1053-1201,
+ *                                     proclist_generic_subroutines_acf.mpy:1-98
  *   initialize_state                  proclist_generic_subroutines.mpy:236-304
  *   put/take/touchup routines         kmos/io/__init__.py:2219-2443 (as data)
  *   lat_int run_proc/nli/touchup_cell kmos/io/__init__.py:1732-2057 (as data)
@@ -117,6 +119,13 @@ typedef struct {
     int msd_nions;
     int32_t *msd_ev;      /* trajectory events (particle, site) */
     int64_t msd_evcap, msd_evn;
+    /* autocorrelation bins (base_acf.mpy module variables) */
+    int32_t *acf_pa, *acf_po;    /* property_acf(0:V), property_o(0:V) */
+    double *acf_buf, *acf_pp;    /* buffer_acf(0:V), product_property(0:T,0:T) */
+    int acf_nt1, acf_nbins, acf_ext, acf_bin, acf_nr_before;
+    double *acf_cb;              /* config_bin(1:ext) */
+    int32_t *acf_cc;             /* counter_write_in_bin(1:ext) */
+    double acf_tbin, acf_to, acf_wrap;
 } orc_t;
 
 #define HDR(o, i) ((o)->blob[(i)])
@@ -371,6 +380,7 @@ void orc_free(orc_t *o)
     free(o->rates); free(o->accum); free(o->integ); free(o->procstat);
     free(o->msd_hops); free(o->msd_d); free(o->msd_id); free(o->msd_site); free(o->msd_disp);
     free(o->msd_ev);
+    free(o->acf_pa); free(o->acf_po); free(o->acf_buf); free(o->acf_pp); free(o->acf_cb); free(o->acf_cc);
     free(o);
 }
 
@@ -666,6 +676,191 @@ int64_t orc_run_displacement(orc_t *o, int64_t n, int traj_on)
         if (rc) { i += 1; break; }
     }
     return i;
+}
+
+/* ---------------- autocorrelation bins ---------------- */
+
+/* allocate_tracing_arr + allocate_config_bin_acf (base_acf.mpy:382-524);
+ * safety/extending 0 = the default factor 2 */
+void orc_acf_alloc(orc_t *o, int nr_types, double t_bin, double t_f, int safety, int extending)
+{
+    if (safety == 0) safety = 2;
+    if (extending == 0) extending = 2;
+    free(o->acf_pa); free(o->acf_po); free(o->acf_buf); free(o->acf_pp); free(o->acf_cb); free(o->acf_cc);
+    o->acf_nt1 = nr_types + 1;
+    o->acf_pa = calloc(o->V + 1, sizeof(int32_t));
+    o->acf_po = calloc(o->V + 1, sizeof(int32_t));
+    o->acf_buf = calloc(o->V + 1, sizeof(double));
+    o->acf_pp = calloc((size_t)o->acf_nt1 * o->acf_nt1, sizeof(double));
+    if (!o->msd_id) {
+        o->msd_id = calloc(o->V + 1, sizeof(int32_t));
+        o->msd_site = calloc(o->V + 1, sizeof(int32_t));
+        o->msd_disp = calloc(3 * (size_t)(o->V + 1), sizeof(double));
+    }
+    double tf = t_f * safety;
+    o->acf_tbin = t_bin;
+    o->acf_nbins = (int)(tf / t_bin + 1);
+    o->acf_ext = o->acf_nbins * extending;
+    o->acf_cb = calloc(o->acf_ext + 1, sizeof(double));     /* 1-based */
+    o->acf_cc = calloc(o->acf_ext + 1, sizeof(int32_t));
+    o->acf_bin = 0;
+    o->acf_wrap = 0;
+    o->acf_to = o->kmc_time;
+}
+
+/* set_property_acf for every site (pa[0..V]) and, if given, product_property */
+void orc_acf_set(orc_t *o, const int32_t *pa, const double *pp)
+{
+    if (pa) memcpy(o->acf_pa, pa, sizeof(int32_t) * (o->V + 1));
+    if (pp) memcpy(o->acf_pp, pp, sizeof(double) * o->acf_nt1 * o->acf_nt1);
+}
+
+/* update_property_and_buffer_acf (base_acf.mpy:1075-1096) */
+static void acf_update_property_and_buffer(orc_t *o)
+{
+    for (int i = 1; i <= o->V; ++i) {
+        o->acf_po[i] = o->acf_pa[o->msd_site[i]];
+        o->acf_buf[i] = o->acf_pp[o->acf_po[i] * o->acf_nt1 + o->acf_po[i]];
+    }
+}
+
+/* initialize_acf (base_acf.mpy:274-312) with the hop table of orc_msd_init */
+void orc_acf_init(orc_t *o, int trace_species, const int32_t *hops)
+{
+    if (!o->msd_hops) {
+        o->msd_hops = malloc(sizeof(int32_t) * 9 * o->P);
+        o->msd_d = calloc(3 * (size_t)o->P, sizeof(double));
+    }
+    memcpy(o->msd_hops, hops, sizeof(int32_t) * 9 * o->P);
+    int i = 1;
+    for (int k = 1; k <= o->V; ++k) {
+        if (o->lat[k] == trace_species) {
+            o->msd_id[k] = i;
+            o->msd_site[i] = k;
+            i += 1;
+        } else {
+            o->msd_id[k] = 0;
+        }
+    }
+    o->msd_nions = i - 1;
+    o->acf_nr_before = o->msd_nions;
+    o->msd_evn = 0;
+    acf_update_property_and_buffer(o);
+}
+
+/* set_acf_to_zero (base_acf.mpy:526-551) */
+void orc_acf_zero(orc_t *o)
+{
+    o->acf_nr_before = o->msd_nions;
+    acf_update_property_and_buffer(o);
+    memset(o->acf_cb, 0, sizeof(double) * (o->acf_ext + 1));
+    memset(o->acf_cc, 0, sizeof(int32_t) * (o->acf_ext + 1));
+    o->acf_bin = 0;
+    o->acf_wrap = 0;
+    o->acf_to = o->kmc_time;
+}
+
+/* update_config_bin (base_acf.mpy:1098-1172); the inner loop stops at the
+ * last extended bin, where the reference would write past its array */
+static void acf_update_config_bin(orc_t *o)
+{
+    double entry_bin = 0.0;
+    for (int i = 0; i <= o->V; ++i)
+        entry_bin = entry_bin + o->acf_buf[i];
+    if (entry_bin == 0.0)
+        entry_bin = 0.0;
+    else
+        entry_bin = entry_bin / (double)o->acf_nr_before;
+    const double t = o->kmc_time, dt = o->kmc_time_step, t_bin = o->acf_tbin, t_o = o->acf_to;
+    int b = o->acf_bin;
+    if (t - t_o < (double)(b + 1) * t_bin) {
+        o->acf_cb[b + 1] = o->acf_cb[b + 1] + entry_bin * dt;
+        o->acf_cc[b + 1] += 1;
+    } else {
+        o->acf_cb[b + 1] = o->acf_cb[b + 1] + entry_bin * ((double)(b + 1) * t_bin - (t - t_o - dt));
+        o->acf_cc[b + 1] += 1;
+        b += 1;
+        if (b < o->acf_nbins) {
+            while ((double)(b + 1) * t_bin < t - t_o && b + 1 < o->acf_ext) {
+                o->acf_cb[b + 1] = o->acf_cb[b + 1] + entry_bin * t_bin;
+                o->acf_cc[b + 1] += 1;
+                b += 1;
+            }
+            o->acf_cb[b + 1] = o->acf_cb[b + 1] + entry_bin * (t - t_o - (double)b * t_bin);
+            o->acf_cc[b + 1] += 1;
+        }
+    }
+    o->acf_bin = b;
+}
+
+/* do_kmc_steps_acf (proclist_generic_subroutines_acf.mpy:1-98) */
+int64_t orc_run_acf(orc_t *o, int64_t n, int traj_on)
+{
+    int64_t i;
+    if (o->status != ORC_OK || !o->msd_hops || !o->acf_cb) return 0;
+    for (i = 0; i < n; ++i) {
+        double u[3];
+        update_accum_rate(o);
+        if (!(o->accum[o->P - 1] > 0.0)) {
+            o->status = ORC_ABSORBING;
+            break;
+        }
+        orc_step_uniforms(o->seed, o->stream, o->rng_ctr, u);
+        o->rng_ctr += 1;
+        o->kmc_time_step = -log(u[0]) / o->accum[o->P - 1];
+        o->kmc_time = o->kmc_time + o->kmc_time_step;
+        o->kmc_step += 1;
+        update_integ_rate(o);
+        int proc, site;
+        determine_procsite(o, u[1], u[2], &proc, &site);
+        acf_update_config_bin(o);
+        const int32_t *h = o->msd_hops + 9 * (proc - 1);
+        int particle = 0, fin = 0;
+        if (h[0]) {                                   /* get_diff_sites_acf */
+            int c[4];
+            nr2lattice(o, site, c);
+            int init = lattice2nr(o, c[0] + h[1], c[1] + h[2], c[2] + h[3], h[4]);
+            fin = lattice2nr(o, c[0] + h[5], c[1] + h[6], c[2] + h[7], h[8]);
+            particle = o->msd_id[init];               /* assign_particle_id */
+            o->msd_id[init] = 0;                      /* update_id_arr */
+            o->msd_id[fin] = particle;
+            o->msd_site[particle] = fin;
+            o->msd_site[0] = 0;
+        }
+        if (o->acf_bin > o->acf_nbins - 1) {          /* update_after_wrap_acf */
+            o->acf_to = o->kmc_time;
+            o->acf_bin = 0;
+            o->acf_wrap = o->acf_wrap + 1;
+            acf_update_property_and_buffer(o);
+            o->acf_nr_before = o->msd_nions;
+        }
+        int rc = orc_run_proc_nr(o, proc, site);
+        /* update_buffer_acf */
+        o->acf_buf[particle] = o->acf_pp[o->acf_po[particle] * o->acf_nt1 + o->acf_pa[o->msd_site[particle]]];
+        if (traj_on && o->msd_evn < o->msd_evcap) {   /* update_trajectory */
+            o->msd_ev[2 * o->msd_evn] = particle;
+            o->msd_ev[2 * o->msd_evn + 1] = fin;
+            o->msd_evn += 1;
+        }
+        if (rc) { i += 1; break; }
+    }
+    return i;
+}
+
+/* config_bin and counters (ext each), d4 = {sum(buffer_acf), t_o,
+ * wrap_count, t_bin}, i5 = {bin_index, nr_of_bins, ext, nr_before, nions},
+ * buffer_acf and property_o (V+1 each) */
+void orc_acf_get(const orc_t *o, double *cb, int32_t *cc, double *d4, int32_t *i5, double *buf, int32_t *po)
+{
+    memcpy(cb, o->acf_cb + 1, sizeof(double) * o->acf_ext);
+    memcpy(cc, o->acf_cc + 1, sizeof(int32_t) * o->acf_ext);
+    double sum = 0.0;
+    for (int i = 0; i <= o->V; ++i)
+        sum = sum + o->acf_buf[i];
+    d4[0] = sum; d4[1] = o->acf_to; d4[2] = o->acf_wrap; d4[3] = o->acf_tbin;
+    i5[0] = o->acf_bin; i5[1] = o->acf_nbins; i5[2] = o->acf_ext; i5[3] = o->acf_nr_before; i5[4] = o->msd_nions;
+    memcpy(buf, o->acf_buf, sizeof(double) * (o->V + 1));
+    memcpy(po, o->acf_po, sizeof(int32_t) * (o->V + 1));
 }
 
 /* get_next_kmc_step (proclist_generic_subroutines.mpy:85-110): draws one

@@ -73,6 +73,13 @@ def lib():
         L.orc_msd_get.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.orc_run_displacement.argtypes = [vp, i64, i32]
         L.orc_run_displacement.restype = i64
+        L.orc_acf_alloc.argtypes = [vp, i32, dbl, dbl, i32, i32]
+        L.orc_acf_set.argtypes = [vp, vp, vp]
+        L.orc_acf_init.argtypes = [vp, i32, vp]
+        L.orc_acf_zero.argtypes = [vp]
+        L.orc_run_acf.argtypes = [vp, i64, i32]
+        L.orc_run_acf.restype = i64
+        L.orc_acf_get.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.orc_get_species.argtypes = [vp, i32]
         L.orc_get_species.restype = i32
         L.orc_replace_species.argtypes = [vp, i32, i32, i32]
@@ -257,6 +264,44 @@ class Oracle:
         lib().orc_msd_get(self.h, _ptr(ni), _ptr(sa), _ptr(ia), _ptr(dp), _ptr(ne), _ptr(ev))
         return {"nr_of_ions": int(ni[0]), "site_arr": sa, "id_arr": ia, "displacement": dp,
                 "events": ev[: int(ne[0])]}
+
+    # autocorrelation bins ---------------------------------------------------
+    def acf_alloc(self, nr_types, t_bin, t_f, safety=0, extending=0):
+        lib().orc_acf_alloc(self.h, int(nr_types), float(t_bin), float(t_f), int(safety), int(extending))
+        self._nt1 = int(nr_types) + 1
+        self._ext = int(float(t_f) * (safety or 2) / float(t_bin) + 1) * (extending or 2)
+
+    def acf_set(self, property_acf=None, product_property=None):
+        self._pa = None if property_acf is None else np.ascontiguousarray(property_acf, np.int32)
+        self._pp = None if product_property is None else np.ascontiguousarray(product_property, np.float64)
+        assert self._pa is None or self._pa.size == self.V + 1
+        lib().orc_acf_set(self.h, _ptr(self._pa), _ptr(self._pp))
+
+    def acf_init(self, trace_species, hops):
+        self._hops = np.ascontiguousarray(hops, np.int32)
+        lib().orc_acf_init(self.h, int(trace_species), _ptr(self._hops))
+
+    def acf_zero(self):
+        lib().orc_acf_zero(self.h)
+
+    def run_acf(self, n, traj_on=False):
+        return lib().orc_run_acf(self.h, int(n), 1 if traj_on else 0)
+
+    def acf_state(self):
+        i5 = np.zeros(5, np.int32)
+        d4 = np.zeros(4, np.float64)
+        cb = np.zeros(self._ext, np.float64)
+        cc = np.zeros(self._ext, np.int32)
+        buf = np.zeros(self.V + 1, np.float64)
+        po = np.zeros(self.V + 1, np.int32)
+        lib().orc_acf_get(self.h, _ptr(cb), _ptr(cc), _ptr(d4), _ptr(i5), _ptr(buf), _ptr(po))
+        ext = int(i5[2])
+        assert ext == self._ext
+        return {"config_bin": cb, "counter_write_in_bin": cc,
+                "sum_buffer": float(d4[0]), "t_o": float(d4[1]), "wrap_count": float(d4[2]),
+                "t_bin": float(d4[3]), "bin_index": int(i5[0]), "nr_of_bins": int(i5[1]),
+                "extended_nr_of_bins": ext, "nr_of_ions_before": int(i5[3]), "nr_of_ions": int(i5[4]),
+                "buffer_acf": buf, "property_o": po}
 
     def set_state(self, st):
         """reload_system: overwrite the state (rng_counter defaults to kmc_step)."""

@@ -209,3 +209,70 @@ def test_oracle_non_hops_move_no_particle(oracle):
     # adsorbed ions were never numbered: every hop moves particle 0
     assert set(st["events"][:, 0].tolist()) == {0}
     assert not st["site_arr"].any() and not st["id_arr"].any()
+
+
+# ---------------------------------------------------------------------------
+# autocorrelation bins
+# ---------------------------------------------------------------------------
+
+def _acf_oracle(oracle, cm, size, seed, prop_of_site, types, t_bin, t_f, config_seed=3):
+    """An oracle replica of the hop model with property types per site type
+    (prop_of_site[n-1]) and type values `types`."""
+    g = np.random.default_rng(config_seed)
+    V = size[0] * size[1] * cm.spuck
+    ion, empty = cm.species_names.index("ion"), cm.species_names.index("empty")
+    config = np.where(g.random(V) < 0.3, ion, empty).astype(np.int8)
+    o = oracle_replica(oracle, cm, size, seed, 0, np.array([3, 2, 1.5, 1, 0.7]), config)
+    o.acf_alloc(len(types), t_bin, t_f)
+    pa = np.zeros(V + 1, np.int32)
+    pa[1:] = [prop_of_site[s % cm.spuck] for s in range(V)]
+    T = len(types)
+    pp = np.zeros((T + 1, T + 1))
+    pp[1:, 1:] = np.outer(types, types)
+    o.acf_set(pa, pp)
+    hops, _ = displacement_table(cm)
+    o.acf_init(ion, hops)
+    return o, pa, pp, hops
+
+
+def test_oracle_acf_spreads_each_step_over_the_bins(oracle):
+    """One property for every site: each tracked particle contributes 1, so
+    the bins hold the elapsed time, split at the bin boundaries."""
+    cm = compile_model(hop_project(), "local_smart")
+    o, _, _, _ = _acf_oracle(oracle, cm, (7, 5), 4, [1, 1], [1.0], t_bin=0.02, t_f=50.0)
+    st0 = o.acf_state()
+    assert st0["nr_of_bins"] == int(50.0 * 2 / 0.02 + 1) and st0["extended_nr_of_bins"] == 2 * st0["nr_of_bins"]
+    assert st0["sum_buffer"] == st0["nr_of_ions"] == st0["nr_of_ions_before"]
+    o.run_acf(2000)
+    st = o.acf_state()
+    elapsed = o.kmc_time - st0["t_o"]
+    assert st["wrap_count"] == 0 and 1.0 < elapsed < 50.0 * 2
+    cb = st["config_bin"]
+    np.testing.assert_allclose(cb.sum(), elapsed, rtol=1e-12)
+    b = st["bin_index"]
+    assert b == int(elapsed // 0.02)
+    np.testing.assert_allclose(cb[:b], 0.02, rtol=1e-9)          # every full bin: its width
+    assert not cb[b + 1:].any()
+    assert st["counter_write_in_bin"][:b + 1].min() >= 1
+
+
+def test_oracle_acf_wraps_and_refreshes(oracle):
+    """Two property types (+1 on site a, -1 on site b): each wrap restarts
+    the origin properties, so right after a wrap every particle's product is
+    +1 again; the entries stay within [-1, 1], so no bin exceeds its width
+    per wrap."""
+    cm = compile_model(hop_project(), "local_smart")
+    o, pa, pp, _ = _acf_oracle(oracle, cm, (8, 6), 5, [1, 2], [1.0, -1.0], t_bin=0.01, t_f=0.05)
+    st0 = o.acf_state()
+    nb = st0["nr_of_bins"]
+    assert nb == int(0.05 * 2 / 0.01 + 1)
+    o.run_acf(4000)
+    st = o.acf_state()
+    assert st["wrap_count"] >= 2
+    n = st["nr_of_ions"]
+    assert abs(st["sum_buffer"]) <= n
+    per = st["config_bin"][:nb] / (st["wrap_count"] * 0.01)
+    assert np.all(np.abs(per) <= 1.0 + 1e-9)
+    assert per[0] > 0.5                                            # lag 0: still mostly +1
+    # a step that overshoots the last bin spills into the extended bins
+    assert np.all(np.abs(st["config_bin"][nb:]) <= st["wrap_count"] * 0.01 + 1e-12)
