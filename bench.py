@@ -18,6 +18,9 @@ bounded sample of the same workload).  Two more legs ride along:
            1:2:1, 2e5 steps) on the GPU, next to the reference's own
            generated Fortran (built by oracle/build_ref.py with the injected
            uniform stream) run one replica per host core
+  pairwise the lateral-interaction workload (512 replicas of 256^2,
+           pairwise lat_int, 5e5 steps) with its roofline and the C oracle
+           on the host cores
   serving  a Poisson stream of requests drawn from the quickstart / ZGB /
            RuO2 workloads through the continuous-batching server: p50/p99
            request latency and aggregate steps/s
@@ -188,14 +191,45 @@ def ruo2_leg(engine, K, no_cpu):
     b, wall, executed, kernel_ms, chunk = run_batched(engine, model, wl, K, 1)
     st = b.scalars_all()[2]
     b.close()
+    bps = algorithmic_bytes_per_step(wl, cm)
+    avg_s = float(np.mean(kernel_ms)) / 1e3
+    achieved = executed / K * bps / avg_s / 1e9
     out = {"workload": wl.name, "replicas": wl.n, "steps_per_replica": wl.steps,
-           "gpu_steps_per_s": executed / wall, "avg_launch_ms": float(np.mean(kernel_ms)),
-           "replica_status_counts": {str(k): int(v) for k, v in zip(*np.unique(st, return_counts=True))}}
+           "gpu_steps_per_s": executed / wall, "avg_launch_ms": avg_s * 1e3,
+           "replica_status_counts": {str(k): int(v) for k, v in zip(*np.unique(st, return_counts=True))},
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(wl.name),
+                        "algorithmic_bytes_per_kmc_step": bps}}
     if not no_cpu:
         ref = reference_cpu_baseline(wl)
         if ref is not None:
             out["cpu_baseline"] = ref
             out["gpu_over_cpu_reference"] = out["gpu_steps_per_s"] / ref["value"]
+    return out
+
+
+def pairwise_leg(engine, K, no_cpu):
+    """The lateral-interaction workload (512 replicas of 256^2, pairwise
+    lat_int, 5e5 steps each) with its own roofline figures and the C
+    oracle on the host cores beside it (no reference build exists for it)."""
+    wl = W.pairwise()
+    cm = compile_model(wl.project, wl.backend)
+    model = engine.Model(cm)
+    b, wall, executed, kernel_ms, chunk = run_batched(engine, model, wl, K, 1)
+    st = b.scalars_all()[2]
+    b.close()
+    bps = algorithmic_bytes_per_step(wl, cm)
+    avg_s = float(np.mean(kernel_ms)) / 1e3
+    achieved = executed / K * bps / avg_s / 1e9
+    out = {"workload": wl.name, "replicas": wl.n, "steps_per_replica": wl.steps,
+           "gpu_steps_per_s": executed / wall, "avg_launch_ms": avg_s * 1e3,
+           "replica_status_counts": {str(k): int(v) for k, v in zip(*np.unique(st, return_counts=True))},
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(wl.name),
+                        "algorithmic_bytes_per_kmc_step": bps}}
+    if not no_cpu:
+        out["cpu_baseline"] = cpu_baseline(wl, cm, 15.0)
+        out["gpu_over_cpu"] = out["gpu_steps_per_s"] / out["cpu_baseline"]["value"]
     return out
 
 
@@ -256,6 +290,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=20000)
     ap.add_argument("--no-ruo2", action="store_true", help="skip the RuO2 leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-pairwise", action="store_true", help="skip the lateral-interaction leg")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="request arrivals per second")
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -349,6 +384,9 @@ def main():
     if not a.no_ruo2 and a.workload == "zgb":
         out["ruo2"] = ruo2_leg(engine, K, a.no_cpu)
         _progress("ruo2 leg done")
+    if not a.no_pairwise and a.workload == "zgb":
+        out["pairwise"] = pairwise_leg(engine, K, a.no_cpu)
+        _progress("pairwise leg done")
     if not a.no_serve:
         out["serving"] = serving_leg(a.serve_rate, a.serve_requests)
         _progress("serving leg done")
