@@ -108,6 +108,7 @@ struct KParams {
     long long *kstep;
     unsigned long long *rngctr;
     int *status;
+    const int *lists_ok;           // per replica: lists match the lattice (or null)
     long long *executed;
     long long *budget;             // optional, per launch slot
     int *out;                      // misc kernel outputs

@@ -50,6 +50,7 @@ struct Wave {
     int *win;          // [64][WIN] list-tail window of each lane (replay)
     int *occ;          // [S][spuck] species counts, kept while `track`
     bool track;
+    bool chk;          // lists match the lattice: disable checks may test conditions
     bool local_smart;
     int *avail;        // sites[P][V+1] then back[P][V+1]
     long long V1;
@@ -304,7 +305,8 @@ __device__ __forceinline__ int cond_species(const Wave<PPL, BITS> &W, int cw, in
 // order keeps the reference order within every queue.
 template <int PPL, int BITS, bool IN>
 __device__ __forceinline__ void eval_post(Wave<PPL, BITS> &W, bool valid, int2 op, const int (&cw)[4],
-                                          const int *cd, bool as_add, int nrounds, int lb, int bx, int by, int bz)
+                                          const int *cd, bool as_add, int nrounds, int lb, int bx, int by, int bz,
+                                          int sw, int oldw)
 {
     int proc = 0, code = 0, s = 0, rnd = 0;
     if (valid) {
@@ -317,28 +319,41 @@ __device__ __forceinline__ void eval_post(Wave<PPL, BITS> &W, bool valid, int2 o
         const int dx = sext7(op.y & 0x7F), dy = sext7((op.y >> 7) & 0x7F), dz = sext7((op.y >> 14) & 0x7F);
         if (kind == OP_DEL_IF || kind == OP_ADD_IF) {
             s = site_from<IN>(W.d, lb, bx, by, bz, dx, dy, dz, n);
-            if (kind == OP_DEL_IF) {
-                proc = arg;
-                code = MB_DEL_IF;
-            } else {
-                const int nc = (op.x >> 17) & 0x3F;
+            const bool del = kind == OP_DEL_IF;
+            const int nc = (op.x >> 17) & 0x3F;
+            bool ok = true;
+            // an addition tests its conditions on the lattice after the
+            // segment's write; a disable check (while the lists match the
+            // lattice) tests its process's conditions on the lattice before
+            // it: the process is registered exactly when they hold
+            if (!del || (W.chk && nc > 0)) {
+                int cs[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int w = (c < nc) ? cw[c] : (32 | 32 << 6 | 32 << 12 | 1 << 18);
+                    cs[c] = site_from<IN>(W.d, lb, bx, by, bz, (w & 63) - 32, ((w >> 6) & 63) - 32,
+                                          ((w >> 12) & 63) - 32, (w >> 18) & 63);
+                }
                 int q[4];
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
-                    q[c] = cond_species<PPL, BITS, IN>(W, (c < nc) ? cw[c] : (32 | 32 << 6 | 32 << 12 | 1 << 18),
-                                                       lb, bx, by, bz);
-                bool ok = true;
+                    q[c] = W.lat.get(cs[c]);
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    ok &= (c >= nc) || (q[c] == ((cw[c] >> 24) & 0xFF));
+                for (int c = 0; c < 4; ++c) {
+                    const int sp = (del && cs[c] == sw) ? oldw : q[c];
+                    ok &= (c >= nc) || (sp == ((cw[c] >> 24) & 0xFF));
+                }
                 for (int c = 4; c < nc; ++c) {
                     const int w = cd[c];
-                    ok &= cond_species<PPL, BITS, IN>(W, w, lb, bx, by, bz) == ((w >> 24) & 0xFF);
+                    const int sc = site_from<IN>(W.d, lb, bx, by, bz, (w & 63) - 32, ((w >> 6) & 63) - 32,
+                                                 ((w >> 12) & 63) - 32, (w >> 18) & 63);
+                    const int sp = (del && sc == sw) ? oldw : W.lat.get(sc);
+                    ok &= sp == ((w >> 24) & 0xFF);
                 }
-                if (ok) {
-                    proc = arg;
-                    code = MB_ADD;
-                }
+            }
+            if (ok) {
+                proc = arg;
+                code = del ? MB_DEL_IF : MB_ADD;
             }
         } else {
             const int cx = wrap1(bx + dx, W.d.X);
@@ -378,7 +393,8 @@ __device__ __forceinline__ void load_op(const int *prog, int ooff, int cbase, in
     cd = prog;
     if (valid)
         op = *(const int2 *)(prog + ooff + OP_W * j);
-    const int nc = ((op.x & 7) == OP_ADD_IF) ? (op.x >> 17) & 0x3F : 0;
+    const int kind = op.x & 7;
+    const int nc = (kind == OP_ADD_IF || kind == OP_DEL_IF) ? (op.x >> 17) & 0x3F : 0;
     cd = prog + cbase + (int)((unsigned)op.y >> 21);
     // unconditional reads (the program ends with 4 padding words)
 #pragma unroll
@@ -433,6 +449,9 @@ __device__ __forceinline__ int post_segment_t(Wave<PPL, BITS> &W, const SegPre &
     const int *cd = q.cd;
     const int lb = W.d.lin(bx, by, bz);
 
+    // the site of a single write and its species before it (the lattice
+    // the disable checks' conditions are tested on)
+    int sw = -1, oldw = 0;
     if (nw > 0) {
         // the writes of a segment hit distinct sites (checked on the host)
         int s = 0, cur = 0, nsp = 0;
@@ -461,15 +480,19 @@ __device__ __forceinline__ int post_segment_t(Wave<PPL, BITS> &W, const SegPre &
                 wave_sync();
                 return ST_MISMATCH;
             }
+            if (nw == 1) {
+                sw = readlane_i(s, 0);
+                oldw = readlane_i(cur, 0);
+            }
         }
         wave_sync();
     }
 
-    eval_post<PPL, BITS, IN>(W, W.lane < nops, op, cw, cd, as_add, nrounds, lb, bx, by, bz);
+    eval_post<PPL, BITS, IN>(W, W.lane < nops, op, cw, cd, as_add, nrounds, lb, bx, by, bz, sw, oldw);
     for (int base = 64; base < nops; base += 64) {
         const int j = base + W.lane;
         load_op(prog, ooff, cbase, j, j < nops, op, cw, cd);
-        eval_post<PPL, BITS, IN>(W, j < nops, op, cw, cd, as_add, nrounds, lb, bx, by, bz);
+        eval_post<PPL, BITS, IN>(W, j < nops, op, cw, cd, as_add, nrounds, lb, bx, by, bz, sw, oldw);
     }
     wave_sync();
     return ST_OK;
@@ -507,6 +530,7 @@ __device__ __forceinline__ int post_pair(Wave<PPL, BITS> &W, const SegPre &q1, c
         bad = cur < 0 || !((w1 >> cur) & 1);
     }
     const int sA = readlane_i(s, 0), sB = readlane_i(s, 1), oldB = readlane_i(cur, 1);
+    const int oldA = readlane_i(cur, 0);
     if (__ballot(W.lane < 2 && bad) || sA == sB)
         return -1;
     if (W.lane < 2) {
@@ -536,11 +560,14 @@ __device__ __forceinline__ int post_pair(Wave<PPL, BITS> &W, const SegPre &q1, c
         const int arg = (int)((unsigned)op.x >> 23);
         const int dx = sext7(op.y & 0x7F), dy = sext7((op.y >> 7) & 0x7F), dz = sext7((op.y >> 14) & 0x7F);
         so = site_from<IN>(W.d, lb, ex, ey, ez, dx, dy, dz, n);
-        if (kind == OP_DEL_IF) {
-            proc = arg;
-            code = MB_DEL_IF;
-        } else {
-            const int nc = (op.x >> 17) & 0x3F;
+        const bool del = kind == OP_DEL_IF;
+        const int nc = (op.x >> 17) & 0x3F;
+        bool ok = true;
+        // additions: the first segment's ops see site B before its write;
+        // disable checks (lists matching the lattice): their process's
+        // conditions on the lattice before their own segment's write, so
+        // B is old for both and A old for the first segment's
+        if (!del || (W.chk && nc > 0)) {
             int q[4], cs[4];
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
@@ -551,23 +578,23 @@ __device__ __forceinline__ int post_pair(Wave<PPL, BITS> &W, const SegPre &q1, c
 #pragma unroll
             for (int c = 0; c < 4; ++c)
                 q[c] = W.lat.get(cs[c]);
-            bool ok = true;
+            const bool oldb = first || del, olda = first && del;
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                const int sp = (first && cs[c] == sB) ? oldB : q[c];
+                const int sp = (oldb && cs[c] == sB) ? oldB : (olda && cs[c] == sA) ? oldA : q[c];
                 ok &= (c >= nc) || (sp == ((cw[c] >> 24) & 0xFF));
             }
             for (int c = 4; c < nc; ++c) {
                 const int w = cd[c];
                 const int sc = site_from<IN>(W.d, lb, ex, ey, ez, (w & 63) - 32, ((w >> 6) & 63) - 32,
                                              ((w >> 12) & 63) - 32, (w >> 18) & 63);
-                const int sp = (first && sc == sB) ? oldB : W.lat.get(sc);
+                const int sp = (oldb && sc == sB) ? oldB : (olda && sc == sA) ? oldA : W.lat.get(sc);
                 ok &= sp == ((w >> 24) & 0xFF);
             }
-            if (ok) {
-                proc = arg;
-                code = MB_ADD;
-            }
+        }
+        if (ok) {
+            proc = arg;
+            code = del ? MB_DEL_IF : MB_ADD;
         }
     }
     const int nrt = q1.nrounds + q2.nrounds;
@@ -803,6 +830,7 @@ __device__ __forceinline__ int wave_setup(Wave<PPL, BITS> &W, const KParams &kp,
     W.win = W.cnt + W.mstride;
     W.occ = W.win + 64 * WIN;
     W.track = false;
+    W.chk = false;
     W.local_smart = kp.backend == 0;
     for (int i = W.lane; i < W.mstride; i += 64)
         W.cnt[i] = 0;
@@ -1070,6 +1098,7 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
     const int lane = W.lane, P = W.P;
     // a replica that has stopped takes no step (and still gives its record)
     const int st0 = kp.status[r];
+    W.chk = kp.lists_ok && kp.lists_ok[r] && W.local_smart;
     long long n = kp.nsteps;
     if (kp.budget)
         n = min(n, kp.budget[slot]);
@@ -1583,6 +1612,9 @@ struct kmcg_batch {
     int n_last = 0;
     float last_ms = 0.f;
     bool timed_run_pending = false;   // ev0/ev1 bracket a launch not yet read
+    // 1 per replica whose lists match its lattice (set by kmcg_init and
+    // kmcg_set_config, cleared by kmcg_replace_species and kmcg_set_state)
+    int *lists_ok = nullptr;
     // displacement tracking (kmcg_msd_*): allocated by the first kmcg_msd_init
     int *msd_hop = nullptr;
     double *msd_d = nullptr;
@@ -1709,6 +1741,40 @@ int build_prog(kmcg_model *m)
             }
             for (int n = 0; n < m->spuck; ++n)
                 touchup.push_back(W(W(12) + n));
+            // every process's conditions, relative to its executing site's
+            // cell: the enable of the touchup routine of that site type
+            std::vector<std::vector<int>> pconds(m->P + 1);
+            std::vector<bool> have(m->P + 1, false);
+            for (int n = 0; n < m->spuck; ++n) {
+                const int t = touchup[n];
+                if (t < 0 || t >= (int)segs.size())
+                    return fail(KMCG_EINVAL, "bad touchup routine");
+                for (auto &op : segs[t].ops)
+                    if (op.kind == OP_ADD_IF && op.arg >= 1 && op.arg <= m->P && !have[op.arg]) {
+                        pconds[op.arg] = op.conds;
+                        have[op.arg] = true;
+                    }
+            }
+            // a disable check of a put/take routine carries the conditions of
+            // its process at the checked site: while the lists match the
+            // lattice, the process is registered there exactly when they
+            // hold on the lattice before the routine's write, so the check
+            // needs no read of avail_sites when they fail
+            std::vector<bool> is_touchup(segs.size(), false);
+            for (int t : touchup)
+                is_touchup[t] = true;
+            for (size_t si = 0; si < segs.size(); ++si) {
+                if (is_touchup[si] || segs[si].writes.size() != 6)
+                    continue;
+                for (auto &op : segs[si].ops) {
+                    if (op.kind != OP_DEL_IF || op.arg < 1 || op.arg > m->P || !have[op.arg])
+                        continue;
+                    const std::vector<int> &pc = pconds[op.arg];
+                    for (size_t c = 0; c < pc.size(); c += 5)
+                        op.conds.insert(op.conds.end(),
+                                        {op.dx + pc[c], op.dy + pc[c + 1], op.dz + pc[c + 2], pc[c + 3], pc[c + 4]});
+                }
+            }
         } else {
             const int G = W(13), gtab = W(14);
             for (int g = 0; g < G; ++g) {
@@ -2122,6 +2188,7 @@ KParams make_params(kmcg_batch *b)
     kp.kstep = b->kstep;
     kp.rngctr = b->rngctr;
     kp.status = b->status;
+    kp.lists_ok = b->lists_ok;
     kp.executed = b->executed;
     kp.out = b->out;
     kp.P = m->P;
@@ -2384,7 +2451,7 @@ void kmcg_batch_free(kmcg_batch *b)
                     b->avail_pool, b->nr, b->rates, b->integ, b->pstat, b->ktime, b->kdt, b->kstep,
                     b->rngctr, b->status, b->executed, b->budget, b->rep_ids, b->out, b->occ,
                     b->samples, b->msd_hop, b->msd_d, b->msd_id, b->msd_site, b->msd_disp,
-                    b->msd_off_d, b->msd_ev, b->msd_evn, b->acf_pa, b->acf_po, b->acf_buf,
+                    b->msd_off_d, b->msd_ev, b->msd_evn, b->lists_ok, b->acf_pa, b->acf_po, b->acf_buf,
                     b->acf_pp, b->acf_bin, b->acf_cnt, b->acf_boff, b->acf_d, b->acf_i};
     for (void *p : ptrs)
         if (p)
@@ -2474,6 +2541,7 @@ int kmcg_batch_create(const kmcg_model *m, int32_t n, const int32_t *sizes, int3
     BALLOC(b->kstep, n);
     BALLOC(b->rngctr, n);
     BALLOC(b->status, n);
+    BALLOC(b->lists_ok, n);
     BALLOC(b->executed, n);
     BALLOC(b->budget, n);
     BALLOC(b->rep_ids, n);
@@ -2496,6 +2564,7 @@ int kmcg_batch_create(const kmcg_model *m, int32_t n, const int32_t *sizes, int3
         return bail(fail(KMCG_EHIP, "copy to device failed"));
     if (hipMemsetAsync(b->rates, 0, n * P * sizeof(double), b->stream) != hipSuccess ||
         hipMemsetAsync(b->status, 0, n * sizeof(int), b->stream) != hipSuccess ||
+        hipMemsetAsync(b->lists_ok, 0, n * sizeof(int), b->stream) != hipSuccess ||
         hipMemsetAsync(b->avail_pool, 0, b->avail_total * sizeof(int), b->stream) != hipSuccess ||
         hipMemsetAsync(b->lat_pool, 0, b->lat_total, b->stream) != hipSuccess ||
         hipStreamSynchronize(b->stream) != hipSuccess)
@@ -2529,6 +2598,16 @@ static int pack_config(kmcg_batch *b, int r, const int8_t *config, std::vector<u
     }
     return KMCG_OK;
 }
+
+namespace {
+// mark replicas [r0, r0+n) as having lists that do (1) or may not (0) match
+// their lattices
+int set_lists_ok(kmcg_batch *b, int r0, int n, int v)
+{
+    std::vector<int> f(n, v);
+    return copy_in(b, b->lists_ok + r0, f.data(), (size_t)n);
+}
+} // namespace
 
 int kmcg_init(kmcg_batch *b, int32_t r0, int32_t n, const uint32_t *seeds, const uint32_t *streams,
               const int8_t *configs)
@@ -2588,7 +2667,7 @@ int kmcg_init(kmcg_batch *b, int32_t r0, int32_t n, const uint32_t *seeds, const
         hipLaunchKernelGGL((kmc_init_lists_kernel<8>), grid, dim3(256), 0, b->stream, kp);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(b->stream));
-    return KMCG_OK;
+    return set_lists_ok(b, r0, n, 1);
 }
 
 int kmcg_set_rates(kmcg_batch *b, int32_t r0, int32_t n, const double *rates)
@@ -2628,7 +2707,8 @@ int kmcg_set_config(kmcg_batch *b, int32_t r0, int32_t n, const int8_t *config)
         off += b->V[r];
         ids[i] = r;
     }
-    return run_misc(b, ids, MODE_ADJUST, 0, 0, 0, 0);
+    int rc = run_misc(b, ids, MODE_ADJUST, 0, 0, 0, 0);
+    return rc ? rc : set_lists_ok(b, r0, n, 1);
 }
 
 // track: bit 0 displacement tracking, bit 1 trajectory, bit 2 ACF bins
@@ -2843,6 +2923,9 @@ int kmcg_replace_species(kmcg_batch *b, int32_t r, int32_t site, int32_t old_sp,
         return rc;
     if (lat[site - 1] != old_sp)
         return fail(KMCG_EINVAL, "replace_species: site holds another species");
+    // the lists no longer follow the lattice until the next set_config
+    if ((rc = set_lists_ok(b, r, 1, 0)))
+        return rc;
     const long long s = site - 1;
     const long long byte = b->bits == 8 ? s : (s * 2) >> 3;
     unsigned char cur;
@@ -3445,7 +3528,7 @@ int kmcg_set_state(kmcg_batch *b, int32_t r, const double *time2, const int64_t 
         (rc = copy_in(b, b->rngctr + r, (const unsigned long long *)(step2 + 1), 1)) ||
         (rc = copy_in(b, b->status + r, &running, 1)))
         return rc;
-    return KMCG_OK;
+    return set_lists_ok(b, r, 1, 0);      // a restored state is not checked against its lattice
 }
 
 int kmcg_get_occupation(kmcg_batch *b, int32_t r0, int32_t n, double *out)
