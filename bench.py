@@ -228,7 +228,7 @@ def pairwise_leg(engine, K, no_cpu):
                         "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(wl.name),
                         "algorithmic_bytes_per_kmc_step": bps}}
     if not no_cpu:
-        out["cpu_baseline"] = cpu_baseline(wl, cm, 15.0)
+        out["cpu_baseline"] = cpu_baseline(wl, cm, 5.0)
         out["gpu_over_cpu"] = out["gpu_steps_per_s"] / out["cpu_baseline"]["value"]
     return out
 
