@@ -10,7 +10,10 @@ workload (4.1e8 kMC steps); W warm-up launches run on a separate batch.
 Printed (one JSON line): whole-job kMC steps/s (value), the plain path (one
 replica per launch) on the same model, their ratio, the HBM roofline of
 the step kernel from the algorithmic bytes per kMC step (int32 avail_sites
-entries read or written, counted by the instrumented oracle), and a CPU
+entries read or written, counted by the instrumented oracle) with the LDS
+bandwidth beside it (LDS bytes per launch from the committed rocprofv3
+counter summary over this run's launch time), the launch shape the
+autotuner chose (kmos_amd/autotune.py, trials on a probe batch), and a CPU
 baseline (the C oracle on the host cores, one process per core, on a
 bounded sample of the same workload).  Two more legs ride along:
 
@@ -55,6 +58,33 @@ def measured_traffic(workload):
         if line.startswith("traffic_bytes_per_launch:"):
             return float(line.split(":")[1])
     return None
+
+
+PEAK_LDS_GBS = 256 * 128 * 2.4   # CUs x bytes per clock per CU x GHz
+
+
+def measured_lds(workload):
+    """LDS bytes per launch of the step kernel from the rocprofv3 PMC summary
+    committed for this workload (the SQ_INSTS_LDS_*_BANDWIDTH counters), or
+    None."""
+    path = os.path.join(REPO, "profiles", "kmc_steps_kernel_%s_lds.md" % workload)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("lds_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def lds_figures(workload, avg_launch_s):
+    """Achieved LDS bandwidth of the step kernel: the profiled LDS bytes per
+    launch over this run's average launch time, against the LDS peak."""
+    b = measured_lds(workload)
+    if b is None:
+        return None
+    achieved = b / avg_launch_s / 1e9
+    return {"bytes_per_launch": b, "achieved": achieved, "peak": PEAK_LDS_GBS, "unit": "GB/s",
+            "frac": achieved / PEAK_LDS_GBS}
 
 
 def _oracle_worker(args):
@@ -161,34 +191,59 @@ def reference_cpu_baseline(wl, per_core=12):
                       % (nrep, wl.steps, wl.name, wall)}
 
 
-def run_batched(engine, model, wl, K, warmup):
+def run_batched(engine, model, wl, K, warmup, tune=True):
     """Time K launches that together run the whole workload; returns the
-    batch, wall seconds, executed steps and per-launch device ms."""
+    batch, wall seconds, executed steps, per-launch device ms, the kMC steps
+    per launch and the launch shape the autotuner chose (None untuned).
+    The warm-up launches and the autotuner's trials run on a probe batch of
+    the same replicas, so the timed batch starts from the workload's
+    initial state."""
+    from kmos_amd import autotune
     chunk = max(1, wl.steps // K)
-    if warmup > 0:
+    probe = min(chunk, 1000)
+    choice = None
+    if tune:
+        choice = autotune.tune_launch(lambda: make_batch(engine, model, wl), steps=probe,
+                                      warmup=probe * max(warmup, 0), repeats=2)
+    elif warmup > 0:
         wb = make_batch(engine, model, wl, np.arange(min(wl.n, 256)))
         for _ in range(warmup):
             wb.run(min(chunk, 2000))
         wb.close()
     b = make_batch(engine, model, wl)
+    ids = autotune.apply(b, choice) if choice else None
     kernel_ms = []
     done_before = b.scalars_all()[1].sum()
     b.sync()
     t0 = time.perf_counter()
     for _ in range(K):
-        b.run(chunk, sync=False)
+        b.run(chunk, ids, sync=False)
         kernel_ms.append(b.last_run_ms())   # waits for the launch, reads its HIP events
     b.sync()
     wall = time.perf_counter() - t0
     executed = int(b.scalars_all()[1].sum() - done_before)
-    return b, wall, executed, kernel_ms, chunk
+    return b, wall, executed, kernel_ms, chunk, choice
 
 
-def ruo2_leg(engine, K, no_cpu):
+def launch_summary(choice):
+    """The autotuner's choice and how it compares with the default shape
+    (lattice in LDS when it fits, four replica waves per workgroup, replicas
+    in the given order)."""
+    if not choice:
+        return None
+    default = [t["ms"] for t in choice["table"]
+               if (t["lattice_in_lds"], t["waves_per_group"], t["order"]) == (-1, 4, "given")]
+    return {"lattice_in_lds": choice["lattice_in_lds"], "waves_per_group": choice["waves_per_group"],
+            "replica_order": choice["order"], "candidates": len(choice["table"]),
+            "trial_kmc_steps_per_replica": choice["steps"], "best_trial_ms": choice["ms"],
+            "default_trial_ms": default[0] if default else None}
+
+
+def ruo2_leg(engine, K, no_cpu, tune=True):
     wl = W.ruo2()
     cm = compile_model(wl.project, wl.backend)
     model = engine.Model(cm)
-    b, wall, executed, kernel_ms, chunk = run_batched(engine, model, wl, K, 1)
+    b, wall, executed, kernel_ms, chunk, choice = run_batched(engine, model, wl, K, 1, tune)
     st = b.scalars_all()[2]
     b.close()
     bps = algorithmic_bytes_per_step(wl, cm)
@@ -196,10 +251,11 @@ def ruo2_leg(engine, K, no_cpu):
     achieved = executed / K * bps / avg_s / 1e9
     out = {"workload": wl.name, "replicas": wl.n, "steps_per_replica": wl.steps,
            "gpu_steps_per_s": executed / wall, "avg_launch_ms": avg_s * 1e3,
+           "launch": launch_summary(choice),
            "replica_status_counts": {str(k): int(v) for k, v in zip(*np.unique(st, return_counts=True))},
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                         "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(wl.name),
-                        "algorithmic_bytes_per_kmc_step": bps}}
+                        "algorithmic_bytes_per_kmc_step": bps, "lds": lds_figures(wl.name, avg_s)}}
     if not no_cpu:
         ref = reference_cpu_baseline(wl)
         if ref is not None:
@@ -208,14 +264,14 @@ def ruo2_leg(engine, K, no_cpu):
     return out
 
 
-def pairwise_leg(engine, K, no_cpu):
+def pairwise_leg(engine, K, no_cpu, tune=True):
     """The lateral-interaction workload (512 replicas of 256^2, pairwise
     lat_int, 5e5 steps each) with its own roofline figures and the C
     oracle on the host cores beside it (no reference build exists for it)."""
     wl = W.pairwise()
     cm = compile_model(wl.project, wl.backend)
     model = engine.Model(cm)
-    b, wall, executed, kernel_ms, chunk = run_batched(engine, model, wl, K, 1)
+    b, wall, executed, kernel_ms, chunk, choice = run_batched(engine, model, wl, K, 1, tune)
     st = b.scalars_all()[2]
     b.close()
     bps = algorithmic_bytes_per_step(wl, cm)
@@ -223,20 +279,32 @@ def pairwise_leg(engine, K, no_cpu):
     achieved = executed / K * bps / avg_s / 1e9
     out = {"workload": wl.name, "replicas": wl.n, "steps_per_replica": wl.steps,
            "gpu_steps_per_s": executed / wall, "avg_launch_ms": avg_s * 1e3,
+           "launch": launch_summary(choice),
            "replica_status_counts": {str(k): int(v) for k, v in zip(*np.unique(st, return_counts=True))},
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                         "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(wl.name),
-                        "algorithmic_bytes_per_kmc_step": bps}}
+                        "algorithmic_bytes_per_kmc_step": bps, "lds": lds_figures(wl.name, avg_s)}}
     if not no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, cm, 5.0)
         out["gpu_over_cpu"] = out["gpu_steps_per_s"] / out["cpu_baseline"]["value"]
     return out
 
 
-def serving_leg(rate, n, seed=1):
-    from kmos_amd import serve
+def serving_leg(rate, n, seed=1, tune=True):
+    """The serving stream; with `tune`, the sampling intervals per launch
+    are first chosen by the autotuner on a shorter stream of the same
+    distribution (another seed)."""
+    from kmos_amd import autotune, serve
     reqs = serve.poisson_stream(rate=rate, n=n, seed=seed)
-    srv = serve.Server(hbm_budget=200 << 30)
+    ipl, tuning = 4, None
+    if tune:
+        res = autotune.tune_serving(lambda k: serve.Server(hbm_budget=200 << 30, intervals_per_launch=k),
+                                    lambda: serve.poisson_stream(rate=rate, n=max(20, n // 4), seed=seed + 1000),
+                                    candidates=(1, 2, 4, 8))
+        ipl = res["intervals_per_launch"]
+        tuning = [{k: r[k] for k in ("intervals_per_launch", "p50_latency_s", "p99_latency_s", "steps_per_s")}
+                  for r in res["table"]]
+    srv = serve.Server(hbm_budget=200 << 30, intervals_per_launch=ipl)
     t0 = time.perf_counter()
     res = srv.serve(reqs, realtime=True)
     wall = time.perf_counter() - t0
@@ -244,7 +312,8 @@ def serving_leg(rate, n, seed=1):
     out.update({"arrival_rate_per_s": rate, "ticks": srv.stats["ticks"], "pages": srv.stats["pages"],
                 "peak_active": srv.stats["peak_active"], "peak_hbm_bytes": srv.stats["peak_bytes"],
                 "steps_range": [1e4, 1e6], "sample_every": 1000,
-                "sizes": list(serve.SIZES), "models": ["quickstart", "zgb", "ruo2"]})
+                "sizes": list(serve.SIZES), "models": ["quickstart", "zgb", "ruo2"],
+                "intervals_per_launch": ipl, "autotune": tuning})
     srv.close()
     return out
 
@@ -293,6 +362,8 @@ def main():
     ap.add_argument("--no-pairwise", action="store_true", help="skip the lateral-interaction leg")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="request arrivals per second")
     ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="default launch shape and serving chunk length instead of the autotuner's")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of 1 GiB five times (calibrates HBM byte counters)")
     a = ap.parse_args()
@@ -318,7 +389,7 @@ def main():
         return 1
     model = engine.Model(cm)
     K = max(1, a.steps)
-    b, wall, executed, kernel_ms, chunk = run_batched(engine, model, wl, K, a.warmup)
+    b, wall, executed, kernel_ms, chunk, choice = run_batched(engine, model, wl, K, a.warmup, not a.no_tune)
     _progress("batched %s: %d launches timed" % (wl.name, K))
     t_after, k_after, st_after = b.scalars_all()
     value = executed / wall
@@ -367,6 +438,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (workload parameters from a seeded Philox generator)",
         "config": dict(wl.describe(), kmc_steps_per_launch=chunk),
+        "launch": launch_summary(choice),
         "plain_gpu_steps_per_s": plain,
         "batched_over_plain": value / plain,
         "replica_status_counts": {str(k): int(v) for k, v in zip(*np.unique(st_after, return_counts=True))},
@@ -374,7 +446,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_kmc_step": bytes_per_step,
-                     "kernel": "kmc_steps_kernel", "avg_launch_ms": avg_kernel_s * 1e3},
+                     "kernel": "kmc_steps_kernel", "avg_launch_ms": avg_kernel_s * 1e3,
+                     "lds": lds_figures(wl.name, avg_kernel_s)},
     }
     _progress("plain path and spot check done")
     if not a.no_cpu:
@@ -382,13 +455,13 @@ def main():
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
         _progress("cpu baseline done")
     if not a.no_ruo2 and a.workload == "zgb":
-        out["ruo2"] = ruo2_leg(engine, K, a.no_cpu)
+        out["ruo2"] = ruo2_leg(engine, K, a.no_cpu, not a.no_tune)
         _progress("ruo2 leg done")
     if not a.no_pairwise and a.workload == "zgb":
-        out["pairwise"] = pairwise_leg(engine, K, a.no_cpu)
+        out["pairwise"] = pairwise_leg(engine, K, a.no_cpu, not a.no_tune)
         _progress("pairwise leg done")
     if not a.no_serve:
-        out["serving"] = serving_leg(a.serve_rate, a.serve_requests)
+        out["serving"] = serving_leg(a.serve_rate, a.serve_requests, tune=not a.no_tune)
         _progress("serving leg done")
         out["p50_latency_s"] = out["serving"]["p50_latency_s"]
         out["p99_latency_s"] = out["serving"]["p99_latency_s"]
