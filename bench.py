@@ -225,6 +225,26 @@ def run_batched(engine, model, wl, K, warmup, tune=True):
     return b, wall, executed, kernel_ms, chunk, choice
 
 
+def graph_leg(engine, model, wl, steps_per_run=50, runs=300):
+    """Host cost of short runs: one replica advanced `runs` times by
+    `steps_per_run` kMC steps, by direct launches and by replays of a
+    captured HIP graph (kmcg_set_graph), wall microseconds per run."""
+    out = {"steps_per_run": steps_per_run, "runs": runs}
+    for mode in ("direct", "graph"):
+        b = make_batch(engine, model, wl, [0])
+        b.set_graph(mode == "graph")
+        ids = np.zeros(1, np.int32)
+        for _ in range(5):
+            b.run(steps_per_run, ids)
+        t0 = time.perf_counter()
+        for _ in range(runs):
+            b.run(steps_per_run, ids)
+        out[mode + "_us_per_run"] = (time.perf_counter() - t0) / runs * 1e6
+        out[mode + "_kernel_us"] = b.last_run_ms() * 1e3
+        b.close()
+    return out
+
+
 def launch_summary(choice):
     """The autotuner's choice and how it compares with the default shape
     (lattice in LDS when it fits, four replica waves per workgroup, replicas
@@ -450,6 +470,7 @@ def main():
                      "lds": lds_figures(wl.name, avg_kernel_s)},
     }
     _progress("plain path and spot check done")
+    out["graph_replay"] = graph_leg(engine, model, wl)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, cm, a.cpu_seconds)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

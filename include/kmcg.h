@@ -108,6 +108,16 @@ int kmcg_sync(kmcg_batch *b);
  * no size limit), -1 = automatic; replica-waves per workgroup sharing one
  * copy of the op program (1..4, 0 = as many as LDS allows). */
 int kmcg_set_launch(kmcg_batch *b, int32_t lattice_in_lds, int32_t waves_per_group);
+/* Runs through HIP graphs (1) or direct launches (0, the default).  With
+ * graphs, the stream work of a kmcg_run / kmcg_run_recorded (replica list
+ * and budget copies, the step kernel, copies back) is captured once per
+ * distinct set of launch values and replayed while runs repeat them (same
+ * replica count, steps, launch shape, buffers); the copies read the pinned
+ * staging buffers, so each replay uses that run's replica list and budgets.
+ * Up to 8 graphs are kept per batch; 0 frees them.  No counterpart in the
+ * reference (do_kmc_steps is a host loop, proclist_generic_subroutines.mpy:
+ * 1-80); it is for callers that repeat short runs. */
+int kmcg_set_graph(kmcg_batch *b, int32_t enable);
 /* Milliseconds the last kmcg_run took on the device (HIP events around the
  * kernel on the batch's stream). */
 int kmcg_last_run_ms(kmcg_batch *b, float *ms);

@@ -1612,6 +1612,18 @@ struct kmcg_batch {
     int n_last = 0;
     float last_ms = 0.f;
     bool timed_run_pending = false;   // ev0/ev1 bracket a launch not yet read
+    // kmcg_set_graph: a run's stream work (replica list and budget copies,
+    // the step kernel, budget and record copies back) captured in a HIP
+    // graph, keyed by everything the captured nodes hold, and replayed
+    // while a run repeats it; the copies read the pinned staging buffers,
+    // so a replay moves their contents at the time it runs
+    struct Captured {
+        std::vector<unsigned char> key;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    bool use_graph = false;
+    std::vector<Captured> graphs;     // oldest first, at most MAX_GRAPHS
     // 1 per replica whose lists match its lattice (set by kmcg_init and
     // kmcg_set_config, cleared by kmcg_replace_species and kmcg_set_state)
     int *lists_ok = nullptr;
@@ -2168,7 +2180,9 @@ int ppl_of(int P) { return P <= 64 ? 1 : (P <= 128 ? 2 : 4); }
 
 KParams make_params(kmcg_batch *b)
 {
-    KParams kp{};
+    // every byte zeroed, padding included: run_impl keys graphs on them
+    KParams kp;
+    memset(&kp, 0, sizeof kp);
     const kmcg_model *m = b->m;
     kp.prog = b->prog;
     kp.rep_ids = b->rep_ids;
@@ -2357,6 +2371,17 @@ int copy_in(kmcg_batch *b, T *dst, const T *src, size_t count)
     return KMCG_OK;
 }
 
+constexpr size_t MAX_GRAPHS = 8;
+
+void drop_graphs(kmcg_batch *b)
+{
+    for (auto &c : b->graphs) {
+        (void)hipGraphExecDestroy(c.exec);
+        (void)hipGraphDestroy(c.graph);
+    }
+    b->graphs.clear();
+}
+
 } // namespace
 
 // ===========================================================================
@@ -2466,6 +2491,7 @@ void kmcg_batch_free(kmcg_batch *b)
         (void)hipHostFree(b->h_srec);
     if (b->srec)
         (void)hipFree(b->srec);
+    drop_graphs(b);
     if (b->ev0)
         (void)hipEventDestroy(b->ev0);
     if (b->ev1)
@@ -2570,6 +2596,19 @@ int kmcg_batch_create(const kmcg_model *m, int32_t n, const int32_t *sizes, int3
         hipStreamSynchronize(b->stream) != hipSuccess)
         return bail(fail(KMCG_EHIP, "memset failed"));
     *out = b;
+    return KMCG_OK;
+}
+
+int kmcg_set_graph(kmcg_batch *b, int32_t enable)
+{
+    if (!b || enable < 0 || enable > 1)
+        return fail(KMCG_EINVAL, "bad argument");
+    int rc = settle(b);
+    if (rc)
+        return rc;
+    b->use_graph = enable != 0;
+    if (!b->use_graph)
+        drop_graphs(b);
     return KMCG_OK;
 }
 
@@ -2724,11 +2763,8 @@ static int run_impl(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t s
     if (rc)
         return rc;
     memcpy(b->h_ids, replicas, n * sizeof(int));
-    HIPCHK(hipMemcpyAsync(b->rep_ids, b->h_ids, n * sizeof(int), hipMemcpyHostToDevice, b->stream));
-    if (budget) {
+    if (budget)
         memcpy(b->h_budget, budget, n * sizeof(long long));
-        HIPCHK(hipMemcpyAsync(b->budget, b->h_budget, n * sizeof(long long), hipMemcpyHostToDevice, b->stream));
-    }
     KParams kp = make_params(b);
     const Shape s = launch_shape(b, replicas, n, kp, b->max_wpg);
     kp.nsteps = steps;
@@ -2777,18 +2813,75 @@ static int run_impl(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t s
         kp.srec_cap = max_records;
         kp.srec_width = width;
     }
-    HIPCHK(hipEventRecord(b->ev0, b->stream));
-    HIPCHK(launch(b, true, s, kp));
-    HIPCHK(hipEventRecord(b->ev1, b->stream));
+    const size_t cnt = records ? (size_t)n * max_records * width : 0;
+    // the stream work of the run; `timed` brackets the kernel with the
+    // batch's events (not inside a graph: a replay is timed as a whole)
+    auto enqueue = [&](bool timed) -> hipError_t {
+        hipError_t e = hipMemcpyAsync(b->rep_ids, b->h_ids, n * sizeof(int), hipMemcpyHostToDevice, b->stream);
+        if (e == hipSuccess && budget)
+            e = hipMemcpyAsync(b->budget, b->h_budget, n * sizeof(long long), hipMemcpyHostToDevice, b->stream);
+        if (e == hipSuccess && timed)
+            e = hipEventRecord(b->ev0, b->stream);
+        if (e == hipSuccess)
+            e = launch(b, true, s, kp);
+        if (e == hipSuccess && timed)
+            e = hipEventRecord(b->ev1, b->stream);
+        if (e == hipSuccess && budget)
+            e = hipMemcpyAsync(b->h_budget, b->budget, n * sizeof(long long), hipMemcpyDeviceToHost, b->stream);
+        if (e == hipSuccess && records)
+            e = hipMemcpyAsync(b->h_srec, b->srec, cnt * sizeof(double), hipMemcpyDeviceToHost, b->stream);
+        return e;
+    };
+    if (!b->use_graph) {
+        HIPCHK(enqueue(true));
+    } else {
+        // the key: every value a captured node holds
+        const int flags[6] = {n, budget ? 1 : 0, records ? 1 : 0, s.lds_lat ? 1 : 0, s.wpg, s.grid};
+        const size_t lds = s.lds;
+        const void *hs = b->h_srec;
+        std::vector<unsigned char> key(sizeof kp + sizeof flags + sizeof lds + sizeof hs + sizeof cnt);
+        unsigned char *kq = key.data();
+        memcpy(kq, &kp, sizeof kp); kq += sizeof kp;
+        memcpy(kq, flags, sizeof flags); kq += sizeof flags;
+        memcpy(kq, &lds, sizeof lds); kq += sizeof lds;
+        memcpy(kq, &hs, sizeof hs); kq += sizeof hs;
+        memcpy(kq, &cnt, sizeof cnt);
+        hipGraphExec_t exec = nullptr;
+        for (auto &c : b->graphs)
+            if (c.key == key)
+                exec = c.exec;
+        if (!exec) {
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            const hipError_t e = enqueue(false);
+            const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (g)
+                    (void)hipGraphDestroy(g);
+                HIPCHK(e != hipSuccess ? e : e2);
+            }
+            const hipError_t e3 = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+            if (e3 != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                HIPCHK(e3);
+            }
+            if (b->graphs.size() >= MAX_GRAPHS) {
+                (void)hipGraphExecDestroy(b->graphs.front().exec);
+                (void)hipGraphDestroy(b->graphs.front().graph);
+                b->graphs.erase(b->graphs.begin());
+            }
+            b->graphs.push_back({std::move(key), g, exec});
+        }
+        HIPCHK(hipEventRecord(b->ev0, b->stream));
+        HIPCHK(hipGraphLaunch(exec, b->stream));
+        HIPCHK(hipEventRecord(b->ev1, b->stream));
+    }
     b->timed_run_pending = true;
     if (budget) {
-        HIPCHK(hipMemcpyAsync(b->h_budget, b->budget, n * sizeof(long long), hipMemcpyDeviceToHost, b->stream));
         b->user_budget = budget;
         b->n_last = n;
     }
     if (records) {
-        const size_t cnt = (size_t)n * max_records * width;
-        HIPCHK(hipMemcpyAsync(b->h_srec, b->srec, cnt * sizeof(double), hipMemcpyDeviceToHost, b->stream));
         b->user_srec = records;
         b->n_srec = cnt;
     }

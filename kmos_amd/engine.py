@@ -124,6 +124,10 @@ class Batch:
     def set_launch(self, lattice_in_lds: int = -1, waves_per_group: int = 0):
         _lib.check(_lib.load().kmcg_set_launch(self.h, lattice_in_lds, waves_per_group))
 
+    def set_graph(self, enable: bool = True):
+        """Replay runs through captured HIP graphs (kmcg_set_graph)."""
+        _lib.check(_lib.load().kmcg_set_graph(self.h, 1 if enable else 0))
+
     # ---- stepping ---------------------------------------------------------
     def run(self, steps: int, replicas=None, budget=None, sync: bool = True):
         """do_kmc_steps(steps) on the given replicas (all by default)."""
