@@ -152,52 +152,54 @@ struct Rows {
     __device__ __forceinline__ int &b(int i) const { return *(int *)(base + (bo + ((unsigned)i << 2))); }
 };
 
-// w: this lane's window in LDS, slot q at w[q - (nr0 - RBLK + 1)]
-template <int PPL, int BITS>
+// w: this lane's window in LDS, slot q at w[q - (nr0 - RB + 1)].  RB (1, 2
+// or RBLK) bounds the entries any lane replays in this block: the reads,
+// the window and the pairwise tests are set up for RB entries only.
+template <int PPL, int BITS, int RB>
 __device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, const Rows &L, int *w,
                                              const int (&v)[RBLK], int m, int &err)
 {
     const int nr0 = nr;
     // per entry: deletion?, site; pairwise equal sites, computed once
-    bool isdel[RBLK];
-    int st[RBLK];
+    bool isdel[RB];
+    int st[RB];
 #pragma unroll
-    for (int i = 0; i < RBLK; ++i) {
+    for (int i = 0; i < RB; ++i) {
         isdel[i] = i < m && (v[i] & 3) != MB_ADD;
         st[i] = v[i] >> 2;
     }
     // all reads are issued unconditionally (index 0 = the unused slot 0
     // when not needed) so that they form one batch with a single wait
-    int bv[RBLK];
+    int bv[RB];
     int nd = 0;
 #pragma unroll
-    for (int i = 0; i < RBLK; ++i) {
+    for (int i = 0; i < RB; ++i) {
         bv[i] = L.b(isdel[i] ? st[i] : 0);
         nd += isdel[i] ? 1 : 0;
     }
-    int tv[RBLK];
+    int tv[RB];
 #pragma unroll
-    for (int j = 0; j < RBLK; ++j)
+    for (int j = 0; j < RB; ++j)
         tv[j] = L.s((j < nd && nr0 - j >= 1) ? nr0 - j : 0);
-    bool same[RBLK][RBLK];     // same[i][j], i < j: entry j deletes entry i's site
+    bool same[RB][RB];     // same[i][j], i < j: entry j deletes entry i's site
 #pragma unroll
-    for (int i = 0; i < RBLK; ++i)
+    for (int i = 0; i < RB; ++i)
 #pragma unroll
-        for (int j = i + 1; j < RBLK; ++j)
+        for (int j = i + 1; j < RB; ++j)
             same[i][j] = isdel[j] && st[j] == st[i];
 #pragma unroll
-    for (int i = 0; i < RBLK; ++i)
+    for (int i = 0; i < RB; ++i)
         bv[i] = isdel[i] ? bv[i] : 0;
     // slots above nr0 are only read after an addition of this block wrote
     // them, slots below nr0-nd+1 are never read (their entries here are
     // written anyway, so that no read above is conditional)
 #pragma unroll
-    for (int j = 0; j < RBLK; ++j)
-        w[RBLK - 1 - j] = tv[j];
+    for (int j = 0; j < RB; ++j)
+        w[RB - 1 - j] = tv[j];
     int cur = nr0;
-    const int base = nr0 - (RBLK - 1);
+    const int base = nr0 - (RB - 1);
 #pragma unroll
-    for (int i = 0; i < RBLK; ++i) {
+    for (int i = 0; i < RB; ++i) {
         if (!__ballot(i < m))
             break;
         if (i < m) {
@@ -211,7 +213,7 @@ __device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, const 
                     L.b(site) = cur;
                     w[cur - base] = site;
 #pragma unroll
-                    for (int j = i + 1; j < RBLK; ++j)
+                    for (int j = i + 1; j < RB; ++j)
                         bv[j] = same[i][j] ? cur : bv[j];
                 }
             } else {
@@ -229,13 +231,13 @@ __device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, const 
                             w[mpos - base] = moved;
                         // bv of additions is unused, so no deletion test here
 #pragma unroll
-                        for (int j = i + 1; j < RBLK; ++j)
+                        for (int j = i + 1; j < RB; ++j)
                             bv[j] = (st[j] == moved) ? mpos : bv[j];
                     }
                     L.s(last) = 0;
                     L.b(site) = 0;
 #pragma unroll
-                    for (int j = i + 1; j < RBLK; ++j)
+                    for (int j = i + 1; j < RB; ++j)
                         bv[j] = same[i][j] ? 0 : bv[j];
                     cur = last - 1;
                 }
@@ -248,7 +250,7 @@ __device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, const 
 // Apply the queued entries of the step: each owner lane replays its
 // processes' queues in order, RBLK entries per batch of HBM reads, and
 // empties them.
-template <int PPL, int BITS>
+template <int PPL, int BITS, bool RB2 = false>
 __device__ __forceinline__ int apply_queues(Wave<PPL, BITS> &W)
 {
     int err = 0;
@@ -281,7 +283,15 @@ __device__ __forceinline__ int apply_queues(Wave<PPL, BITS> &W)
                 L.base = (char *)W.avail;
                 L.so = (unsigned)((long long)p * W.V1) << 2;
                 L.bo = (unsigned)((long long)(W.P + p) * W.V1) << 2;
-                replay_block<PPL, BITS>(W, nrk, L, W.win + WIN * W.lane, v, m, err);
+                // blocks of one entry per lane are the common case; a
+                // two-entry form as well (RB2) costs code size, so only the
+                // plain step loop takes it
+                if (__ballot(m > 2) || (!RB2 && __ballot(m > 1)))
+                    replay_block<PPL, BITS, RBLK>(W, nrk, L, W.win + WIN * W.lane, v, m, err);
+                else if (RB2 && __ballot(m > 1))
+                    replay_block<PPL, BITS, 2>(W, nrk, L, W.win + WIN * W.lane, v, m, err);
+                else
+                    replay_block<PPL, BITS, 1>(W, nrk, L, W.win + WIN * W.lane, v, m, err);
                 W.nr[k] = nrk;
             }
         }
@@ -648,7 +658,7 @@ __device__ __forceinline__ void proc_prefetch(const int *prog, int off_proc, int
 // `merged`, all segments of the process are queued and applied once, else
 // each segment is applied before the next is posted.  The first segment's
 // static words come read ahead in `pre`.
-template <int PPL, int BITS>
+template <int PPL, int BITS, bool RB2 = false>
 __device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, bool merged)
 {
     const int s0 = site - 1;
@@ -675,7 +685,7 @@ __device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, bool m
             // near the faces the segments run one by one
             if (W.d.interior(bx1, by1, bz1) && W.d.interior(bx2, by2, bz2) &&
                 post_pair<PPL, BITS, true>(W, cur, q2, bx1, by1, bz1, bx2, by2, bz2) >= 0)
-                return apply_queues<PPL, BITS>(W);
+                return apply_queues<PPL, BITS, RB2>(W);
         }
     }
     for (int i = 0; i < nit; ++i, it += IT_W) {
@@ -689,12 +699,12 @@ __device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, bool m
         if (e)
             break;
         if (!merged) {
-            e = apply_queues<PPL, BITS>(W);
+            e = apply_queues<PPL, BITS, RB2>(W);
             if (e)
                 return e;
         }
     }
-    const int a = merged ? apply_queues<PPL, BITS>(W) : ST_OK;
+    const int a = merged ? apply_queues<PPL, BITS, RB2>(W) : ST_OK;
     return e ? e : a;
 }
 
@@ -1211,7 +1221,7 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
 #pragma unroll
         for (int k = 0; k < PPL; ++k)
             nr_before[k] = W.nr[k];
-        st = run_proc<PPL, BITS>(W, pre, site, kp.merged);
+        st = run_proc<PPL, BITS, !MSD>(W, pre, site, kp.merged);
         if (MSD && kp.acf_pa && lane == 0)
             acf_after(kp, r);
         if (st)
