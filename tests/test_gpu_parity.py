@@ -265,3 +265,32 @@ def test_species_mismatch_matches_oracle(engine, oracle, proc_name, second):
     oracle.lib().orc_run_proc_nr(o.h, proc, site)
     assert o.status == 2
     compare_state(b, 0, o, where="mismatch %s" % proc_name)
+
+
+@pytest.mark.parametrize("factory,backend", [(W.ruo2_project, "local_smart"), (W.pairwise_project, "lat_int")])
+def test_kmc_time_within_one_ulp_per_step(engine, oracle, factory, backend):
+    """kmc_time after every one of 400 steps against the oracle's: the
+    clocks may part by the device log's last bit, at most one unit in the
+    last place of kmc_time per step so far."""
+    cm = compile_model(factory(), backend)
+    g = np.random.default_rng(17)
+    sizes = [(9, 7), (16, 12)]
+    rates = 10.0 ** g.uniform(-2, 6, (len(sizes), cm.nr_of_proc))
+    m = engine.Model(cm)
+    b = engine.Batch(m, [list(s) + [1] for s in sizes])
+    seeds = np.array([5, 6], np.uint32)
+    b.init(seeds, np.zeros(2, np.uint32))
+    b.set_rates(rates)
+    configs = [_random_config(cm.project, int(v), g) for v in b.volumes]
+    b.set_config(configs)
+    worst = 0.0
+    for r, s in enumerate(sizes):
+        o = oracle_replica(oracle, cm, s, int(seeds[r]), 0, rates[r], configs[r])
+        _, _, _, tref = o.run(400, trace=True)
+        for k in range(400):
+            b.run(1, [r])
+            t = b.scalars(r)[0]
+            err = abs(t - tref[k]) / np.spacing(tref[k])
+            worst = max(worst, err / (k + 1))
+            assert err <= k + 1, "%s r%d step %d: %g ulp" % (backend, r, k + 1, err)
+    print("kmc_time: worst %.3f ulp per step" % worst, flush=True)
