@@ -149,17 +149,6 @@ int kmcg_get_accum_rates(kmcg_batch *b, int32_t r, double *out);
 int kmcg_get_lattice(kmcg_batch *b, int32_t r, int8_t *out);
 /* avail_sites(:,:,1) as sites[P][V] and avail_sites(:,:,2) as back[P][V] */
 int kmcg_get_avail(kmcg_batch *b, int32_t r, int32_t *sites, int32_t *back);
-/* Overwrite the whole state of replica r: the restore half of a .reload
- * snapshot (base.mpy:363-514 reload_system; its writer is save_system,
- * base.mpy:517-578).  time2 = kmc_time, kmc_time_step; step2 = kmc_step,
- * rng counter; lattice[V] species; nr_of_sites[P]; procstat[P];
- * integ_rates[P] (NULL: zero, as reload_system leaves them); sites[P][V],
- * back[P][V] laid out as kmcg_get_avail returns them.  The lists must hold
- * the invariant add_proc/del_proc keep (base.mpy:211-301): for every
- * process, sites[1..nr] distinct sites in 1..V with back[site] = position,
- * every other slot and back entry 0; anything else is rejected with
- * KMCG_EINVAL before any byte reaches the device.  The replica's status
- * is reset to running; its seed and stream are kept. */
 /* Displacement tracking for mean squared displacements (kmos/run/acf.py
  * initialize_msd / do_kmc_steps_displacement / get_displacement /
  * get_trajectory / set_displacement_to_zero over base_acf.mpy:217-361,
@@ -219,9 +208,28 @@ int kmcg_run_acf(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t step
 int kmcg_acf_zero(kmcg_batch *b, int32_t r0, int32_t n);
 int kmcg_acf_get(kmcg_batch *b, int32_t r, double *config_bin, int32_t *counter_write_in_bin, double *d4,
                  int32_t *i5, double *buffer_acf, int32_t *property_o);
+/* Overwrite the whole state of replica r: the restore half of a .reload
+ * snapshot (base.mpy:363-514 reload_system; its writer is save_system,
+ * base.mpy:517-578).  time2 = kmc_time, kmc_time_step; step2 = kmc_step,
+ * rng counter; lattice[V] species; nr_of_sites[P]; procstat[P];
+ * integ_rates[P] (NULL: zero, as reload_system leaves them); sites[P][V],
+ * back[P][V] laid out as kmcg_get_avail returns them.  The lists must hold
+ * the invariant add_proc/del_proc keep (base.mpy:211-301): for every
+ * process, sites[1..nr] distinct sites in 1..V with back[site] = position,
+ * every other slot and back entry 0; anything else is rejected with
+ * KMCG_EINVAL before any byte reaches the device.  The replica's status
+ * is reset to running; its seed and stream are kept. */
 int kmcg_set_state(kmcg_batch *b, int32_t r, const double *time2, const int64_t *step2,
                    const int8_t *lattice, const int32_t *nr_of_sites, const int64_t *procstat,
                    const double *integ_rates, const int32_t *sites, const int32_t *back);
+/* Debug check (a slow path, for tests and suspicion, not for the step
+ * loop): for replicas [r0, r0+n) counts in bad[i] the violations of the
+ * list invariant add_proc/del_proc keep (base.mpy:211-301) and the
+ * (process, site) pairs whose registration disagrees with the process's
+ * conditions on the current lattice (the set the reference's touchups
+ * build, kmos/io/__init__.py:2411-2443; lat_int: the nli case trees).  0
+ * everywhere for a consistent state.  Synchronous. */
+int kmcg_check_lists(kmcg_batch *b, int32_t r0, int32_t n, int32_t *bad);
 /* proclist.get_occupation for replicas [r0, r0+n): out[n][S][spuck] */
 int kmcg_get_occupation(kmcg_batch *b, int32_t r0, int32_t n, double *out);
 /* procstat and integ_rates of replicas [r0, r0+n): [n][P] each */

@@ -33,6 +33,7 @@ SIGNATURES = [
     ("kmcg_sync", _i32, [_vp]),
     ("kmcg_set_launch", _i32, [_vp, _i32, _i32]),
     ("kmcg_set_graph", _i32, [_vp, _i32]),
+    ("kmcg_check_lists", _i32, [_vp, _i32, _i32, _vp]),
     ("kmcg_last_run_ms", _i32, [_vp, _vp]),
     ("kmcg_get_next_step", _i32, [_vp, _i32, _vp, _vp]),
     ("kmcg_run_proc_nr", _i32, [_vp, _i32, _i32, _i32]),

@@ -1463,6 +1463,77 @@ __global__ __launch_bounds__(256) void kmc_init_lists_kernel(KParams kp)
         kp.nr[(long long)r * P + p] = base;
 }
 
+// The debug check of one replica's lists (kmcg_check_lists), one block per
+// (replica, process): the invariant add_proc/del_proc keep (base.mpy:
+// 211-301: sites[1..nr] distinct sites of the process's executing site type
+// with back[site] = position, every other slot and back entry 0), and the
+// registered set against a scan of every cell with the process's
+// conditions (local_smart: the touchup's if-tree conditions; lat_int: the
+// nli case tree), the set kmc_init_lists_kernel builds.  Adds the number of
+// violations to bad[slot].
+template <int BITS>
+__global__ __launch_bounds__(256) void kmc_check_lists_kernel(KParams kp, int *bad)
+{
+    const int P = kp.P;
+    const int slot = blockIdx.x / P, p = blockIdx.x % P;
+    const int r = kp.rep_ids[slot];
+    const int4 dd = kp.dims[r];
+    Dims d;
+    d.X = dd.x; d.Y = dd.y; d.Z = dd.z; d.V = dd.w; d.spuck = kp.spuck; d.dim = kp.dim; d.moff = kp.moff;
+    d.strides();
+    Lattice<BITS> lat;
+    lat.p = kp.lat_pool + kp.lat_off[r];
+    const int *prog = kp.prog;
+    const int *ie = prog + kp.arg0 + 3 * p;
+    const int n_exec = ie[0], arg = ie[1], cbase = ie[2];
+    const bool lat_int = kp.backend == 1;
+    const int V = dd.w;
+    const long long V1 = (long long)V + 1;
+    const int *sp = kp.avail_pool + kp.avail_off[r] + (long long)p * V1;
+    const int *bp = kp.avail_pool + kp.avail_off[r] + (long long)(P + p) * V1;
+    const int nr = kp.nr[(long long)r * P + p];
+    const bool never = !lat_int && arg < 0;      // no condition set: never enabled
+    int nbad = (nr < 0 || nr > V || (never && nr != 0)) ? 1 : 0;
+    for (int k = threadIdx.x + 1; k <= V; k += blockDim.x) {
+        const int site = sp[k];
+        if (k <= nr)
+            nbad += (site < 1 || site > V || bp[site] != k || (site - 1) % d.spuck + 1 != n_exec) ? 1 : 0;
+        else
+            nbad += site != 0 ? 1 : 0;
+        const int back = bp[k];
+        if (back != 0)
+            nbad += (back < 0 || back > nr || sp[back] != k) ? 1 : 0;
+    }
+    if (!never) {
+        int w0 = 0, w1 = 0;
+        if (!lat_int) {
+            w0 = prog[arg];
+            w1 = prog[arg + 1];
+        }
+        const int nc = (w0 >> 17) & 0x3F;
+        const int *cd = prog + cbase + (int)((unsigned)w1 >> 21);
+        const int ncell = dd.x * dd.y * dd.z;
+        for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+            const int x = c % dd.x, y = (c / dd.x) % dd.y, z = c / (dd.x * dd.y);
+            bool en;
+            if (lat_int) {
+                en = nli_eval_g<BITS>(prog, kp.off_nli, arg, lat, d, x, y, z) == p + 1;
+            } else {
+                en = true;
+                for (int k = 0; k < nc && en; ++k) {
+                    const int cw = cd[k];
+                    en = lat.get(site0(d, x, y, z, (cw & 63) - 32, ((cw >> 6) & 63) - 32,
+                                       ((cw >> 12) & 63) - 32, (cw >> 18) & 63)) == ((cw >> 24) & 0xFF);
+                }
+            }
+            const bool reg = bp[kp.spuck * c + n_exec] != 0;
+            nbad += en != reg ? 1 : 0;
+        }
+    }
+    if (nbad)
+        atomicAdd(&bad[slot], nbad);
+}
+
 // get_occupation (proclist_generic_subroutines.mpy:113-158): counts per
 // (species, site type); out[slot][S][spuck] as fractions of the cell count.
 template <int BITS>
@@ -2717,6 +2788,41 @@ int kmcg_init(kmcg_batch *b, int32_t r0, int32_t n, const uint32_t *seeds, const
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(b->stream));
     return set_lists_ok(b, r0, n, 1);
+}
+
+int kmcg_check_lists(kmcg_batch *b, int32_t r0, int32_t n, int32_t *bad)
+{
+    if (!b || r0 < 0 || n < 1 || r0 + n > b->n || !bad)
+        return fail(KMCG_EINVAL, "bad replica range");
+    int rc = settle(b);
+    if (rc)
+        return rc;
+    std::vector<int> ids(n);
+    for (int i = 0; i < n; ++i)
+        ids[i] = r0 + i;
+    int *dbad = nullptr;
+    if (hipMalloc((void **)&dbad, (size_t)n * sizeof(int)) != hipSuccess)
+        return fail(KMCG_ENOMEM, "check buffer");
+    KParams kp = make_params(b);
+    kp.arg0 = b->m->off_init;
+    const dim3 grid((unsigned)(n * b->m->P));
+    hipError_t e = hipMemcpyAsync(b->rep_ids, ids.data(), n * sizeof(int), hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess)
+        e = hipMemsetAsync(dbad, 0, (size_t)n * sizeof(int), b->stream);
+    if (e == hipSuccess) {
+        if (b->bits == 2)
+            hipLaunchKernelGGL((kmc_check_lists_kernel<2>), grid, dim3(256), 0, b->stream, kp, dbad);
+        else
+            hipLaunchKernelGGL((kmc_check_lists_kernel<8>), grid, dim3(256), 0, b->stream, kp, dbad);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(bad, dbad, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(b->stream);
+    (void)hipFree(dbad);
+    HIPCHK(e);
+    return KMCG_OK;
 }
 
 int kmcg_set_rates(kmcg_batch *b, int32_t r0, int32_t n, const double *rates)

@@ -124,6 +124,15 @@ class Batch:
     def set_launch(self, lattice_in_lds: int = -1, waves_per_group: int = 0):
         _lib.check(_lib.load().kmcg_set_launch(self.h, lattice_in_lds, waves_per_group))
 
+    def check_lists(self, r0: int = 0, n: Optional[int] = None) -> np.ndarray:
+        """Violations per replica of the list invariant and of the
+        registration against the conditions (kmcg_check_lists); all 0 for a
+        consistent state."""
+        n = self.n - r0 if n is None else n
+        out = np.zeros(n, np.int32)
+        _lib.check(_lib.load().kmcg_check_lists(self.h, r0, n, _ptr(out)))
+        return out
+
     def set_graph(self, enable: bool = True):
         """Replay runs through captured HIP graphs (kmcg_set_graph)."""
         _lib.check(_lib.load().kmcg_set_graph(self.h, 1 if enable else 0))
