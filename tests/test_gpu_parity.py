@@ -294,3 +294,42 @@ def test_kmc_time_within_one_ulp_per_step(engine, oracle, factory, backend):
             worst = max(worst, err / (k + 1))
             assert err <= k + 1, "%s r%d step %d: %g ulp" % (backend, r, k + 1, err)
     print("kmc_time: worst %.3f ulp per step" % worst, flush=True)
+
+
+LONG = [
+    # (name, project factory, backend, size, steps): workload sizes, long runs
+    ("zgb_128x128", W.zgb_project, "local_smart", (128, 128), 100_000),
+    ("ruo2_64x64", W.ruo2_project, "local_smart", (64, 64), 200_000),
+    ("ruo2_lat_int_64x64", W.ruo2_project, "lat_int", (64, 64), 200_000),
+    ("pairwise_256x256", W.pairwise_project, "lat_int", (256, 256), 200_000),
+]
+
+
+@pytest.mark.parametrize("name,factory,backend,size,steps", LONG, ids=[c[0] for c in LONG])
+def test_long_trajectories_match_oracle(engine, oracle, name, factory, backend, size, steps):
+    """Whole-workload lattice sizes and long runs: two replicas per case
+    (the workload's rate draw for the first, random rates and a random
+    starting lattice for the second), bit-exact against the oracle at the
+    end."""
+    print("case", name, flush=True)
+    pt = factory()
+    cm = compile_model(pt, backend)
+    g = np.random.default_rng(len(name) + steps)
+    P = cm.nr_of_proc
+    wl = {"zgb_128x128": lambda: W.zgb(n=2), "ruo2_64x64": lambda: W.ruo2(n=2),
+          "ruo2_lat_int_64x64": lambda: W.ruo2(n=2), "pairwise_256x256": lambda: W.pairwise(n=2)}[name]()
+    rates = np.array([wl.rates[0], 10.0 ** g.uniform(-1, 3, P)])
+    m = engine.Model(cm)
+    b = engine.Batch(m, [list(size) + [1]] * 2)
+    seeds = np.array([101, 202], np.uint32)
+    b.init(seeds, np.zeros(2, np.uint32))
+    b.set_rates(rates)
+    V = int(b.volumes[0])
+    configs = [np.full(V, cm.species_names.index(pt.default_species), np.int8), _random_config(pt, V, g)]
+    b.set_config(configs)
+    b.run(steps)
+    for r in range(2):
+        o = oracle_replica(oracle, cm, size, int(seeds[r]), 0, rates[r], configs[r])
+        o.run(steps)
+        compare_state(b, r, o, where="%s r%d after %d steps" % (name, r, steps))
+    b.close()
