@@ -44,3 +44,16 @@ def test_tampered_program_is_rejected(tmp_path):
     np.save(os.path.join(d, "kmcg_model.npy"), blob)
     with pytest.raises(RuntimeError):
         compiled_for_export(d)
+
+
+def test_multi_layer_model_is_rejected_clearly():
+    """Multi-layer models (the reference's pdopd.xml: Pd100 and PdO layers,
+    create/annihilate actions, kmos/io/__init__.py:375-404) are outside the
+    path: compiling one fails with a clear message for both backends."""
+    from kmos_amd.compiler import compile_model
+    from kmos_amd.model import load_project
+    pt = load_project(os.path.join(GOLDEN, "pdopd.xml"))
+    assert len(pt.layers) == 2
+    for backend in ("local_smart", "lat_int"):
+        with pytest.raises(NotImplementedError, match="multi-layer"):
+            compile_model(pt, backend)
