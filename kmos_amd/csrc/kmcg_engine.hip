@@ -1134,6 +1134,10 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
     const unsigned seed = kp.seed[r], stream = kp.stream[r];
     int st = st0;
     long long done = 0;
+    // records (while `track`): one every sample_every steps, counted down
+    // instead of a 64-bit remainder per step
+    int rec_left = kp.sample_every;
+    long long rec_idx = 0;
     // accum_rates stay in registers across steps; `first` is the lowest
     // lane per slot whose term changed (64 = none), everything at the start
     double acc[PPL];
@@ -1231,8 +1235,11 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
             const unsigned long long ch = __ballot(W.nr[k] != nr_before[k]);
             first[k] = ch ? __ffsll((long long)ch) - 1 : 64;
         }
-        if (W.track && done % kp.sample_every == 0)
-            write_record<PPL, BITS>(W, kp, slot, done / kp.sample_every - 1, t, dt_last, ks, ST_OK);
+        if (W.track && --rec_left == 0) {
+            write_record<PPL, BITS>(W, kp, slot, rec_idx, t, dt_last, ks, ST_OK);
+            rec_left = kp.sample_every;
+            rec_idx += 1;
+        }
     }
     if (W.track)
         write_record<PPL, BITS>(W, kp, slot, done == 0 ? 0 : (done - 1) / kp.sample_every, t, dt_last, ks, st);
