@@ -26,7 +26,8 @@ bounded sample of the same workload).  Two more legs ride along:
            on the host cores
   serving  a Poisson stream of requests drawn from the quickstart / ZGB /
            RuO2 workloads through the continuous-batching server: p50/p99
-           request latency and aggregate steps/s
+           request latency and aggregate steps/s at 100 requests/s, and the
+           same at 10 and 1000 requests/s (serving_sweep)
 """
 import argparse
 import json
@@ -338,6 +339,24 @@ def serving_leg(rate, n, seed=1, tune=True):
     return out
 
 
+def serving_sweep(rates=(10.0, 1000.0), seed=2, ipl=4):
+    """The serving stream at other arrival rates (the stream's sweep from 10
+    to 1000 requests/s), each with about five seconds of arrivals (at least
+    40 and at most 400 requests): p50/p99 latency and steps/s per rate."""
+    from kmos_amd import serve
+    out = []
+    for rate in rates:
+        n = int(min(400, max(40, rate * 5)))
+        srv = serve.Server(hbm_budget=200 << 30, intervals_per_launch=ipl)
+        t0 = time.perf_counter()
+        res = srv.serve(serve.poisson_stream(rate=rate, n=n, seed=seed), realtime=True)
+        s = serve.latency_summary(res, time.perf_counter() - t0)
+        srv.close()
+        out.append({"arrival_rate_per_s": rate, "requests": n, "p50_latency_s": s["p50_latency_s"],
+                    "p99_latency_s": s["p99_latency_s"], "steps_per_s": s["steps_per_s"]})
+    return out
+
+
 def calibrate_copy(nbytes=1 << 30, reps=5):
     """A plain device-to-device copy of known size: what FETCH_SIZE and
     WRITE_SIZE read for it calibrates the step kernel's counters."""
@@ -484,6 +503,8 @@ def main():
     if not a.no_serve:
         out["serving"] = serving_leg(a.serve_rate, a.serve_requests, tune=not a.no_tune)
         _progress("serving leg done")
+        out["serving_sweep"] = serving_sweep(ipl=out["serving"]["intervals_per_launch"])
+        _progress("serving sweep done")
         out["p50_latency_s"] = out["serving"]["p50_latency_s"]
         out["p99_latency_s"] = out["serving"]["p99_latency_s"]
     print(json.dumps(out))
