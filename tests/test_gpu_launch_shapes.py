@@ -68,3 +68,24 @@ def test_tuned_launch_matches_default(engine):
     a.close()
     c.close()
     del m
+
+
+def test_lattice_too_large_for_lds_runs_from_hbm(engine, oracle):
+    """A lattice larger than the LDS holds (RuO2 1024 x 512 cells, 2^20
+    sites, 256 KiB packed) makes the automatic launch keep it in HBM; the
+    trajectory is the oracle's."""
+    cm = compile_model(W.ruo2_project(), "local_smart")
+    g = np.random.default_rng(8)
+    rates = 10.0 ** g.uniform(-1, 4, (1, cm.nr_of_proc))
+    m = engine.Model(cm)
+    b = engine.Batch(m, [[1024, 512, 1]])
+    b.init(np.array([77], np.uint32), np.zeros(1, np.uint32))
+    b.set_rates(rates)
+    ids = sorted(cm.project.species_ids().values())
+    config = g.choice(ids, size=int(b.volumes[0])).astype(np.int8)
+    b.set_config([config])
+    b.run(20000)
+    o = oracle_replica(oracle, cm, (1024, 512), 77, 0, rates[0], config)
+    o.run(20000)
+    compare_state(b, 0, o, where="ruo2 1024x512 from HBM")
+    b.close()
