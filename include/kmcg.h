@@ -100,7 +100,9 @@ int kmcg_set_config(kmcg_batch *b, int32_t r0, int32_t n, const int8_t *config);
  * previous run, if any); kmcg_sync waits for it and only then writes the
  * decreased budgets back, so `budget` must stay valid until kmcg_sync (or
  * any other call on the batch, all of which wait for the run first).
- * Batches have their own streams: runs of different batches overlap. */
+ * Batches have their own streams: runs of different batches overlap.
+ * `replicas` names each replica at most once (KMCG_EINVAL otherwise, before
+ * anything is queued). */
 int kmcg_run(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t steps,
              int64_t *budget);
 int kmcg_sync(kmcg_batch *b);

@@ -2879,9 +2879,15 @@ static int run_impl(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t s
 {
     if (!b || !replicas || n < 1 || n > b->n || steps < 0)
         return fail(KMCG_EINVAL, "bad run arguments");
-    for (int i = 0; i < n; ++i)
+    // each replica is one wave's state: a replica named twice would be
+    // stepped by two waves at once
+    std::vector<unsigned char> seen(b->n, 0);
+    for (int i = 0; i < n; ++i) {
         if (replicas[i] < 0 || replicas[i] >= b->n)
             return fail(KMCG_EINVAL, "replica index out of range");
+        if (seen[replicas[i]]++)
+            return fail(KMCG_EINVAL, "replica named twice in one run");
+    }
     int rc = settle(b);      // the staging buffers belong to one run at a time
     if (rc)
         return rc;

@@ -103,6 +103,32 @@ def test_budget_and_absorbing(engine, oracle):
         compare_state(b, r, o, where="budget r%d" % r)
 
 
+def test_bad_replica_lists_are_rejected(engine, oracle):
+    """A replica named twice, an index out of range, an empty list or a
+    negative step count is refused before anything is queued: the batch's
+    state stays as it was, and a valid run afterwards matches the oracle."""
+    from kmos_amd._lib import KmcgError
+    pt = W.zgb_project()
+    cm = compile_model(pt, "local_smart")
+    m = engine.Model(cm)
+    b = engine.Batch(m, [(8, 8, 1)] * 3)
+    b.init(np.array([5, 6, 7], np.uint32))
+    rates = np.tile(np.array([0.5, 1e10, 1e10, 1e10, 1e10, 0.25, 0.25]), (3, 1))
+    b.set_rates(rates)
+    for ids, steps in (([0, 2, 0], 100), ([1, 1], 100), ([0, 3], 100), ([-1], 100), ([], 100),
+                       ([0, 1], -5)):
+        print("bad list", ids, steps, flush=True)
+        with pytest.raises(KmcgError):
+            b.run(steps, replicas=np.array(ids, np.int32))
+    for r in range(3):
+        assert b.scalars(r)[2] == 0
+    b.run(300, replicas=np.array([2, 0, 1], np.int32))
+    for r in range(3):
+        o = oracle_replica(oracle, cm, (8, 8), 5 + r, 0, rates[r])
+        o.run(300)
+        compare_state(b, r, o, where="after rejected lists r%d" % r)
+
+
 @pytest.mark.parametrize("factory,backend", [
     (W.ruo2_project, "local_smart"), (W.ruo2_project, "lat_int"),
     (W.pairwise_project, "lat_int"), (W.zgb_project, "local_smart"),
