@@ -745,7 +745,16 @@ __device__ __forceinline__ double accumulate(const double (&c)[PPL], double (&ac
             // lane 0's term carries the sum of the slots before (shifted-in 0)
             const double ce = (lane == 0) ? carry + c[k] : c[k];
             double a = (lane >= f) ? ce : acc[k];
-            for (int it = max(f, 1); it < lim; ++it)
+            // four passes per loop trip: one branch per four links of the
+            // chain instead of one per link
+            int it = max(f, 1);
+            for (; it + 4 <= lim; it += 4) {
+                a = wave_shr1z(a) + ce;
+                a = wave_shr1z(a) + ce;
+                a = wave_shr1z(a) + ce;
+                a = wave_shr1z(a) + ce;
+            }
+            for (; it < lim; ++it)
                 a = wave_shr1z(a) + ce;
             acc[k] = a;
             dirty = true;
