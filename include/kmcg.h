@@ -76,7 +76,8 @@ void kmcg_batch_free(kmcg_batch *b);
  * one after the other, V values each in site order), empty bookkeeping and
  * the touchups in k,j,i order, clocks, procstat and integ_rates reset, and
  * the uniform stream keyed (seeds[i], streams[i]) (streams may be NULL)
- * (proclist_generic_subroutines.mpy:166-304). */
+ * (proclist_generic_subroutines.mpy:166-304).  A configuration holding an
+ * unknown species fails the call before any replica is touched. */
 int kmcg_init(kmcg_batch *b, int32_t r0, int32_t n, const uint32_t *seeds,
               const uint32_t *streams, const int8_t *configs);
 
@@ -88,7 +89,8 @@ int kmcg_get_rates(kmcg_batch *b, int32_t r, double *rates);
 /* KMC_Model._set_configuration + _adjust_database for replicas
  * [r0, r0+n): species per site (config holds the n lattices one after the
  * other, each V values in site order), then touchups with x outermost
- * (kmos/run/__init__.py:1408-1457). */
+ * (kmos/run/__init__.py:1408-1457).  A configuration holding an unknown
+ * species fails the call before any lattice is replaced. */
 int kmcg_set_config(kmcg_batch *b, int32_t r0, int32_t n, const int8_t *config);
 
 /* proclist.do_kmc_steps(steps) for the listed replicas

@@ -2753,6 +2753,19 @@ int kmcg_init(kmcg_batch *b, int32_t r0, int32_t n, const uint32_t *seeds, const
     int rc0 = settle(b);
     if (rc0)
         return rc0;
+    // every configuration is checked before any replica is touched, so a
+    // rejected call leaves all replicas as they were
+    std::vector<std::vector<unsigned char>> packed;   // alive until the stream is synchronised
+    if (configs) {
+        long long off = 0;
+        packed.resize(n);
+        for (int i = 0; i < n; ++i) {
+            const int rc = pack_config(b, r0 + i, configs + off, packed[i]);
+            if (rc)
+                return rc;
+            off += b->V[r0 + i];
+        }
+    }
     const long long P = b->m->P;
     std::vector<unsigned> st(n, 0u);
     if (streams)
@@ -2775,20 +2788,10 @@ int kmcg_init(kmcg_batch *b, int32_t r0, int32_t n, const uint32_t *seeds, const
         ids[i] = r0 + i;
     HIPCHK(hipMemcpyAsync(b->rep_ids, ids.data(), n * sizeof(int), hipMemcpyHostToDevice, b->stream));
     KParams kp = make_params(b);
-    std::vector<std::vector<unsigned char>> packed;   // alive until the stream is synchronised
     if (configs) {
-        long long off = 0;
-        packed.resize(n);
-        for (int i = 0; i < n; ++i) {
-            int rc = pack_config(b, r0 + i, configs + off, packed[i]);
-            if (rc) {
-                (void)hipStreamSynchronize(b->stream);
-                return rc;
-            }
+        for (int i = 0; i < n; ++i)
             HIPCHK(hipMemcpyAsync(b->lat_pool + b->lat_off[r0 + i], packed[i].data(), packed[i].size(),
                                   hipMemcpyHostToDevice, b->stream));
-            off += b->V[r0 + i];
-        }
     } else if (b->bits == 2) {
         hipLaunchKernelGGL((kmc_fill_default_kernel<2>), dim3(n), dim3(256), 0, b->stream, kp);
     } else {
@@ -2864,19 +2867,23 @@ int kmcg_set_config(kmcg_batch *b, int32_t r0, int32_t n, const int8_t *config)
 {
     if (!b || r0 < 0 || n < 1 || r0 + n > b->n || !config)
         return fail(KMCG_EINVAL, "bad replica range");
+    // every configuration is checked before any lattice is replaced, so a
+    // rejected call leaves all replicas as they were
     long long off = 0;
     std::vector<int> ids(n);
+    std::vector<std::vector<unsigned char>> packed(n);
+    for (int i = 0; i < n; ++i) {
+        const int rc = pack_config(b, r0 + i, config + off, packed[i]);
+        if (rc)
+            return rc;
+        off += b->V[r0 + i];
+        ids[i] = r0 + i;
+    }
     for (int i = 0; i < n; ++i) {
         const int r = r0 + i;
-        std::vector<unsigned char> packed;
-        int rc = pack_config(b, r, config + off, packed);
+        const int rc = copy_in(b, b->lat_pool + b->lat_off[r], packed[i].data(), packed[i].size());
         if (rc)
             return rc;
-        rc = copy_in(b, b->lat_pool + b->lat_off[r], packed.data(), packed.size());
-        if (rc)
-            return rc;
-        off += b->V[r];
-        ids[i] = r;
     }
     int rc = run_misc(b, ids, MODE_ADJUST, 0, 0, 0, 0);
     return rc ? rc : set_lists_ok(b, r0, n, 1);

@@ -129,6 +129,47 @@ def test_bad_replica_lists_are_rejected(engine, oracle):
         compare_state(b, r, o, where="after rejected lists r%d" % r)
 
 
+def test_rejected_configuration_changes_nothing(engine, oracle):
+    """A configuration with an unknown species in its second replica makes
+    set_config and init fail before any replica is touched: every replica
+    keeps its lattice, lists, counters and clock, and keeps stepping like
+    the oracle."""
+    from kmos_amd._lib import KmcgError
+    pt = W.zgb_project()
+    cm = compile_model(pt, "local_smart")
+    m = engine.Model(cm)
+    b = engine.Batch(m, [(8, 8, 1), (9, 7, 1), (8, 8, 1)])
+    seeds = np.array([5, 6, 7], np.uint32)
+    b.init(seeds)
+    rates = np.tile(np.array([0.5, 1e10, 1e10, 1e10, 1e10, 0.25, 0.25]), (3, 1))
+    b.set_rates(rates)
+    b.run(200)
+
+    def state(r):
+        s, bk = b.avail(r)
+        return (b.lattice(r).copy(), s.copy(), bk.copy(), b.nr_of_sites(r).copy(),
+                b.procstat(r).copy(), tuple(b.scalars(r)))
+
+    before = [state(r) for r in range(3)]
+    rng = np.random.default_rng(3)
+    configs = [_random_config(pt, int(v), rng) for v in b.volumes]
+    configs[1][5] = 7
+    print("bad set_config", flush=True)
+    with pytest.raises(KmcgError):
+        b.set_config(configs)
+    print("bad init", flush=True)
+    with pytest.raises(KmcgError):
+        b.init(seeds, None, configs=configs)
+    for r in range(3):
+        for x, y in zip(before[r], state(r)):
+            np.testing.assert_array_equal(np.asarray(x), np.asarray(y), err_msg="replica %d" % r)
+    b.run(300)
+    for r, size in enumerate([(8, 8), (9, 7), (8, 8)]):
+        o = oracle_replica(oracle, cm, size, 5 + r, 0, rates[r])
+        o.run(500)
+        compare_state(b, r, o, where="after rejected configurations r%d" % r)
+
+
 @pytest.mark.parametrize("factory,backend", [
     (W.ruo2_project, "local_smart"), (W.ruo2_project, "lat_int"),
     (W.pairwise_project, "lat_int"), (W.zgb_project, "local_smart"),
