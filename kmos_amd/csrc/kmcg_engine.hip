@@ -3604,6 +3604,8 @@ int kmcg_acf_set_properties(kmcg_batch *b, int32_t r0, int32_t n, const int32_t 
     if (r0 < 0 || n < 0 || r0 + n > b->n || (n > 0 && !property_acf))
         return fail(KMCG_EINVAL, "bad replica range");
     const int nt1 = b->acf_nt1;
+    // everything is checked before anything is copied, so a rejected call
+    // changes nothing
     if (product_property) {
         // sum(buffer_acf) is kept as a running sum: exact only for integers
         for (int i = 0; i < nt1 * nt1; ++i) {
@@ -3611,9 +3613,6 @@ int kmcg_acf_set_properties(kmcg_batch *b, int32_t r0, int32_t n, const int32_t 
             if (!(std::fabs(v) <= 1048576.0) || v != std::floor(v))
                 return fail(KMCG_EUNSUPPORTED, "acf: product properties must be integers of magnitude <= 2^20");
         }
-        int rc = copy_in(b, b->acf_pp, product_property, (size_t)nt1 * nt1);
-        if (rc)
-            return rc;
     }
     long long off = 0;
     for (int r = r0; r < r0 + n; ++r) {
@@ -3621,7 +3620,17 @@ int kmcg_acf_set_properties(kmcg_batch *b, int32_t r0, int32_t n, const int32_t 
         for (long long i = 0; i < V1; ++i)
             if (property_acf[off + i] < 0 || property_acf[off + i] >= nt1)
                 return fail(KMCG_EINVAL, "acf: property type out of range");
-        int rc = copy_in(b, b->acf_pa + b->msd_off[r], property_acf + off, (size_t)V1);
+        off += V1;
+    }
+    if (product_property) {
+        const int rc = copy_in(b, b->acf_pp, product_property, (size_t)nt1 * nt1);
+        if (rc)
+            return rc;
+    }
+    off = 0;
+    for (int r = r0; r < r0 + n; ++r) {
+        const long long V1 = b->V[r] + 1;
+        const int rc = copy_in(b, b->acf_pa + b->msd_off[r], property_acf + off, (size_t)V1);
         if (rc)
             return rc;
         off += V1;

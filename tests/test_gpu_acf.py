@@ -163,6 +163,14 @@ def test_acf_matches_oracle(engine, oracle, with_adsorption):
     pp = np.zeros((3, 3))
     pp[1:, 1:] = np.outer([2.0, -1.0], [2.0, -1.0])
     b.acf_set_properties(pas, pp)
+    # a call rejected for a type out of range in its last replica changes
+    # neither the products nor the types of the replicas before it: the
+    # comparisons below use pas and pp
+    from kmos_amd._lib import KmcgError
+    bad = [np.where(pa > 0, 3 - pa, 0).astype(np.int32) for pa in pas]
+    bad[-1][1] = 5
+    with pytest.raises(KmcgError):
+        b.acf_set_properties(bad, 3.0 * pp)
     b.acf_init(ion, hops)
     b.msd_trajectory(3000)
     orcs = []
