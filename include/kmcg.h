@@ -112,6 +112,14 @@ int kmcg_sync(kmcg_batch *b);
  * no size limit), -1 = automatic; replica-waves per workgroup sharing one
  * copy of the op program (1..4, 0 = as many as LDS allows). */
 int kmcg_set_launch(kmcg_batch *b, int32_t lattice_in_lds, int32_t waves_per_group);
+/* Step kernel form for kmcg_run with one process slot per lane (P <= 64)
+ * and no tracking: 1 = the next step's accumulate and search run while the
+ * replay's HBM reads are in flight, 0 = after them, -1 = automatic (on for
+ * local_smart models of more than 16 processes in runs of at most 8
+ * replicas per compute unit).  Both forms
+ * give identical trajectories.  No counterpart in the reference (a launch
+ * setting of the batched engine). */
+int kmcg_set_overlap(kmcg_batch *b, int32_t mode);
 /* Runs through HIP graphs (1) or direct launches (0, the default).  With
  * graphs, the stream work of a kmcg_run / kmcg_run_recorded (replica list
  * and budget copies, the step kernel, copies back) is captured once per

@@ -32,6 +32,7 @@ SIGNATURES = [
     ("kmcg_run", _i32, [_vp, _vp, _i32, _i64, _vp]),
     ("kmcg_sync", _i32, [_vp]),
     ("kmcg_set_launch", _i32, [_vp, _i32, _i32]),
+    ("kmcg_set_overlap", _i32, [_vp, _i32]),
     ("kmcg_set_graph", _i32, [_vp, _i32]),
     ("kmcg_check_lists", _i32, [_vp, _i32, _i32, _vp]),
     ("kmcg_last_run_ms", _i32, [_vp, _vp]),

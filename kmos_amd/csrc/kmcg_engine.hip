@@ -155,9 +155,18 @@ struct Rows {
 // w: this lane's window in LDS, slot q at w[q - (nr0 - RB + 1)].  RB (1, 2
 // or RBLK) bounds the entries any lane replays in this block: the reads,
 // the window and the pairwise tests are set up for RB entries only.
-template <int PPL, int BITS, int RB>
+// Work to run while a replay block's reads are on their way (wave-uniform,
+// every lane active); the default does nothing.
+struct NoMid {
+    static constexpr bool active = false;
+    bool armed;
+    template <class Wv>
+    __device__ __forceinline__ void operator()(const Wv &, int) {}
+};
+
+template <int PPL, int BITS, int RB, class Mid>
 __device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, const Rows &L, int *w,
-                                             const int (&v)[RBLK], int m, int &err)
+                                             const int (&v)[RBLK], int m, int &err, Mid &mid)
 {
     const int nr0 = nr;
     // per entry: deletion?, site; pairwise equal sites, computed once
@@ -181,6 +190,15 @@ __device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, const 
 #pragma unroll
     for (int j = 0; j < RB; ++j)
         tv[j] = L.s((j < nd && nr0 - j >= 1) ? nr0 - j : 0);
+    if (Mid::active && mid.armed) {
+        // the count this lane's block leaves when every deletion finds its
+        // site registered
+        int dn = 0;
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+            dn += (i < m) ? (isdel[i] ? -1 : 1) : 0;
+        mid(W, nr0 + dn);
+    }
     bool same[RB][RB];     // same[i][j], i < j: entry j deletes entry i's site
 #pragma unroll
     for (int i = 0; i < RB; ++i)
@@ -250,8 +268,8 @@ __device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, const 
 // Apply the queued entries of the step: each owner lane replays its
 // processes' queues in order, RBLK entries per batch of HBM reads, and
 // empties them.
-template <int PPL, int BITS, bool RB2 = false>
-__device__ __forceinline__ int apply_queues(Wave<PPL, BITS> &W)
+template <int PPL, int BITS, bool RB2 = false, class Mid = NoMid>
+__device__ __forceinline__ int apply_queues(Wave<PPL, BITS> &W, Mid &mid)
 {
     int err = 0;
 #pragma unroll
@@ -277,21 +295,26 @@ __device__ __forceinline__ int apply_queues(Wave<PPL, BITS> &W)
 #pragma unroll
             for (int i = 0; i < RBLK; ++i)
                 v[i] = (i < m) ? v[i] : 0;
-            if (m > 0) {
+            // with an active hook (one process slot per lane) every lane
+            // takes part in the block; a lane without entries reads the
+            // table's first word only
+            mid.armed = Mid::active && i0 == 0 && !__ballot(n > RBLK);
+            if (Mid::active || m > 0) {
+                const bool any = m > 0;
                 int nrk = W.nr[k];
                 Rows L;
                 L.base = (char *)W.avail;
-                L.so = (unsigned)((long long)p * W.V1) << 2;
-                L.bo = (unsigned)((long long)(W.P + p) * W.V1) << 2;
+                L.so = any ? (unsigned)((long long)p * W.V1) << 2 : 0u;
+                L.bo = any ? (unsigned)((long long)(W.P + p) * W.V1) << 2 : 0u;
                 // blocks of one entry per lane are the common case; a
                 // two-entry form as well (RB2) costs code size, so only the
                 // plain step loop takes it
                 if (__ballot(m > 2) || (!RB2 && __ballot(m > 1)))
-                    replay_block<PPL, BITS, RBLK>(W, nrk, L, W.win + WIN * W.lane, v, m, err);
+                    replay_block<PPL, BITS, RBLK>(W, nrk, L, W.win + WIN * W.lane, v, max(m, 0), err, mid);
                 else if (RB2 && __ballot(m > 1))
-                    replay_block<PPL, BITS, 2>(W, nrk, L, W.win + WIN * W.lane, v, m, err);
+                    replay_block<PPL, BITS, 2>(W, nrk, L, W.win + WIN * W.lane, v, max(m, 0), err, mid);
                 else
-                    replay_block<PPL, BITS, 1>(W, nrk, L, W.win + WIN * W.lane, v, m, err);
+                    replay_block<PPL, BITS, 1>(W, nrk, L, W.win + WIN * W.lane, v, max(m, 0), err, mid);
                 W.nr[k] = nrk;
             }
         }
@@ -634,7 +657,8 @@ __device__ int run_segment(Wave<PPL, BITS> &W, int seg, int bx, int by, int bz)
     SegPre q;
     seg_prefetch(W.prog, W.off_seg, seg, W.lane, q);
     const int e = post_segment<PPL, BITS>(W, q, bx, by, bz);
-    const int a = apply_queues<PPL, BITS>(W);
+    NoMid none;
+    const int a = apply_queues<PPL, BITS>(W, none);
     return e ? e : a;
 }
 
@@ -658,8 +682,8 @@ __device__ __forceinline__ void proc_prefetch(const int *prog, int off_proc, int
 // `merged`, all segments of the process are queued and applied once, else
 // each segment is applied before the next is posted.  The first segment's
 // static words come read ahead in `pre`.
-template <int PPL, int BITS, bool RB2 = false>
-__device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, bool merged)
+template <int PPL, int BITS, bool RB2 = false, class Mid = NoMid>
+__device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, bool merged, Mid &mid)
 {
     const int s0 = site - 1;
     const int cell = divm(s0, W.ms);
@@ -685,7 +709,7 @@ __device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, bool m
             // near the faces the segments run one by one
             if (W.d.interior(bx1, by1, bz1) && W.d.interior(bx2, by2, bz2) &&
                 post_pair<PPL, BITS, true>(W, cur, q2, bx1, by1, bz1, bx2, by2, bz2) >= 0)
-                return apply_queues<PPL, BITS, RB2>(W);
+                return apply_queues<PPL, BITS, RB2>(W, mid);
         }
     }
     for (int i = 0; i < nit; ++i, it += IT_W) {
@@ -699,12 +723,13 @@ __device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, bool m
         if (e)
             break;
         if (!merged) {
-            e = apply_queues<PPL, BITS, RB2>(W);
+            NoMid none;
+            e = apply_queues<PPL, BITS, RB2>(W, none);
             if (e)
                 return e;
         }
     }
-    const int a = merged ? apply_queues<PPL, BITS, RB2>(W) : ST_OK;
+    const int a = merged ? apply_queues<PPL, BITS, RB2>(W, mid) : ST_OK;
     return e ? e : a;
 }
 
@@ -791,6 +816,42 @@ __device__ __forceinline__ int interval_search(const double (&acc)[PPL], double 
     }
     return pick;
 }
+
+// The next step's accumulate and search, run as the replay's hook in the
+// plain step loop (one process slot per lane) while the replay's HBM reads
+// are on their way.  From the count each lane's process will have if every
+// deletion of the block finds its site registered, it updates this lane's
+// prefix in place and gives the next step's total and pick; the caller
+// keeps them only when the replay left exactly those counts, and otherwise
+// recomputes every prefix.
+struct NextSel {
+    static constexpr bool active = true;
+    bool armed = false;
+    bool on;             // the next step's u_proc is at hand
+    double u1;           // the next step's u_proc
+    double &acc;         // this lane's prefix (PPL == 1)
+    int nrs = 0;         // predicted count of this lane's process
+    double total = 0.0;
+    int pick = 0;
+    bool made = false;   // prefixes were changed
+    __device__ NextSel(bool on_, double u1_, double &acc_) : on(on_), u1(u1_), acc(acc_) {}
+    template <class Wv>
+    __device__ __forceinline__ void operator()(const Wv &W, int nr_after)
+    {
+        nrs = nr_after;
+        if (!on)
+            return;
+        const double c[1] = {(double)nr_after * W.rate[0]};
+        const unsigned long long ch = __ballot(nr_after != W.nr[0]);
+        const int fs[1] = {ch ? __ffsll((long long)ch) - 1 : 64};
+        double a[1] = {acc};
+        total = accumulate<1>(c, a, W.P, W.lane, fs);
+        acc = a[0];
+        if (total > 0.0)
+            pick = interval_search<1>(a, u1 * total, W.P, W.lane);
+        made = true;
+    }
+};
 
 // Copy the op program to LDS (whole workgroup), then set up this wave.
 // Returns the replica index, or -1 for a wave without one.
@@ -1105,7 +1166,11 @@ __device__ __forceinline__ void acf_after(const KParams &kp, int r)
     }
 }
 
-template <int PPL, int BITS, bool LDS_LAT, bool MSD>
+// OV (one process slot per lane, no tracking): the next step's accumulate
+// and search run during the replay's HBM reads.  It shortens a trajectory's
+// chain of dependent steps and adds instructions, so it pays where few
+// replica-waves share a SIMD (latency-bound) and not where many do.
+template <int PPL, int BITS, bool LDS_LAT, bool MSD, bool OV = false>
 __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4))) void kmc_steps_kernel(KParams kp)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1168,6 +1233,11 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
         lu1 = uu[1];
         lu2 = uu[2];
     }
+    // the next step's total and pick, when the replay computed them (one
+    // process slot per lane, no tracking)
+    bool have = false;
+    double s_total = 0.0;
+    int s_pick = 0;
     while (done < n) {
         if (uj == 64) {
             ubase += 64;
@@ -1186,7 +1256,7 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
 #pragma unroll
         for (int k = 0; k < PPL; ++k)
             c[k] = (double)W.nr[k] * W.rate[k];
-        const double total = accumulate<PPL>(c, acc, P, lane, first);
+        const double total = have ? s_total : accumulate<PPL>(c, acc, P, lane, first);
         if (!(total > 0.0)) {
             st = ST_ABSORBING;
             break;
@@ -1202,7 +1272,7 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
         for (int k = 0; k < PPL; ++k)
             W.integ[k] = W.integ[k] + c[k] * dt;
         // determine_procsite (base.mpy:1075-1120)
-        const int pick = interval_search<PPL>(acc, u[1] * total, P, lane);
+        const int pick = have ? s_pick : interval_search<PPL>(acc, u[1] * total, P, lane);
         const int pk = pick >> 6, pl = pick & 63;
         const int nrp = readlane_i(slot_get<PPL>(W.nr, pk), pl);
         int kk = (int)(1.0 + u[2] * (double)nrp);
@@ -1234,7 +1304,18 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
 #pragma unroll
         for (int k = 0; k < PPL; ++k)
             nr_before[k] = W.nr[k];
-        st = run_proc<PPL, BITS, !MSD>(W, pre, site, kp.merged);
+        bool redo = false;
+        if constexpr (OV && PPL == 1 && !MSD) {
+            NextSel nx(uj < 64, readlane_d(lu1, uj & 63), acc[0]);
+            st = run_proc<PPL, BITS, true>(W, pre, site, kp.merged, nx);
+            have = nx.made && !st && !__ballot(W.nr[0] != nx.nrs);
+            redo = nx.made && !have;
+            s_total = nx.total;
+            s_pick = nx.pick;
+        } else {
+            NoMid none;
+            st = run_proc<PPL, BITS, !MSD>(W, pre, site, kp.merged, none);
+        }
         if (MSD && kp.acf_pa && lane == 0)
             acf_after(kp, r);
         if (st)
@@ -1242,7 +1323,8 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
 #pragma unroll
         for (int k = 0; k < PPL; ++k) {
             const unsigned long long ch = __ballot(W.nr[k] != nr_before[k]);
-            first[k] = ch ? __ffsll((long long)ch) - 1 : 64;
+            // prefixes the replay's hook changed on a wrong guess: all anew
+            first[k] = redo ? 0 : ch ? __ffsll((long long)ch) - 1 : 64;
         }
         if (W.track && --rec_left == 0) {
             write_record<PPL, BITS>(W, kp, slot, rec_idx, t, dt_last, ks, ST_OK);
@@ -1307,7 +1389,8 @@ __global__ __launch_bounds__(64 * MAX_WPG) void kmc_misc_kernel(KParams kp)
         }
         ProcPre pre;
         proc_prefetch(W.prog, W.off_proc, W.off_seg, proc, lane, pre);
-        st = run_proc<PPL, BITS>(W, pre, site, kp.merged);
+        NoMid none;
+        st = run_proc<PPL, BITS>(W, pre, site, kp.merged, none);
         changed = true;
     } else if (kp.mode == MODE_NEXT_STEP) {
         double c[PPL], acc[PPL];
@@ -1668,6 +1751,8 @@ struct kmcg_batch {
     int bits = 8;
     int lds_mode = -1;
     int max_wpg = MAX_WPG;
+    int ov_mode = -1;      // overlapped step kernel: -1 automatic, 0 off, 1 on
+    int n_cu = 256;        // compute units of the batch's device
     int max_lds = 65536;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -2325,6 +2410,7 @@ KParams make_params(kmcg_batch *b)
 // replica-waves per workgroup (sharing one program copy) as LDS allows.
 struct Shape {
     bool lds_lat;
+    bool ov;           // the step kernel's overlapped form
     int wpg, grid;
     size_t lds;
 };
@@ -2341,6 +2427,12 @@ Shape launch_shape(kmcg_batch *b, const int *ids, int n, KParams &kp, int max_wp
     s.lds_lat = b->lds_mode == 0 ? false : (prog + (size_t)maxlb + mbox <= (size_t)b->max_lds);
     const size_t per_wave = (s.lds_lat ? (size_t)maxlb : 0) + mbox;
     s.wpg = std::max(1, std::min(max_wpg, std::min(MAX_WPG, n)));
+    // automatic: local_smart runs with at most two replica-waves per SIMD
+    // (four SIMDs per CU) and a prefix chain long enough to hide (more than
+    // 16 processes; RuO2's 36 gain, ZGB's 7 lose); lat_int steps mostly
+    // queue more than one replay block, where the overlap does not engage
+    s.ov = b->ov_mode == 1 ||
+           (b->ov_mode == -1 && b->m->backend == 0 && b->m->P > 16 && (long long)n <= 8LL * b->n_cu);
     while (s.wpg > 1 && prog + s.wpg * per_wave > (size_t)b->max_lds)
         s.wpg -= 1;
     s.grid = (n + s.wpg - 1) / s.wpg;
@@ -2361,6 +2453,11 @@ hipError_t launch_t(bool steps, const Shape &s, hipStream_t st, const KParams &k
             hipLaunchKernelGGL((kmc_steps_kernel<PPL, BITS, true, true>), grid, block, s.lds, st, kp);
         else
             hipLaunchKernelGGL((kmc_steps_kernel<PPL, BITS, false, true>), grid, block, s.lds, st, kp);
+    } else if (steps && PPL == 1 && s.ov) {
+        if (s.lds_lat)
+            hipLaunchKernelGGL((kmc_steps_kernel<1, BITS, true, false, true>), grid, block, s.lds, st, kp);
+        else
+            hipLaunchKernelGGL((kmc_steps_kernel<1, BITS, false, false, true>), grid, block, s.lds, st, kp);
     } else if (steps) {
         if (s.lds_lat)
             hipLaunchKernelGGL((kmc_steps_kernel<PPL, BITS, true, false>), grid, block, s.lds, st, kp);
@@ -2640,8 +2737,10 @@ int kmcg_batch_create(const kmcg_model *m, int32_t n, const int32_t *sizes, int3
     if (hipSetDevice(device) != hipSuccess)
         return bail(fail(KMCG_EHIP, "hipSetDevice failed"));
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess)
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
         b->max_lds = (int)prop.sharedMemPerBlock;
+        b->n_cu = std::max(1, prop.multiProcessorCount);
+    }
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(KMCG_EHIP, "stream creation failed"));
     if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
@@ -2706,6 +2805,14 @@ int kmcg_set_graph(kmcg_batch *b, int32_t enable)
     b->use_graph = enable != 0;
     if (!b->use_graph)
         drop_graphs(b);
+    return KMCG_OK;
+}
+
+int kmcg_set_overlap(kmcg_batch *b, int32_t mode)
+{
+    if (!b || mode < -1 || mode > 1)
+        return fail(KMCG_EINVAL, "bad argument");
+    b->ov_mode = mode;
     return KMCG_OK;
 }
 

@@ -124,6 +124,10 @@ class Batch:
     def set_launch(self, lattice_in_lds: int = -1, waves_per_group: int = 0):
         _lib.check(_lib.load().kmcg_set_launch(self.h, lattice_in_lds, waves_per_group))
 
+    def set_overlap(self, mode: int = -1):
+        """Step kernel form: 1 overlapped select, 0 plain, -1 automatic."""
+        _lib.check(_lib.load().kmcg_set_overlap(self.h, int(mode)))
+
     def check_lists(self, r0: int = 0, n: Optional[int] = None) -> np.ndarray:
         """Violations per replica of the list invariant and of the
         registration against the conditions (kmcg_check_lists); all 0 for a

@@ -36,17 +36,21 @@ def test_every_launch_shape_gives_the_same_trajectories(engine, oracle, backend)
     b, rates, seeds, configs = _setup(engine, cm, 5)
     orcs = [oracle_replica(oracle, cm, s, int(seeds[i]), 0, rates[i], configs[i]) for i, s in enumerate(SIZES)]
     done = 0
-    for shape in autotune.SHAPES:
-        for order in autotune.ORDERS:
-            print("case", backend, shape, order, flush=True)
-            b.set_launch(*shape)
-            ids = autotune.replica_order(b.volumes, order)
-            b.run(150, ids)
-            for o in orcs:
-                o.run(150)
-            done += 150
-            for r, o in enumerate(orcs):
-                compare_state(b, r, o, where="%s %s %s r%d after %d" % (backend, shape, order, r, done))
+    # both step kernel forms (select after the replay, or overlapped with
+    # its reads) on every shape and order
+    for ov in (0, 1):
+        b.set_overlap(ov)
+        for shape in autotune.SHAPES:
+            for order in autotune.ORDERS:
+                print("case", backend, "overlap", ov, shape, order, flush=True)
+                b.set_launch(*shape)
+                ids = autotune.replica_order(b.volumes, order)
+                b.run(150, ids)
+                for o in orcs:
+                    o.run(150)
+                done += 150
+                for r, o in enumerate(orcs):
+                    compare_state(b, r, o, where="%s ov%d %s %s r%d after %d" % (backend, ov, shape, order, r, done))
     b.close()
 
 
