@@ -3088,7 +3088,7 @@ static int run_impl(kmcg_batch *b, const int32_t *replicas, int32_t n, int64_t s
         HIPCHK(enqueue(true));
     } else {
         // the key: every value a captured node holds
-        const int flags[6] = {n, budget ? 1 : 0, records ? 1 : 0, s.lds_lat ? 1 : 0, s.wpg, s.grid};
+        const int flags[7] = {n, budget ? 1 : 0, records ? 1 : 0, s.lds_lat ? 1 : 0, s.wpg, s.grid, s.ov ? 1 : 0};
         const size_t lds = s.lds;
         const void *hs = b->h_srec;
         std::vector<unsigned char> key(sizeof kp + sizeof flags + sizeof lds + sizeof hs + sizeof cnt);

@@ -86,6 +86,28 @@ class Batch:
     def __del__(self):
         self.close()
 
+    def _species_array(self, configs, r0: int, n: int) -> np.ndarray:
+        """Lattices of replicas r0..r0+n-1 as one int8 array, after checking
+        that every entry is a species id 0..S-1 (before the cast, so that
+        an out-of-range id cannot wrap into a valid one) and that the
+        sizes match the replicas' volumes."""
+        if r0 < 0 or n < 0 or r0 + n > self.n:
+            raise ValueError("replicas %d..%d outside the batch of %d" % (r0, r0 + n - 1, self.n))
+        if n != len(configs):
+            raise ValueError("expected %d lattices, got %d" % (n, len(configs)))
+        parts = [np.asarray(c).reshape(-1) for c in configs]
+        flat = np.concatenate(parts) if parts else np.zeros(0, np.int64)
+        if flat.size and not np.issubdtype(flat.dtype, np.integer):
+            raise ValueError("lattices must hold integer species ids")
+        for i, c in enumerate(parts):
+            if c.size != int(self.volumes[r0 + i]):
+                raise ValueError("lattice of replica %d has %d sites, expected %d"
+                                 % (r0 + i, c.size, int(self.volumes[r0 + i])))
+        if flat.size and (flat.min() < 0 or flat.max() >= self.S):
+            raise ValueError("species ids must lie in 0..%d, got %d..%d"
+                             % (self.S - 1, int(flat.min()), int(flat.max())))
+        return np.ascontiguousarray(flat, dtype=np.int8)
+
     # ---- set-up -----------------------------------------------------------
     def init(self, seeds, streams=None, r0: int = 0, configs=None):
         """initialize_state for replicas r0.. (one per seed), from the
@@ -97,13 +119,13 @@ class Batch:
         if configs is not None:
             if isinstance(configs, np.ndarray) and configs.ndim == 1:
                 configs = [configs]
-            cf = np.ascontiguousarray(np.concatenate([np.asarray(c).reshape(-1) for c in configs]), dtype=np.int8)
-            assert cf.size == int(self.volumes[r0:r0 + n].sum())
+            cf = self._species_array(configs, r0, n)
         _lib.check(_lib.load().kmcg_init(self.h, r0, n, _ptr(seeds), _ptr(st), _ptr(cf)))
 
     def set_rates(self, rates, r0: int = 0):
         r = np.ascontiguousarray(np.atleast_2d(rates), dtype=np.float64)
-        assert r.shape[1] == self.P
+        if r.ndim != 2 or r.shape[1] != self.P:
+            raise ValueError("rates: expected [n][%d] values, got shape %s" % (self.P, r.shape))
         _lib.check(_lib.load().kmcg_set_rates(self.h, r0, r.shape[0], _ptr(r)))
 
     def get_rates(self, r: int) -> np.ndarray:
@@ -116,9 +138,8 @@ class Batch:
         or a single array for replica r0) and rebuild their bookkeeping."""
         if isinstance(configs, np.ndarray) and configs.ndim == 1:
             configs = [configs]
-        flat = np.concatenate([np.asarray(c).reshape(-1) for c in configs]).astype(np.int8)
         n = len(configs)
-        assert flat.size == int(self.volumes[r0:r0 + n].sum())
+        flat = self._species_array(configs, r0, n)
         _lib.check(_lib.load().kmcg_set_config(self.h, r0, n, _ptr(flat)))
 
     def set_launch(self, lattice_in_lds: int = -1, waves_per_group: int = 0):
@@ -256,7 +277,11 @@ class Batch:
             if a.shape != shape:
                 raise ValueError("state[%r] has shape %s, expected %s" % (key, a.shape, shape))
             return a
-        lat = arr("lattice", np.int8, (V,))
+        lat = np.asarray(state["lattice"])
+        if lat.shape != (V,) or not np.issubdtype(lat.dtype, np.integer) or \
+                (V and (lat.min() < 0 or lat.max() >= self.S)):
+            raise ValueError("state['lattice'] must hold %d species ids in 0..%d" % (V, self.S - 1))
+        lat = np.ascontiguousarray(lat, dtype=np.int8)
         nr = arr("nr_of_sites", np.int32, (P,))
         ps = arr("procstat", np.int64, (P,))
         sites = arr("avail_sites", np.int32, (P, V))

@@ -319,7 +319,7 @@ class KMC_Model:
         config = np.asarray(config)
         if config.shape != (X, Y, Z, N):
             raise ValueError("config shape %s does not match model shape %s" % (config.shape, [X, Y, Z, N]))
-        flat = config.transpose(2, 1, 0, 3).reshape(-1).astype(np.int8)
+        flat = config.transpose(2, 1, 0, 3).reshape(-1)
         self._kmc._b().set_config(flat)
 
     def _adjust_database(self):

@@ -154,7 +154,7 @@ class Server:
         P = self.compiled[req.model].nr_of_proc
         rates = np.asarray(req.rates, np.float64).reshape(1, P)
         b.init([req.seed & 0xFFFFFFFF], [0], r0=slot,
-               configs=None if req.config is None else [np.asarray(req.config, np.int8)])
+               configs=None if req.config is None else [np.asarray(req.config)])
         b.set_rates(rates, r0=slot)
         page.active[slot] = _Flight(req, now)
         return True

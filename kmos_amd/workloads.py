@@ -6,8 +6,8 @@ quickstart  one replica, 20x20, 1 site/cell, CO adsorption/desorption at
 zgb         4096 replicas of the Ziff-Gulari-Barshad model on 128x128,
             y ~ U[0.35, 0.60] per replica, 1e5 steps, starting empty.
 ruo2        2048 replicas of the RuO2(110) CO-oxidation model (2 sites/cell,
-            36 processes; tests/golden/ruo2_default.xml, the reference's
-            export fixture), sizes 20^2 : 64^2 : 128^2 = 1 : 2 : 1, each rate
+            36 processes; kmos_amd/data/ruo2_default.xml, the model of the
+            reference's export fixtures), sizes 20^2 : 64^2 : 128^2 = 1 : 2 : 1, each rate
             log-uniform in [1e-2, 1e9], initial coverage 0.3 CO / 0.3 O,
             2e5 steps.
 pairwise    512 replicas of a 256x256 lattice gas with nearest-neighbour
@@ -32,7 +32,8 @@ from .model import CondAct, Coord, Layer, Process, Project, Site, load_project
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
-GOLDEN = os.path.join(REPO, "tests", "golden")
+DATA = os.path.join(HERE, "data")
+RUO2_XML = os.path.join(DATA, "ruo2_default.xml")
 
 
 # ---------------------------------------------------------------------------
@@ -81,7 +82,7 @@ def zgb_project() -> Project:
 
 
 def ruo2_project() -> Project:
-    return load_project(os.path.join(GOLDEN, "ruo2_default.xml"))
+    return load_project(RUO2_XML)
 
 
 NN = [(1, 0, 0), (0, 1, 0), (-1, 0, 0), (0, -1, 0)]

@@ -95,3 +95,23 @@ def test_many_process_models_load_or_are_rejected_cleanly():
         _lib.load().kmcg_model_free(h)
         rc, _, _ = _load(compile_model(pt, "lat_int"))
         assert rc == -4 and b"64 KiB" in _lib.load().kmcg_last_error()
+
+
+def test_lattice_species_checked_before_the_int8_cast():
+    """Species ids outside 0..S-1 are refused before the cast to int8, so
+    that 300 or 256 cannot wrap into a valid id (44, 0)."""
+    import numpy as np
+    from kmos_amd.engine import Batch
+    b = Batch.__new__(Batch)
+    b.n, b.S, b.volumes = 2, 3, np.array([4, 4], np.int64)
+    ok = b._species_array([np.array([0, 1, 2, 0]), np.array([2, 2, 1, 0])], 0, 2)
+    assert ok.dtype == np.int8 and ok.tolist() == [0, 1, 2, 0, 2, 2, 1, 0]
+    for bad in ([300, 0, 0, 0], [256, 0, 0, 0], [-1, 0, 0, 0], [3, 0, 0, 0]):
+        with pytest.raises(ValueError):
+            b._species_array([np.array(bad)], 0, 1)
+    with pytest.raises(ValueError):
+        b._species_array([np.array([0.5, 0, 0, 0])], 0, 1)
+    with pytest.raises(ValueError):
+        b._species_array([np.array([0, 0, 0])], 0, 1)          # wrong size
+    with pytest.raises(ValueError):
+        b._species_array([np.zeros(4, int)], 1, 2)             # beyond the batch

@@ -13,6 +13,7 @@ import pytest
 
 from kmos_amd.compiler import compile_model, iftree_order, _most_common
 from kmos_amd.model import load_project
+from paritylib import model_file
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -35,7 +36,7 @@ def _local_smart_ops(cm, r):
 @pytest.mark.parametrize("name", ["ruo2_local_smart", "cli_local_smart"])
 def test_local_smart_routines_match_reference(name):
     ref = _golden(name)
-    pt = load_project(os.path.join(GOLDEN, ref["model"]))
+    pt = load_project(model_file(ref["model"]))
     cm = compile_model(pt, "local_smart")
     mine = {r.name: _local_smart_ops(cm, r) for r in cm.routines}
     # the reference only writes put/take routines for non-default species
@@ -47,7 +48,7 @@ def test_local_smart_routines_match_reference(name):
 @pytest.mark.parametrize("name", ["ruo2_local_smart", "cli_local_smart"])
 def test_local_smart_run_proc_matches_reference(name):
     ref = _golden(name)
-    pt = load_project(os.path.join(GOLDEN, ref["model"]))
+    pt = load_project(model_file(ref["model"]))
     cm = compile_model(pt, "local_smart")
     for p, acts in enumerate(cm.proc_actions):
         mine = [[cm.routines[r].name, list(s)] for r, s in acts]
@@ -80,7 +81,7 @@ def _nli_leaves(cm, g, sp):
 
 def test_lat_int_groups_match_reference():
     ref = _golden("ruo2_lat_int")
-    pt = load_project(os.path.join(GOLDEN, ref["model"]))
+    pt = load_project(model_file(ref["model"]))
     cm = compile_model(pt, "lat_int")
     gnames = [g.name for g in cm.groups]
     assert sorted(gnames) == sorted(ref["groups"])
@@ -122,3 +123,94 @@ def test_iftree_order_small():
     order = [p for p, _ in iftree_order(items)]
     # x is asked twice (first at index 0), y twice (first at index 2): x wins
     assert order == ["p0", "p2", "p1", "p3"]
+
+
+def _chain_project():
+    """A small lat_int model whose groups have several members and
+    bystanders (single site 'a', species A and empty; names chosen so that
+    the reference's string sort differs from file order: 'H' < 'd' < 'z')."""
+    from kmos_amd.model import CondAct, Coord, Layer, Process, Project, Site
+
+    def c(sp, off=(0, 0, 0)):
+        return CondAct(sp, Coord("a", off, "sq"))
+    pt = Project(model_name="chain", model_dimension=2, species=["A", "empty"],
+                 default_species="empty", layers=[Layer("sq", [Site("a")])],
+                 default_layer="sq", substrate_layer="sq")
+    pt.processes = [
+        Process("z_ads", "1.", [c("empty")], [c("A")]),
+        Process("des_00", "1.", [c("A"), c("empty", (0, 1, 0)), c("empty", (1, 0, 0))], [c("empty")]),
+        Process("des_11", "1.", [c("A"), c("A", (0, 1, 0)), c("A", (1, 0, 0))], [c("empty")]),
+        Process("Hop_0", "1.", [c("A"), c("empty", (1, 0, 0)), c("empty", (-1, 0, 0))],
+                [c("empty"), c("A", (1, 0, 0))]),
+        Process("Hop_1", "1.", [c("A"), c("empty", (1, 0, 0)), c("A", (-1, 0, 0))],
+                [c("empty"), c("A", (1, 0, 0))]),
+    ]
+    pt.validate()
+    return pt
+
+
+def test_lat_int_multi_member_groups_hand_worked():
+    """Known answer worked out by hand from the reference generator's rules,
+    for groups with several members and bystanders (none of the reference's
+    lat_int fixtures has one).
+
+    Grouping (kmos/io/__init__.py:1497-1624): a condition whose species an
+    action on the same coordinate changes is a true condition, every other
+    condition a bystander; processes join the first group (in file order)
+    with equal sorted true conditions and actions and equal sorted
+    bystander coordinates.  So des_11 joins des_00 and Hop_1 joins Hop_0;
+    touchup_cell adds the groups in insertion order z_ads, des_00, Hop_0
+    (io:1788-1789).
+
+    modified_procs (io:1825-1852): for every action of the group's first
+    process and every (true condition or bystander) of every group's first
+    process on the same site name, the pair (group, action offset minus
+    condition offset), deduplicated and sorted by the text
+    "<name> (np.int64(x), np.int64(y), np.int64(z))".  The conditions of
+    z_ads, des_00 and Hop_0 sit at offsets {0}, {0, (0,1,0), (1,0,0)} and
+    {0, (1,0,0), (-1,0,0)}.  An action at 0 gives the negated offsets; the
+    Hop action at (1,0,0) adds (1,0,0) minus each.  In the text sort 'H' <
+    'd' < 'z' and '-' < '0' < '1' < '2'.
+
+    nli (io:2027-2047): per member, its true conditions and bystanders sorted
+    by cmp_coords (offset x, then y, then z) form one path of the case
+    tree; leaves appear in member order.
+    """
+    pt = _chain_project()
+    cm = compile_model(pt, "lat_int")
+    gnames = [g.name for g in cm.groups]
+    assert gnames == ["z_ads", "des_00", "Hop_0"]
+    assert [gnames[g] for g in cm.proc_group] == ["z_ads", "des_00", "des_00", "Hop_0", "Hop_0"]
+    assert [[cm.process_names[p - 1] for p in g.procs] for g in cm.groups] == \
+        [["z_ads"], ["des_00", "des_11"], ["Hop_0", "Hop_1"]]
+
+    single_action = [
+        ["Hop_0", [-1, 0, 0]], ["Hop_0", [0, 0, 0]], ["Hop_0", [1, 0, 0]],
+        ["des_00", [-1, 0, 0]], ["des_00", [0, -1, 0]], ["des_00", [0, 0, 0]],
+        ["z_ads", [0, 0, 0]],
+    ]
+    hop = [
+        ["Hop_0", [-1, 0, 0]], ["Hop_0", [0, 0, 0]], ["Hop_0", [1, 0, 0]], ["Hop_0", [2, 0, 0]],
+        ["des_00", [-1, 0, 0]], ["des_00", [0, -1, 0]], ["des_00", [0, 0, 0]],
+        ["des_00", [1, -1, 0]], ["des_00", [1, 0, 0]],
+        ["z_ads", [0, 0, 0]], ["z_ads", [1, 0, 0]],
+    ]
+    expect_mod = {"z_ads": single_action, "des_00": single_action, "Hop_0": hop}
+    expect_upd = {
+        "z_ads": [[[0, 0, 0, 1], ["empty"], "A"]],
+        "des_00": [[[0, 0, 0, 1], ["A"], "empty"]],
+        "Hop_0": [[[0, 0, 0, 1], ["A"], "empty"], [[1, 0, 0, 1], ["empty"], "A"]],
+    }
+    expect_nli = {
+        "z_ads": [([[0, 0, 0, 1, ["empty"]]], "z_ads")],
+        "des_00": [([[0, 0, 0, 1, ["A"]], [0, 1, 0, 1, ["empty"]], [1, 0, 0, 1, ["empty"]]], "des_00"),
+                   ([[0, 0, 0, 1, ["A"]], [0, 1, 0, 1, ["A"]], [1, 0, 0, 1, ["A"]]], "des_11")],
+        "Hop_0": [([[-1, 0, 0, 1, ["empty"]], [0, 0, 0, 1, ["A"]], [1, 0, 0, 1, ["empty"]]], "Hop_0"),
+                  ([[-1, 0, 0, 1, ["A"]], [0, 0, 0, 1, ["A"]], [1, 0, 0, 1, ["empty"]]], "Hop_1")],
+    }
+    sp = cm.species_names
+    for gi, g in enumerate(cm.groups):
+        assert [[gnames[og], list(off)] for og, off in g.modified] == expect_mod[g.name], g.name
+        assert [[list(s), [sp[o] for o in olds], sp[new]] for s, olds, new in g.updates] \
+            == expect_upd[g.name], g.name
+        assert _nli_leaves(cm, gi, sp) == expect_nli[g.name], g.name

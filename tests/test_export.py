@@ -10,6 +10,7 @@ from kmos_amd.compiler import compile_model
 from kmos_amd.export import export_source
 from kmos_amd.kmc_model import compiled_for_export, load_settings
 from kmos_amd.model import load_project, to_xml
+from paritylib import model_file
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 KEYS = ["model_name", "simulation_size", "random_seed", "setup_model", "parameters", "rate_constants",
@@ -19,12 +20,12 @@ KEYS = ["model_name", "simulation_size", "random_seed", "setup_model", "paramete
 @pytest.mark.parametrize("src,backend", [("ruo2_default.xml", "local_smart"), ("ruo2_default.xml", "lat_int"),
                                          ("zgb_model.ini", "local_smart"), ("ab_model.ini", "lat_int")])
 def test_export_directory(tmp_path, src, backend):
-    d = export_source(os.path.join(GOLDEN, src), str(tmp_path / "exp"), backend)
+    d = export_source(model_file(src), str(tmp_path / "exp"), backend)
     s = load_settings(d)
     for k in KEYS:
         assert hasattr(s, k), k
     assert s.backend == backend
-    pt = load_project(os.path.join(GOLDEN, src))
+    pt = load_project(model_file(src))
     assert sorted(s.rate_constants) == sorted(p.name for p in pt.processes)
     cm = compiled_for_export(d)
     np.testing.assert_array_equal(cm.to_blob(), compile_model(pt, backend).to_blob())
@@ -32,7 +33,7 @@ def test_export_directory(tmp_path, src, backend):
 
 
 def test_export_from_xml_text(tmp_path):
-    pt = load_project(os.path.join(GOLDEN, "ruo2_default.xml"))
+    pt = load_project(model_file("ruo2_default.xml"))
     d = export_source(to_xml(pt), str(tmp_path / "x"), "local_smart")
     np.testing.assert_array_equal(compiled_for_export(d).to_blob(), compile_model(pt, "local_smart").to_blob())
 

@@ -13,7 +13,7 @@ import pytest
 from kmos_amd.export import export_source
 from kmos_amd.model import evaluate_rate_expression
 from kmos_amd.run import KMC_Model
-from paritylib import compare_state, oracle_replica
+from paritylib import compare_state, model_file, oracle_replica
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -21,7 +21,7 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 @pytest.mark.parametrize("backend", ["local_smart", "lat_int"])
 def test_kmc_model_matches_oracle(engine, oracle, tmp_path, backend):
-    d = export_source(os.path.join(GOLDEN, "ruo2_default.xml"), str(tmp_path / backend), backend)
+    d = export_source(model_file("ruo2_default.xml"), str(tmp_path / backend), backend)
     model = KMC_Model(export_dir=d, size=[10, 8], random_seed=42)
     cm = model.compiled
     rates = np.array([evaluate_rate_expression(model.settings.rate_constants[p][0], model.settings_parameters)

@@ -87,6 +87,19 @@ def test_graph_replays_match_direct_launches(engine, oracle):
             for k in x:
                 np.testing.assert_array_equal(x[k], y[k], err_msg="records %s" % k)
         _same(a, b, "recorded %d" % it)
+    # the select-overlapped kernel form is part of the graph key: toggling
+    # it between replays captures (or reuses) the graph of that form
+    for it, ov in enumerate((1, 0, 1, 0)):
+        print("case overlap", ov, flush=True)
+        a.set_overlap(ov)
+        b.set_overlap(ov)
+        a.run(200, allr)
+        b.run(200, allr)
+        _same(a, b, "overlap %d, run %d" % (ov, it))
+        o.run(200)
+        compare_state(b, 0, o, where="graph replica 0 overlap %d run %d" % (ov, it))
+    a.set_overlap(-1)
+    b.set_overlap(-1)
     b.set_graph(False)
     a.run(100)
     b.run(100)

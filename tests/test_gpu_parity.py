@@ -11,7 +11,7 @@ import pytest
 
 from kmos_amd import workloads as W
 from kmos_amd.compiler import compile_model
-from paritylib import compare_state, oracle_replica
+from paritylib import compare_clock, compare_state, oracle_replica
 
 pytestmark = pytest.mark.gpu
 
@@ -296,8 +296,8 @@ def test_run_recorded_matches_oracle(engine, oracle, factory, backend):
             assert recs[r]["kmc_step"][j] == o.kmc_step == mk
             np.testing.assert_array_equal(recs[r]["procstat"][j], o.procstat())
             np.testing.assert_array_equal(recs[r]["occupation"][j], o.occupation())
-            np.testing.assert_allclose(recs[r]["kmc_time"][j], o.kmc_time, rtol=1e-12, atol=0)
-            np.testing.assert_allclose(recs[r]["integ_rates"][j], o.integ_rates(), rtol=1e-9, atol=0)
+            compare_clock(recs[r]["kmc_time"][j], recs[r]["integ_rates"][j], o.kmc_time, o.integ_rates(),
+                          "record %d of r%d" % (j, r))
         compare_state(b, r, o, where="after recorded run r%d" % r)
 
 

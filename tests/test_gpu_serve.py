@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 from kmos_amd import serve
-from paritylib import oracle_replica
+from paritylib import compare_clock, oracle_replica
 
 pytestmark = pytest.mark.gpu
 
@@ -57,8 +57,8 @@ def test_samples_match_oracle(engine, oracle):
             assert r.samples["kmc_step"][i] == o.kmc_step == done
             np.testing.assert_array_equal(r.samples["procstat"][i], o.procstat())
             np.testing.assert_array_equal(r.samples["occupation"][i], o.occupation())
-            np.testing.assert_allclose(r.samples["kmc_time"][i], o.kmc_time, rtol=1e-12, atol=0)
-            np.testing.assert_allclose(r.samples["integ_rates"][i], o.integ_rates(), rtol=1e-9, atol=0)
+            compare_clock(r.samples["kmc_time"][i], r.samples["integ_rates"][i], o.kmc_time,
+                          o.integ_rates(), "served sample %d" % i)
             if o.status:
                 break
     srv.close()

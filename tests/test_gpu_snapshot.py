@@ -20,7 +20,7 @@ from kmos_amd import workloads as W
 from kmos_amd._lib import KmcgError
 from kmos_amd.compiler import compile_model
 from kmos_amd.snapshot import read_reload, write_reload
-from paritylib import compare_state, oracle_replica
+from paritylib import compare_state, model_file, oracle_replica
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -136,7 +136,7 @@ def test_kmc_model_save_and_reload_system(engine, oracle, tmp_path, monkeypatch)
     from kmos_amd.export import export_source
     from kmos_amd.model import evaluate_rate_expression
     from kmos_amd.run import KMC_Model
-    d = export_source(os.path.join(GOLDEN, "ruo2_default.xml"), str(tmp_path / "exp"), "local_smart")
+    d = export_source(model_file("ruo2_default.xml"), str(tmp_path / "exp"), "local_smart")
     model = KMC_Model(export_dir=d, size=[10, 8], random_seed=42)
     cm = model.compiled
     rates = np.array([evaluate_rate_expression(model.settings.rate_constants[p][0], model.settings_parameters)

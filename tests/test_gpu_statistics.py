@@ -14,6 +14,14 @@ reference's own Fortran.
   oracle/build_ref.py, other seeds) run the same number of steps from the
   empty lattice; the ensemble means of the CO and O coverages and of the
   CO2 turnover agree within 3 sigma (Welch).
+* Pairwise lateral interactions (the lat_int workload's model on a 4 x 4
+  periodic lattice, eps = -0.05, 0, 0.05 eV, kT = 0.05 eV): detailed
+  balance makes the stationary law Boltzmann in the number of occupied
+  neighbour pairs, so every process's time-averaged number of enabled
+  sites is known exactly by enumerating the 2^16 states
+  (tests/pairwise_exact.py); 256 replicas per eps agree within 3 sigma
+  family-wise.  At eps = 0 this is the Langmuir law (coverage 1/2,
+  independent sites).
 """
 import numpy as np
 import pytest
@@ -107,3 +115,30 @@ def test_ruo2_ensemble_matches_reference_fortran(engine):
         a, c = gpu[:, j], ref[:, j]
         se = np.sqrt(a.var(ddof=1) / a.size + c.var(ddof=1) / c.size)
         assert abs(a.mean() - c.mean()) < 3 * se, (what, a.mean(), c.mean(), se)
+
+
+def test_pairwise_stationary_law_exact(engine):
+    from pairwise_exact import check_time_averages, enabled_means
+    pt = W.pairwise_project()
+    cm = compile_model(pt, "lat_int")
+    epss = (-0.05, 0.0, 0.05)
+    n = 256
+    rates = np.concatenate([np.tile(W.pairwise_rates(pt, e, 0.05), (n, 1)) for e in epss])
+    m = engine.Model(cm)
+    N = n * len(epss)
+    b = engine.Batch(m, [[4, 4, 1]] * N)
+    b.init(np.full(N, 5, np.uint32), np.arange(N, dtype=np.uint32))
+    b.set_rates(rates)
+    b.run(20000)
+    i0 = np.array([b.integ_rates(r) for r in range(N)])
+    t0 = np.array([b.scalars(r)[0] for r in range(N)])
+    b.run(200000)
+    i1 = np.array([b.integ_rates(r) for r in range(N)])
+    t1 = np.array([b.scalars(r)[0] for r in range(N)])
+    assert all(b.scalars(r)[4] == 0 for r in range(N))
+    assert np.all(b.check_lists() == 0)
+    b.close()
+    for k, e in enumerate(epss):
+        sl = slice(k * n, (k + 1) * n)
+        check_time_averages(cm.process_names, rates[sl], i0[sl], i1[sl], t0[sl], t1[sl],
+                            enabled_means(4, e, 0.05), where="GPU eps=%g" % e)

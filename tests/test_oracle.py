@@ -277,3 +277,32 @@ def test_poisson_stream_is_reproducible():
     for q in a:
         if q.config is not None:
             assert q.config.size == q.size[0] * q.size[1] * 2
+
+
+@pytest.mark.parametrize("eps", [0.05, -0.05])
+def test_oracle_pairwise_stationary_law_exact(oracle, eps):
+    """The pairwise lateral-interaction model (its lat_int op programs and
+    its rate table) samples the Boltzmann law of nearest-neighbour pairs:
+    on a 4 x 4 periodic lattice each process's time-averaged number of
+    enabled sites matches exact enumeration of all 2^16 states
+    (tests/pairwise_exact.py), for repulsive and attractive eps."""
+    from pairwise_exact import check_time_averages, enabled_means
+    pt = W.pairwise_project()
+    cm = compile_model(pt, "lat_int")
+    rates = W.pairwise_rates(pt, eps, 0.05)
+    exact = enabled_means(4, eps, 0.05)
+    n = 16
+    i0, i1, t0, t1 = [], [], [], []
+    for r in range(n):
+        o = oracle.Oracle(cm.to_blob(), (4, 4))
+        o.init(11, r)
+        o.set_rates(rates)
+        o.run(10000)
+        i0.append(o.integ_rates().copy())
+        t0.append(o.kmc_time)
+        o.run(60000)
+        i1.append(o.integ_rates().copy())
+        t1.append(o.kmc_time)
+        assert o.status == 0
+    check_time_averages(cm.process_names, rates, np.array(i0), np.array(i1), np.array(t0),
+                        np.array(t1), exact, where="oracle eps=%g" % eps)
