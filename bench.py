@@ -1,33 +1,38 @@
 #!/usr/bin/env python3
 """Benchmark: aggregate kMC steps/s of the batched engine on one MI355X.
 
-Workload (default): the batched ZGB model, 4096 replicas of 128x128, each
-running 1e5 kMC steps from an empty lattice with y ~ U[0.35, 0.60].  One
-bench "step" is one launch that advances every replica by
-steps_per_replica / K kMC steps, so the K timed launches run the whole
-workload (4.1e8 kMC steps); W warm-up launches run on a separate batch.
+Contract line (default workload): the batched RuO2(110) CO-oxidation model,
+2048 replicas of 20^2 : 64^2 : 128^2 cells (1 : 2 : 1), 36 processes with
+rates log-uniform in [1e-2, 1e9] per replica, random starting coverage
+0.3 CO / 0.3 O, 2e5 kMC steps each.  One bench "step" is one launch that
+advances every replica by steps_per_replica / K kMC steps, so the K timed
+launches run the whole workload (4.1e8 kMC steps); the autotuner's trials
+and the warm-up run on a separate probe batch.
 
 Printed (one JSON line): whole-job kMC steps/s (value), the plain path (one
 replica per launch) on the same model, their ratio, the HBM roofline of
 the step kernel from the algorithmic bytes per kMC step (int32 avail_sites
-entries read or written, counted by the instrumented oracle) with the LDS
-bandwidth beside it (LDS bytes per launch from the committed rocprofv3
-counter summary over this run's launch time), the launch shape the
-autotuner chose (kmos_amd/autotune.py, trials on a probe batch), and a CPU
-baseline (the C oracle on the host cores, one process per core, on a
-bounded sample of the same workload).  Two more legs ride along:
+entries read or written, counted by the instrumented oracle) with the
+measured HBM traffic and LDS bandwidth beside it (bytes per kMC step from
+the committed rocprofv3 counter summaries, times this run's kMC steps per
+launch and over this run's launch time), the launch shape the autotuner
+chose, and the CPU baseline: the reference's own generated Fortran (built
+from its checked-in export by oracle/build_ref.py, injected uniforms), one
+process per host core, with its per-step CPU_TIME call (faithful) and
+without it (sensitivity).  Legs:
 
-  ruo2     the batched RuO2 workload (2048 replicas, 20^2:64^2:128^2 =
-           1:2:1, 2e5 steps) on the GPU, next to the reference's own
-           generated Fortran (built by oracle/build_ref.py with the injected
-           uniform stream) run one replica per host core
-  pairwise the lateral-interaction workload (512 replicas of 256^2,
-           pairwise lat_int, 5e5 steps) with its roofline and the C oracle
-           on the host cores
-  serving  a Poisson stream of requests drawn from the quickstart / ZGB /
-           RuO2 workloads through the continuous-batching server: p50/p99
-           request latency and aggregate steps/s at 100 requests/s, and the
-           same at 10 and 1000 requests/s (serving_sweep)
+  zgb        4096 replicas of the ZGB model on 128^2, 1e5 steps, with the
+             C oracle on the host cores beside it
+  pairwise   512 replicas of 256^2, pairwise lat_int, 5e5 steps, with the
+             C oracle on the host cores
+  quickstart one replica of the quickstart model (20^2, CO ads/des at
+             100/s), 1e6 steps in one launch, beside the reference's
+             published single-core figure and the C oracle on one core
+  serving    Poisson requests from the quickstart / ZGB / RuO2 workloads
+             through the continuous-batching server: p50/p99 latency and
+             steps/s at 100 requests/s, the same at 10 and 1000 requests/s
+             (serving_sweep), and a burst that fills thousands of slots at
+             once (serving_saturated)
 """
 import argparse
 import json
@@ -45,55 +50,66 @@ from kmos_amd import workloads as W  # noqa: E402
 from kmos_amd.compiler import compile_model  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
-METRIC = "aggregate kMC steps/s, batched vs plain GPU; p50/p99 request latency"
-
-
-def measured_traffic(workload):
-    """HBM bytes per launch of the step kernel from the rocprofv3 PMC summary
-    committed for this workload (FETCH_SIZE + WRITE_SIZE, calibrated), or
-    None."""
-    path = os.path.join(REPO, "profiles", "kmc_steps_kernel_%s_pmc.md" % workload)
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":")[1])
-    return None
-
-
 PEAK_LDS_GBS = 256 * 128 * 2.4   # CUs x bytes per clock per CU x GHz
+METRIC = "aggregate kMC steps/s, batched vs plain GPU; p50/p99 request latency"
+# the reference's only published rate: `kmos benchmark` on the quickstart
+# model, 1e6 steps in 1.51 s on one core (README.md:85-96)
+PUBLISHED_QUICKSTART = 1e6 / 1.51
+# worker processes of the CPU baselines: the host cores this process may
+# use, at most 16 (the CPU share one GPU's host gives a job)
+CORE_CAP = 16
 
 
-def measured_lds(workload):
-    """LDS bytes per launch of the step kernel from the rocprofv3 PMC summary
-    committed for this workload (the SQ_INSTS_LDS_*_BANDWIDTH counters), or
-    None."""
-    path = os.path.join(REPO, "profiles", "kmc_steps_kernel_%s_lds.md" % workload)
+def host_cores():
+    avail = len(os.sched_getaffinity(0))
+    return avail, min(avail, CORE_CAP)
+
+
+def profile_figure(workload, kind, key):
+    """A per-kMC-step figure ('traffic' or 'lds' summary) from the rocprofv3
+    counter summary committed for this workload, or None."""
+    path = os.path.join(REPO, "profiles", "kmc_steps_kernel_%s_%s.md" % (workload, kind))
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("lds_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":")[1])
     return None
 
 
-def lds_figures(workload, avg_launch_s):
-    """Achieved LDS bandwidth of the step kernel: the profiled LDS bytes per
-    launch over this run's average launch time, against the LDS peak."""
-    b = measured_lds(workload)
-    if b is None:
-        return None
-    achieved = b / avg_launch_s / 1e9
-    return {"bytes_per_launch": b, "achieved": achieved, "peak": PEAK_LDS_GBS, "unit": "GB/s",
-            "frac": achieved / PEAK_LDS_GBS}
+def roofline(workload, bytes_per_step, steps_per_launch, avg_launch_s):
+    """HBM roofline of the step kernel: algorithmic bytes per launch over the
+    mean launch time; the measured HBM traffic and the LDS bandwidth from
+    the profiles' bytes per kMC step, scaled to this run's launches."""
+    achieved = bytes_per_step * steps_per_launch / avg_launch_s / 1e9
+    t_step = profile_figure(workload, "pmc", "traffic_bytes_per_kmc_step")
+    l_step = profile_figure(workload, "lds", "lds_bytes_per_kmc_step")
+    lds = None
+    if l_step is not None:
+        a = l_step * steps_per_launch / avg_launch_s / 1e9
+        lds = {"bytes_per_kmc_step": l_step, "achieved": a, "peak": PEAK_LDS_GBS, "unit": "GB/s",
+               "frac": a / PEAK_LDS_GBS}
+    return {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS,
+            "traffic": None if t_step is None else t_step * steps_per_launch,
+            "traffic_bytes_per_kmc_step": t_step,
+            "algorithmic_bytes_per_kmc_step": bytes_per_step,
+            "kernel": "kmc_steps_kernel", "avg_launch_ms": avg_launch_s * 1e3,
+            "kmc_steps_per_launch": steps_per_launch, "lds": lds}
 
+
+# ---------------------------------------------------------------------------
+# CPU baselines
+# ---------------------------------------------------------------------------
 
 def _oracle_worker(args):
-    blob, size, seed, stream, rates, steps, count_bytes = args
+    blob, size, seed, stream, rates, config, steps, fast = args
     from oracle import pyoracle
-    o = pyoracle.Oracle(blob, size)
+    o = pyoracle.Oracle(blob, size, fast=fast)
     o.init(seed, stream)
     o.set_rates(rates)
+    if config is not None:
+        o.set_config(config)
     o.reset_touched()
     t0 = time.perf_counter()
     n = o.run(steps)
@@ -101,48 +117,47 @@ def _oracle_worker(args):
     return n, dt, o.touched
 
 
-def algorithmic_bytes_per_step(wl, cm, sample=8, steps=20000):
-    """Mean int32 avail_sites entries touched per kMC step x 4 B (oracle
-    instrumentation), over `sample` replicas of the workload."""
+def _job(wl, r, steps, fast):
+    return (wl.project_blob, tuple(int(v) for v in wl.sizes[r][:2]), int(wl.seeds[r]), int(wl.streams[r]),
+            wl.rates[r], None if wl.configs is None else wl.configs[r], steps, fast)
+
+
+def algorithmic_bytes_per_step(wl, sample=8, steps=20000):
+    """Mean int32 avail_sites entries touched per kMC step x 4 B (the
+    oracle's counter), over `sample` replicas of the workload spread over
+    its size mix."""
     from oracle import pyoracle
     pyoracle.build()
-    blob = cm.to_blob()
     tot_steps, tot_touch = 0, 0
-    for r in range(min(sample, wl.n)):
-        n, _, touched = _oracle_worker((blob, tuple(wl.sizes[r][:2]), int(wl.seeds[r]),
-                                        int(wl.streams[r]), wl.rates[r], steps, True))
+    for r in np.linspace(0, wl.n - 1, min(sample, wl.n)).astype(int):
+        n, _, touched = _oracle_worker(_job(wl, r, steps, False))
         tot_steps += n
         tot_touch += touched
     return 4.0 * tot_touch / max(tot_steps, 1)
 
 
-def cpu_baseline(wl, cm, seconds=15.0):
-    """The C oracle on all host cores (one process per core), each core
-    running whole replicas of the workload for its full step budget."""
+def oracle_cpu_baseline(wl, seconds=15.0):
+    """The C oracle (timing build: -O3, no entry counter) on the host cores,
+    one process per core, each running whole replicas of the workload for
+    its full step budget; the sample is sized to about `seconds` of wall."""
     from oracle import pyoracle
     pyoracle.build()
-    cores = len(os.sched_getaffinity(0))
-    cores = min(cores, 16)
-    blob = cm.to_blob()
-    # size the sample so that it is about `seconds` of CPU work: time one
-    # replica first
-    n1, dt1, _ = _oracle_worker((blob, tuple(wl.sizes[0][:2]), int(wl.seeds[0]),
-                                 int(wl.streams[0]), wl.rates[0], min(wl.steps, 20000), False))
-    rate1 = n1 / max(dt1, 1e-9)
-    per_replica = wl.steps / rate1
+    avail, cores = host_cores()
+    n1, dt1, _ = _oracle_worker(_job(wl, 0, min(wl.steps, 20000), True))
+    per_replica = wl.steps / (n1 / max(dt1, 1e-9))
     nrep = int(max(cores, min(wl.n, round(seconds * cores / per_replica))))
     nrep = max(cores, (nrep // cores) * cores)
-    jobs = [(blob, tuple(wl.sizes[r % wl.n][:2]), int(wl.seeds[r % wl.n]), int(wl.streams[r % wl.n]),
-             wl.rates[r % wl.n], wl.steps, False) for r in range(nrep)]
+    idx = np.linspace(0, wl.n - 1, nrep).astype(int)
     ctx = mp.get_context("fork")
     t0 = time.perf_counter()
     with ctx.Pool(cores) as pool:
-        res = pool.map(_oracle_worker, jobs, chunksize=1)
+        res = pool.map(_oracle_worker, [_job(wl, r, wl.steps, True) for r in idx], chunksize=1)
     wall = time.perf_counter() - t0
     steps = sum(r[0] for r in res)
-    return {"value": steps / wall, "unit": "kMC steps/s", "cores": cores, "kind": "port",
-            "sample": "%d replicas x %d steps of %s (C oracle, one process per core, %.1f s wall)"
-                      % (nrep, wl.steps, wl.name, wall)}
+    return {"value": steps / wall, "unit": "kMC steps/s", "cores": cores, "cores_available": avail,
+            "cores_used": cores, "kind": "port",
+            "sample": "%d replicas x %d steps of %s (C oracle -O3 without its entry counter, one "
+                      "process per core, %.1f s wall)" % (nrep, wl.steps, wl.name, wall)}
 
 
 def _ref_worker(args):
@@ -163,16 +178,16 @@ def _ref_worker(args):
     return steps, secs
 
 
-def reference_cpu_baseline(wl, per_core=12):
+def reference_cpu_baseline(wl, per_core=12, variant=""):
     """The reference's generated Fortran (oracle/_ref, built from the
     reference's checked-in export), one process per host core, each running
-    whole replicas of the workload through the reference's do_kmc_steps."""
-    exe = os.path.join(REPO, "oracle", "_ref", "ruo2_%s" % wl.backend, "ref_driver")
+    whole replicas of the workload through the reference's do_kmc_steps;
+    variant "_noclock" is the build without the per-step CPU_TIME call."""
+    exe = os.path.join(REPO, "oracle", "_ref", "ruo2_%s%s" % (wl.backend, variant), "ref_driver")
     if not os.path.exists(exe):
         return None
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    avail, cores = host_cores()
     nrep = min(wl.n, cores * per_core)
-    # an even spread over the size mix
     idx = np.linspace(0, wl.n - 1, nrep).astype(int)
     jobs = [(exe, tuple(int(v) for v in wl.sizes[r][:2]), int(wl.seeds[r]), int(wl.streams[r]),
              wl.rates[r], None if wl.configs is None else wl.configs[r], wl.steps) for r in idx]
@@ -185,11 +200,28 @@ def reference_cpu_baseline(wl, per_core=12):
     loop = sum(r[1] for r in res)
     # all cores ran concurrently; the rate counts the reference's step loop
     # only (do_kmc_steps wall time), not process start-up or set-up
-    return {"value": cores * steps / loop, "unit": "kMC steps/s", "cores": cores, "kind": "reference",
+    return {"value": cores * steps / loop, "unit": "kMC steps/s", "cores": cores, "cores_available": avail,
+            "cores_used": cores, "kind": "reference",
             "sample": "%d replicas (sizes spread over the mix) x %d steps of %s through the reference "
-                      "Fortran's do_kmc_steps (flang -O3, injected uniforms), one process per core "
+                      "Fortran's do_kmc_steps (flang -O3, injected uniforms%s), one process per core "
                       "running concurrently; value = cores x steps / summed loop seconds; %.1f s wall"
-                      % (nrep, wl.steps, wl.name, wall)}
+                      % (nrep, wl.steps, wl.name,
+                         ", per-step CPU_TIME call removed" if variant else ", per-step CPU_TIME call kept",
+                         wall)}
+
+
+# ---------------------------------------------------------------------------
+# GPU legs
+# ---------------------------------------------------------------------------
+
+def make_batch(engine, model, wl, idx=None):
+    idx = np.arange(wl.n) if idx is None else np.asarray(idx)
+    b = engine.Batch(model, [list(s) for s in wl.sizes[idx]])
+    b.init(wl.seeds[idx], wl.streams[idx])
+    b.set_rates(wl.rates[idx])
+    if wl.configs is not None:
+        b.set_config([wl.configs[i] for i in idx])
+    return b
 
 
 def run_batched(engine, model, wl, K, warmup, tune=True):
@@ -226,6 +258,99 @@ def run_batched(engine, model, wl, K, warmup, tune=True):
     return b, wall, executed, kernel_ms, chunk, choice
 
 
+def launch_summary(choice):
+    """The autotuner's choice and how it compares with the default shape
+    (lattice in LDS when it fits, four replica waves per workgroup, replicas
+    in the given order)."""
+    if not choice:
+        return None
+    default = [t["ms"] for t in choice["table"]
+               if (t["lattice_in_lds"], t["waves_per_group"], t["order"]) == (-1, 4, "given")]
+    return {"lattice_in_lds": choice["lattice_in_lds"], "waves_per_group": choice["waves_per_group"],
+            "replica_order": choice["order"], "candidates": len(choice["table"]),
+            "trial_kmc_steps_per_replica": choice["steps"], "best_trial_ms": choice["ms"],
+            "default_trial_ms": default[0] if default else None}
+
+
+def spot_check(b, wl, cm, k_after, replicas=(0,)):
+    """The listed replicas of the timed batch against the oracle, bit-exact."""
+    from oracle import pyoracle
+    pyoracle.build()
+    from tests.paritylib import compare_state, oracle_replica
+    try:
+        for r in replicas:
+            o = oracle_replica(pyoracle, cm, tuple(wl.sizes[r][:2]), int(wl.seeds[r]), int(wl.streams[r]),
+                               wl.rates[r], None if wl.configs is None else wl.configs[r])
+            o.run(int(k_after[r]))
+            compare_state(b, r, o, where="bench spot check replica %d" % r)
+    except AssertionError as e:
+        print("spot check FAILED: %s" % e, file=sys.stderr)
+        return False
+    return True
+
+
+def plain_rate(engine, model, wl, steps, r=0):
+    """The plain path: one replica per launch, same kernel; kMC steps/s of
+    one launch of `steps` steps after a short warm-up launch."""
+    pb = make_batch(engine, model, wl, [r])
+    pb.run(min(steps, 2000))
+    t0 = time.perf_counter()
+    pb.run(steps)
+    wall = time.perf_counter() - t0
+    pb.close()
+    return steps / wall
+
+
+def batched_leg(engine, wl, K, warmup, tune, cpu, spot=(0,)):
+    """One batched workload: throughput, launch shape, replica status,
+    parity spot check, roofline and (with `cpu`) the C oracle baseline."""
+    cm = compile_model(wl.project, wl.backend)
+    wl.project_blob = cm.to_blob()
+    model = engine.Model(cm)
+    b, wall, executed, kernel_ms, chunk, choice = run_batched(engine, model, wl, K, warmup, tune)
+    _, k_after, st = b.scalars_all()
+    ok = spot_check(b, wl, cm, k_after, spot)
+    b.close()
+    avg_s = float(np.mean(kernel_ms)) / 1e3
+    out = {"workload": wl.name, "replicas": wl.n, "steps_per_replica": wl.steps,
+           "gpu_steps_per_s": executed / wall, "wall_s": wall, "avg_launch_ms": avg_s * 1e3,
+           "launch": launch_summary(choice), "parity_spot_check": ok,
+           "replica_status_counts": {str(k): int(v) for k, v in zip(*np.unique(st, return_counts=True))},
+           "roofline": roofline(wl.name, algorithmic_bytes_per_step(wl), executed / K, avg_s)}
+    if cpu:
+        out["cpu_baseline"] = oracle_cpu_baseline(wl, 10.0)
+        out["gpu_over_cpu"] = out["gpu_steps_per_s"] / out["cpu_baseline"]["value"]
+    return out, model, cm, wall, executed, chunk, choice
+
+
+def quickstart_leg(engine, cpu):
+    """Config 1: one replica of the quickstart model, 1e6 steps in one
+    launch (the plain path at its own workload), beside the reference's
+    published single-core rate and the C oracle on one core."""
+    wl = W.quickstart()
+    cm = compile_model(wl.project, wl.backend)
+    wl.project_blob = cm.to_blob()
+    model = engine.Model(cm)
+    b = make_batch(engine, model, wl)
+    b.run(2000)                     # warm-up on a second batch below
+    b.close()
+    b = make_batch(engine, model, wl)
+    t0 = time.perf_counter()
+    b.run(wl.steps)
+    wall = time.perf_counter() - t0
+    _, k_after, _ = b.scalars_all()
+    ok = spot_check(b, wl, cm, k_after)
+    b.close()
+    out = {"workload": wl.name, "replicas": 1, "steps_per_replica": wl.steps,
+           "gpu_steps_per_s": wl.steps / wall, "kernel_ms": None, "parity_spot_check": ok,
+           "published_single_core_steps_per_s": PUBLISHED_QUICKSTART,
+           "published_source": "kmos README.md:85-96 (kmos benchmark, 1e6 steps in 1.51 s, hardware not stated)"}
+    if cpu:
+        n, dt, _ = _oracle_worker(_job(wl, 0, wl.steps, True))
+        out["cpu_one_core_steps_per_s"] = n / dt
+    return out
+
+
 def graph_leg(engine, model, wl, steps_per_run=50, runs=300):
     """Host cost of short runs: one replica advanced `runs` times by
     `steps_per_run` kMC steps, by direct launches and by replays of a
@@ -243,71 +368,6 @@ def graph_leg(engine, model, wl, steps_per_run=50, runs=300):
         out[mode + "_us_per_run"] = (time.perf_counter() - t0) / runs * 1e6
         out[mode + "_kernel_us"] = b.last_run_ms() * 1e3
         b.close()
-    return out
-
-
-def launch_summary(choice):
-    """The autotuner's choice and how it compares with the default shape
-    (lattice in LDS when it fits, four replica waves per workgroup, replicas
-    in the given order)."""
-    if not choice:
-        return None
-    default = [t["ms"] for t in choice["table"]
-               if (t["lattice_in_lds"], t["waves_per_group"], t["order"]) == (-1, 4, "given")]
-    return {"lattice_in_lds": choice["lattice_in_lds"], "waves_per_group": choice["waves_per_group"],
-            "replica_order": choice["order"], "candidates": len(choice["table"]),
-            "trial_kmc_steps_per_replica": choice["steps"], "best_trial_ms": choice["ms"],
-            "default_trial_ms": default[0] if default else None}
-
-
-def ruo2_leg(engine, K, no_cpu, tune=True):
-    wl = W.ruo2()
-    cm = compile_model(wl.project, wl.backend)
-    model = engine.Model(cm)
-    b, wall, executed, kernel_ms, chunk, choice = run_batched(engine, model, wl, K, 1, tune)
-    st = b.scalars_all()[2]
-    b.close()
-    bps = algorithmic_bytes_per_step(wl, cm)
-    avg_s = float(np.mean(kernel_ms)) / 1e3
-    achieved = executed / K * bps / avg_s / 1e9
-    out = {"workload": wl.name, "replicas": wl.n, "steps_per_replica": wl.steps,
-           "gpu_steps_per_s": executed / wall, "avg_launch_ms": avg_s * 1e3,
-           "launch": launch_summary(choice),
-           "replica_status_counts": {str(k): int(v) for k, v in zip(*np.unique(st, return_counts=True))},
-           "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                        "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(wl.name),
-                        "algorithmic_bytes_per_kmc_step": bps, "lds": lds_figures(wl.name, avg_s)}}
-    if not no_cpu:
-        ref = reference_cpu_baseline(wl)
-        if ref is not None:
-            out["cpu_baseline"] = ref
-            out["gpu_over_cpu_reference"] = out["gpu_steps_per_s"] / ref["value"]
-    return out
-
-
-def pairwise_leg(engine, K, no_cpu, tune=True):
-    """The lateral-interaction workload (512 replicas of 256^2, pairwise
-    lat_int, 5e5 steps each) with its own roofline figures and the C
-    oracle on the host cores beside it (no reference build exists for it)."""
-    wl = W.pairwise()
-    cm = compile_model(wl.project, wl.backend)
-    model = engine.Model(cm)
-    b, wall, executed, kernel_ms, chunk, choice = run_batched(engine, model, wl, K, 1, tune)
-    st = b.scalars_all()[2]
-    b.close()
-    bps = algorithmic_bytes_per_step(wl, cm)
-    avg_s = float(np.mean(kernel_ms)) / 1e3
-    achieved = executed / K * bps / avg_s / 1e9
-    out = {"workload": wl.name, "replicas": wl.n, "steps_per_replica": wl.steps,
-           "gpu_steps_per_s": executed / wall, "avg_launch_ms": avg_s * 1e3,
-           "launch": launch_summary(choice),
-           "replica_status_counts": {str(k): int(v) for k, v in zip(*np.unique(st, return_counts=True))},
-           "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                        "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(wl.name),
-                        "algorithmic_bytes_per_kmc_step": bps, "lds": lds_figures(wl.name, avg_s)}}
-    if not no_cpu:
-        out["cpu_baseline"] = cpu_baseline(wl, cm, 5.0)
-        out["gpu_over_cpu"] = out["gpu_steps_per_s"] / out["cpu_baseline"]["value"]
     return out
 
 
@@ -341,19 +401,58 @@ def serving_leg(rate, n, seed=1, tune=True):
 
 def serving_sweep(rates=(10.0, 1000.0), seed=2, ipl=4):
     """The serving stream at other arrival rates (the stream's sweep from 10
-    to 1000 requests/s), each with about five seconds of arrivals (at least
-    40 and at most 400 requests): p50/p99 latency and steps/s per rate."""
+    to 1000 requests/s), each with five seconds of arrivals (at least 40
+    requests): p50/p99 latency and steps/s per rate."""
     from kmos_amd import serve
     out = []
     for rate in rates:
-        n = int(min(400, max(40, rate * 5)))
+        n = int(max(40, rate * 5))
         srv = serve.Server(hbm_budget=200 << 30, intervals_per_launch=ipl)
         t0 = time.perf_counter()
         res = srv.serve(serve.poisson_stream(rate=rate, n=n, seed=seed), realtime=True)
         s = serve.latency_summary(res, time.perf_counter() - t0)
-        srv.close()
         out.append({"arrival_rate_per_s": rate, "requests": n, "p50_latency_s": s["p50_latency_s"],
-                    "p99_latency_s": s["p99_latency_s"], "steps_per_s": s["steps_per_s"]})
+                    "p99_latency_s": s["p99_latency_s"], "steps_per_s": s["steps_per_s"],
+                    "peak_active": srv.stats["peak_active"]})
+        srv.close()
+    return out
+
+
+def serving_saturated(batched_rate, n=4096, rate=1e5, seed=3, ipl=8):
+    """A burst that fills the server: `n` requests arriving within
+    n / rate seconds, so that thousands of trajectories are resident at
+    once.  Reports p50/p99, aggregate steps/s over the whole stream, the
+    steps/s while at least half the requests were still in flight, and
+    both as a fraction of the contract line's batched rate."""
+    from kmos_amd import serve
+    reqs = serve.poisson_stream(rate=rate, n=n, seed=seed)
+    srv = serve.Server(hbm_budget=200 << 30, intervals_per_launch=ipl, slots_per_size=256)
+    t0 = time.perf_counter()
+    res = srv.serve(reqs, realtime=True)
+    wall = time.perf_counter() - t0
+    s = serve.latency_summary(res, wall)
+    # the saturated window: from the last arrival until half the requests
+    # have finished; steps done in it from each request's sample records
+    fin = np.sort([r.finished for r in res])
+    t_a, t_b = max(r.arrival for r in res), fin[len(fin) // 2]
+    steps_in = 0.0
+    for r in res:
+        if r.finished <= t_a or r.steps_done == 0:
+            continue
+        # steps spread evenly over the request's residence (admission to finish)
+        span = max(r.finished - r.admitted, 1e-9)
+        lo, hi = max(r.admitted, t_a), min(r.finished, t_b)
+        if hi > lo:
+            steps_in += r.steps_done * (hi - lo) / span
+    sat = steps_in / (t_b - t_a) if t_b > t_a else None
+    out = {"requests": n, "arrival_rate_per_s": rate, "p50_latency_s": s["p50_latency_s"],
+           "p99_latency_s": s["p99_latency_s"], "steps_per_s": s["steps_per_s"],
+           "steps_per_s_saturated_window": sat, "saturated_window_s": t_b - t_a,
+           "peak_active": srv.stats["peak_active"], "pages": srv.stats["pages"],
+           "peak_hbm_bytes": srv.stats["peak_bytes"], "intervals_per_launch": ipl,
+           "steps_per_s_over_batched": s["steps_per_s"] / batched_rate,
+           "saturated_over_batched": None if sat is None else sat / batched_rate}
+    srv.close()
     return out
 
 
@@ -375,30 +474,18 @@ def _progress(msg):
     print("bench.py: %s" % msg, file=sys.stderr, flush=True)
 
 
-def make_batch(engine, model, wl, idx=None):
-    idx = np.arange(wl.n) if idx is None else np.asarray(idx)
-    b = engine.Batch(model, [list(s) for s in wl.sizes[idx]])
-    b.init(wl.seeds[idx], wl.streams[idx])
-    b.set_rates(wl.rates[idx])
-    if wl.configs is not None:
-        b.set_config([wl.configs[i] for i in idx])
-    return b
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="zgb", choices=sorted(W.WORKLOADS))
+    ap.add_argument("--workload", default="ruo2", choices=sorted(W.WORKLOADS))
     ap.add_argument("--replicas", type=int, default=None)
     ap.add_argument("--kmc-steps", type=int, default=None, help="kMC steps per replica")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=20000)
-    ap.add_argument("--no-ruo2", action="store_true", help="skip the RuO2 leg")
-    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
-    ap.add_argument("--no-pairwise", action="store_true", help="skip the lateral-interaction leg")
+    ap.add_argument("--legs", default="zgb,pairwise,quickstart,serving",
+                    help="comma-separated legs beside the contract line (empty: none)")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="request arrivals per second")
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--no-tune", action="store_true",
@@ -419,50 +506,19 @@ def main():
     if a.kmc_steps is not None:
         kw["steps"] = a.kmc_steps
     wl = W.WORKLOADS[a.workload](**kw)
-    cm = compile_model(wl.project, wl.backend)
+    legs = [x for x in a.legs.split(",") if x]
 
     from kmos_amd import _lib, engine
     _lib.load()
     if _lib.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         return 1
-    model = engine.Model(cm)
     K = max(1, a.steps)
-    b, wall, executed, kernel_ms, chunk, choice = run_batched(engine, model, wl, K, a.warmup, not a.no_tune)
+    head, model, cm, wall, executed, chunk, choice = batched_leg(engine, wl, K, a.warmup, not a.no_tune,
+                                                                 cpu=False)
     _progress("batched %s: %d launches timed" % (wl.name, K))
-    t_after, k_after, st_after = b.scalars_all()
     value = executed / wall
-    avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
-    steps_per_launch = executed / K
-
-    # spot check: replica 0 against the oracle, bit-exact
-    spot = None
-    try:
-        from oracle import pyoracle
-        pyoracle.build()
-        from tests.paritylib import compare_state, oracle_replica
-        o = oracle_replica(pyoracle, cm, tuple(wl.sizes[0][:2]), int(wl.seeds[0]), int(wl.streams[0]),
-                           wl.rates[0], None if wl.configs is None else wl.configs[0])
-        o.run(int(k_after[0]))
-        compare_state(b, 0, o, where="bench spot check")
-        spot = True
-    except AssertionError as e:
-        print("spot check FAILED: %s" % e, file=sys.stderr)
-        spot = False
-
-    # plain path: one replica per launch, same kernel
-    pb = make_batch(engine, model, wl, [0])
-    pb.run(min(a.plain_steps, 2000))
-    tp0 = time.perf_counter()
-    pb.run(a.plain_steps)
-    plain_wall = time.perf_counter() - tp0
-    plain = a.plain_steps / plain_wall
-    pb.close()
-
-    b.close()
-    bytes_per_step = algorithmic_bytes_per_step(wl, cm)
-    achieved = steps_per_launch * bytes_per_step / avg_kernel_s / 1e9
-    traffic = measured_traffic(wl.name)
+    plain = plain_rate(engine, model, wl, a.plain_steps)
     out = {
         "metric": METRIC,
         "value": value,
@@ -477,34 +533,45 @@ def main():
         "dtype": "f64",
         "data": "synthetic (workload parameters from a seeded Philox generator)",
         "config": dict(wl.describe(), kmc_steps_per_launch=chunk),
-        "launch": launch_summary(choice),
+        "launch": head["launch"],
         "plain_gpu_steps_per_s": plain,
         "batched_over_plain": value / plain,
-        "replica_status_counts": {str(k): int(v) for k, v in zip(*np.unique(st_after, return_counts=True))},
-        "parity_spot_check": spot,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_kmc_step": bytes_per_step,
-                     "kernel": "kmc_steps_kernel", "avg_launch_ms": avg_kernel_s * 1e3,
-                     "lds": lds_figures(wl.name, avg_kernel_s)},
+        "replica_status_counts": head["replica_status_counts"],
+        "parity_spot_check": head["parity_spot_check"],
+        "roofline": head["roofline"],
     }
     _progress("plain path and spot check done")
     out["graph_replay"] = graph_leg(engine, model, wl)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(wl, cm, a.cpu_seconds)
-        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+        ref = reference_cpu_baseline(wl)
+        if ref is not None:
+            out["cpu_baseline"] = ref
+            out["gpu_over_cpu"] = value / ref["value"]
+            nc = reference_cpu_baseline(wl, variant="_noclock")
+            if nc is not None:
+                out["cpu_baseline_no_cpu_time"] = nc
+                out["gpu_over_cpu_no_cpu_time"] = value / nc["value"]
+        else:
+            wl.project_blob = cm.to_blob()
+            out["cpu_baseline"] = oracle_cpu_baseline(wl)
+            out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
         _progress("cpu baseline done")
-    if not a.no_ruo2 and a.workload == "zgb":
-        out["ruo2"] = ruo2_leg(engine, K, a.no_cpu, not a.no_tune)
-        _progress("ruo2 leg done")
-    if not a.no_pairwise and a.workload == "zgb":
-        out["pairwise"] = pairwise_leg(engine, K, a.no_cpu, not a.no_tune)
+    if "zgb" in legs and a.workload != "zgb":
+        out["zgb"] = batched_leg(engine, W.zgb(), K, 1, not a.no_tune, not a.no_cpu)[0]
+        _progress("zgb leg done")
+    if "pairwise" in legs and a.workload != "pairwise":
+        out["pairwise"] = batched_leg(engine, W.pairwise(), K, 1, not a.no_tune, not a.no_cpu)[0]
         _progress("pairwise leg done")
-    if not a.no_serve:
+    if "quickstart" in legs and a.workload != "quickstart":
+        out["quickstart"] = quickstart_leg(engine, not a.no_cpu)
+        _progress("quickstart leg done")
+    if "serving" in legs:
         out["serving"] = serving_leg(a.serve_rate, a.serve_requests, tune=not a.no_tune)
         _progress("serving leg done")
         out["serving_sweep"] = serving_sweep(ipl=out["serving"]["intervals_per_launch"])
         _progress("serving sweep done")
+        out["serving_saturated"] = serving_saturated(value)
+        _progress("saturated serving done")
         out["p50_latency_s"] = out["serving"]["p50_latency_s"]
         out["p99_latency_s"] = out["serving"]["p99_latency_s"]
     print(json.dumps(out))

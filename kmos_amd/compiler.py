@@ -12,11 +12,17 @@ local_smart (kmos/io/__init__.py:305-465, 2219-2443, 2568-2655)
       the ordered disable list (process order x condition order) and the
       enable list in the text order of the generated if-tree
       (write_proclist_put_take, _write_optimal_iftree);
-    * per site type: the touchup routine (write_proclist_touchup).
+    * per site type: the touchup routine (write_proclist_touchup);
+    * multi-layer models: per (site, species) used by a '^'/'$' action, the
+      create/annihilate routine (write_proclist_multilattice): the write
+      from/to the null species, then every process with a condition on
+      that site disabled (annihilate) or enabled through the if-tree over
+      all its conditions (create).
 
 lat_int (kmos/io/__init__.py:1497-2057)
     * lateral-interaction groups (_get_lat_int_groups);
-    * per group: the sorted ``modified_procs`` list, the lattice updates and
+    * per group: the sorted ``modified_procs`` list, the lattice updates
+      (multi-layer: '$' writes the null species, '^' writes over it) and
       the nli case tree that maps a cell to a process of the group or 0
       (write_proclist_lat_int_run_proc, write_proclist_lat_int_nli_casetree).
 
@@ -90,6 +96,9 @@ class CompiledModel:
     process_names: List[str]
     species_names: List[str]
     site_names: List[str]
+    # 1 per site type that initialize_state touches up (the sites of the
+    # initial layer; every site of a one-layer model)
+    site_init: List[int] = dataclasses.field(default_factory=list)
     # local_smart
     proc_exec_site: List[int] = dataclasses.field(default_factory=list)
     proc_actions: List[List[Tuple[int, RelSite]]] = dataclasses.field(default_factory=list)
@@ -203,16 +212,42 @@ def iftree_order(items: List[Tuple[List[Tuple[object, str]], object]]) -> List[T
 # local_smart
 # ---------------------------------------------------------------------------
 
+def multilattice_routine(pt: Project, op: str, lname: str, sname: str, species: str) -> Routine:
+    """``create_`` / ``annihilate_<layer>_<site>(site, species)`` of a
+    multi-layer model (kmos/io/__init__.py:2445-2560) for one species: the
+    write from (create) or to (annihilate) the null species, then, for
+    every process and every condition it has on this site type (process
+    order, condition order), annihilate's ``del_proc`` of the process at its
+    executing coordinate, or create's ``add_proc`` in the text order of the
+    if-tree over all the process's conditions, this site's own included."""
+    sid = pt.species_ids()
+    pid = pt.process_ids()
+    if species not in sid:
+        raise ValueError("%s of unknown species %r" % (op, species))
+    enabled, disabled = [], []
+    for proc in pt.processes:
+        ex = proc.executing_coord()
+        for cond in proc.conditions:
+            if (cond.coord.name, cond.coord.layer) != (sname, lname):
+                continue
+            if op == "create":
+                conds = [(_rel(o.coord, cond.coord, pt), o.species) for o in proc.conditions]
+                enabled.append((conds, (pid[proc.name], _rel(ex, cond.coord, pt))))
+            else:
+                disabled.append((pid[proc.name], _rel(ex, cond.coord, pt)))
+    enables = [EnableItem(proc=payload[0], site=payload[1], conds=[(c, sid[sp]) for c, sp in checks.items()])
+               for payload, checks in iftree_order(enabled)]
+    null = pt.null_species()
+    old, new = (null, sid[species]) if op == "create" else (sid[species], null)
+    return Routine(name="%s_%s_%s:%s" % (op, lname, sname, species), site_type=pt.site_index(lname, sname),
+                   old_species=old, new_species=new, disables=disabled, enables=enables)
+
+
 def _compile_local_smart(pt: Project, cm: CompiledModel) -> None:
     sid = pt.species_ids()
     pid = pt.process_ids()
     default = pt.default_species
-    for p in pt.processes:
-        for ca in p.actions:
-            if ca.species[:1] in ("^", "$"):
-                raise NotImplementedError(
-                    "multi-lattice create/annihilate actions (process %s) are not "
-                    "supported by this engine" % p.name)
+    null = pt.null_species()
 
     # routines: put/take per non-default species x site (io:2219-2409)
     routine_index: Dict[Tuple[str, str, str, str], int] = {}
@@ -251,6 +286,17 @@ def _compile_local_smart(pt: Project, cm: CompiledModel) -> None:
                         old_species=old, new_species=new,
                         disables=disabled, enables=enables))
 
+    # create_/annihilate_<layer>_<site> (io:2445-2560), multi-layer models
+    # only: kmos writes one routine per site taking the species as an
+    # argument; here one routine per (site, species) that an action uses
+    if len(pt.layers) > 1:
+        used = sorted(set(("create" if ca.species[0] == "^" else "annihilate", ca.species[1:],
+                           ca.coord.layer, ca.coord.name)
+                          for p in pt.processes for ca in p.actions if ca.species[:1] in ("^", "$")))
+        for op, species, lname, sname in used:
+            routine_index[(op, species, lname, sname)] = len(cm.routines)
+            cm.routines.append(multilattice_routine(pt, op, lname, sname, species))
+
     # touchups (io:2411-2443)
     for layer in pt.layers:
         for site in layer.sites:
@@ -279,12 +325,18 @@ def _compile_local_smart(pt: Project, cm: CompiledModel) -> None:
         acts = []
         for action in proc.actions:
             rel = _rel(action.coord, ex, pt)
+            key = (action.coord.layer, action.coord.name)
+            if action.species[:1] == "^":
+                acts.append((routine_index[("create", action.species[1:]) + key], rel))
+                continue
+            if action.species[:1] == "$":
+                acts.append((routine_index[("annihilate", action.species[1:]) + key], rel))
+                continue
             prev = [c.species for c in proc.conditions if c.coord == action.coord]
             if not prev:
                 raise ValueError("process %s: action on %s has no condition on the same site"
                                  % (proc.name, action.coord))
             prev = prev[0]
-            key = (action.coord.layer, action.coord.name)
             if action.species == default and action.species != prev:
                 acts.append((routine_index[("take", prev) + key], rel))
             elif prev != action.species:
@@ -320,9 +372,11 @@ def lat_int_groups(pt: Project) -> List[Tuple[str, List[dict]]]:
                     byst.append(c)
             else:
                 byst.append(c)
+        # actions without a changing condition: multi-lattice create/annihilate
+        # (io:1536-1551); any other is ignored, as kmos does after its warning
         for a in p.actions:
             if a not in true_a and a.species[:1] in ("^", "$"):
-                raise NotImplementedError("multi-lattice actions are not supported (%s)" % p.name)
+                true_a.append(a)
         procs.append(dict(name=p.name, conditions=true_c, actions=true_a, bystanders=byst))
 
     key = functools.cmp_to_key(lambda a, b: cmp_coords(a.coord, b.coord))
@@ -364,14 +418,37 @@ def _compile_lat_int(pt: Project, cm: CompiledModel) -> None:
                         off = tuple(action.coord.offset[i] - cond.coord.offset[i] for i in range(3))
                         modified.add((oname, off))
         modified = sorted(modified, key=lambda x: "%s %s" % (x[0], _np2_tuple_str(x[1])))
-        # lattice updates (io:1858-1910)
+        # lattice updates (io:1858-1965): changing conditions in order, then
+        # the multi-lattice actions without a condition
+        null = pt.null_species()
         updates = []
+        matched = []
+
+        def site_of(a):
+            return (a.coord.offset[0], a.coord.offset[1], a.coord.offset[2],
+                    pt.site_index(a.coord.layer, a.coord.name))
+
         for cond in p0["conditions"]:
             action = [a for a in p0["actions"] if cond.coord == a.coord][0]
+            matched.append(action)
             olds = [sid[s.strip()] for s in cond.species.split(" or ")]
-            site = (action.coord.offset[0], action.coord.offset[1], action.coord.offset[2],
-                    pt.site_index(action.coord.layer, action.coord.name))
-            updates.append((site, olds, sid[action.species]))
+            if action.species[:1] == "$":
+                new = null
+            elif action.species[:1] == "^":
+                # kmos writes a replace_species with the '^' species text here,
+                # which does not compile
+                raise NotImplementedError("%s: create action on a site with a condition" % gname)
+            else:
+                new = sid[action.species]
+            updates.append((site_of(action), olds, new))
+        for action in p0["actions"]:
+            if action in matched:
+                continue
+            sp = action.species[1:]
+            if action.species[:1] == "$":
+                updates.append((site_of(action), [sid[s.strip()] for s in sp.split(" or ")], null))
+            else:
+                updates.append((site_of(action), [null], sid[sp]))
         # nli case tree (io:2027-2047): nested dicts in insertion order
         tree: Dict = {}
         for m in members:
@@ -437,8 +514,6 @@ def compile_model(pt: Project, backend: str = "local_smart") -> CompiledModel:
     """Compile ``pt`` for ``backend`` ('local_smart' or 'lat_int')."""
     if backend not in BACKENDS:
         raise ValueError("unknown backend %r" % backend)
-    if len(pt.layers) != 1:
-        raise NotImplementedError("multi-layer (multi-lattice) models are not supported")
     pt.validate()
     species_sorted = sorted(pt.species)
     cm = CompiledModel(
@@ -448,7 +523,9 @@ def compile_model(pt: Project, backend: str = "local_smart") -> CompiledModel:
         dimension=pt.model_dimension, site_default=pt.site_default_species(),
         process_names=[p.name for p in pt.processes],
         species_names=species_sorted,
-        site_names=["%s_%s" % ls for ls in pt.site_list()])
+        site_names=["%s_%s" % ls for ls in pt.site_list()],
+        site_init=[1 if (len(pt.layers) == 1 or lname == pt.default_layer) else 0
+                   for lname, _ in pt.site_list()])
     if len(pt.species) > 8:
         raise NotImplementedError("at most 8 species are supported")
     if backend == "local_smart":
@@ -470,7 +547,10 @@ def _to_blob(cm: CompiledModel) -> np.ndarray:
       5 null_species, 6 spuck, 7 dimension,
       8 off_site_default, 9 off_proc_table, 10 n_routines, 11 off_routine_table,
       12 off_touchup (spuck words: routine id per site type),
-      13 n_groups, 14 off_group_table, 15 max_offset, 16 total words
+      13 n_groups, 14 off_group_table, 15 max_offset, 16 total words,
+      17 off_site_init (spuck words: 1 = initialize_state touches the site
+         type up, proclist_generic_subroutines.mpy:279-300 runs the touchups
+         of the initial layer only)
     local_smart:
       proc table [P][3]: exec_site, n_actions, off_actions -> [routine, dx, dy, dz, n]*
       routine table [R][7]: site_type, old, new, n_dis, off_dis, n_en, off_en
@@ -499,6 +579,8 @@ def _to_blob(cm: CompiledModel) -> np.ndarray:
     hdr[7] = cm.dimension
     hdr[8] = here()
     words.extend(cm.site_default)
+    hdr[17] = here()
+    words.extend(cm.site_init)
     if cm.backend == BACKEND_LOCAL_SMART:
         hdr[9] = here()
         ptab = here()

@@ -54,7 +54,7 @@ struct Wave {
     bool local_smart;
     int *avail;        // sites[P][V+1] then back[P][V+1]
     long long V1;
-    int P, lane, mstride, mrows;
+    int P, S, lane, mstride, mrows;
     int off_seg, off_proc, off_touch, off_nli, off_nlitab;
     int nr[PPL];
     double rate[PPL];
@@ -504,9 +504,13 @@ __device__ __forceinline__ int post_segment_t(Wave<PPL, BITS> &W, const SegPre &
             if (w < nw && W.lane < nok) {
                 W.lat.xset(s, cur, nsp);
                 if (W.track) {
+                    // counts per species and site type (the null species of
+                    // a multi-layer model is not counted)
                     const int t = prog[woff + WR_W * w] >> 24 & 0xFF;   // site type n
-                    atomicAdd(&W.occ[cur * W.d.spuck + t - 1], -1);
-                    atomicAdd(&W.occ[nsp * W.d.spuck + t - 1], 1);
+                    if (cur < W.S)
+                        atomicAdd(&W.occ[cur * W.d.spuck + t - 1], -1);
+                    if (nsp < W.S)
+                        atomicAdd(&W.occ[nsp * W.d.spuck + t - 1], 1);
                 }
             }
             if (bm) {
@@ -569,8 +573,10 @@ __device__ __forceinline__ int post_pair(Wave<PPL, BITS> &W, const SegPre &q1, c
     if (W.lane < 2) {
         W.lat.xset(s, cur, nsp);
         if (W.track) {
-            atomicAdd(&W.occ[cur * W.d.spuck + t - 1], -1);
-            atomicAdd(&W.occ[nsp * W.d.spuck + t - 1], 1);
+            if (cur < W.S)
+                atomicAdd(&W.occ[cur * W.d.spuck + t - 1], -1);
+            if (nsp < W.S)
+                atomicAdd(&W.occ[nsp * W.d.spuck + t - 1], 1);
         }
     }
     wave_sync();
@@ -872,6 +878,7 @@ __device__ __forceinline__ int wave_setup(Wave<PPL, BITS> &W, const KParams &kp,
     W.prog = sprog;
     W.lane = threadIdx.x & 63;
     W.P = kp.P;
+    W.S = kp.S;
     W.mstride = 64 * PPL;
     W.off_seg = kp.off_seg;
     W.off_proc = kp.off_proc;
@@ -1197,7 +1204,7 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
         wave_sync();
         for (int s = lane; s < W.d.V; s += 64) {
             const int q = W.lat.get(s);
-            if (q >= 0)
+            if (q >= 0 && q < W.S)
                 atomicAdd(&W.occ[q * W.d.spuck + (s - divm(s, W.ms) * W.d.spuck)], 1);
         }
         wave_sync();
@@ -1490,9 +1497,10 @@ __device__ int nli_eval_g(const int *prog, int off_nli, int group, const Lattice
 // site) pair at most once and in increasing site order, so the list of each
 // process is the ordered set of sites where it is enabled.  One workgroup
 // builds one process's list of one replica by a block-wide compaction.
-// init entry per process (3 words at kp.arg0): site type of its anchor, the
-// word offset of its touchup addition op (local_smart, -1: never enabled)
-// or its group (lat_int), the op's condition base.
+// init entry per process (3 words at kp.arg0): site type of its anchor (bit
+// 16: a site type initialize_state does not touch up, bit 17: multi-layer
+// model), the word offset of its touchup addition op (local_smart, -1:
+// never enabled) or its group (lat_int), the op's condition base.
 template <int BITS>
 __global__ __launch_bounds__(256) void kmc_init_lists_kernel(KParams kp)
 {
@@ -1508,9 +1516,9 @@ __global__ __launch_bounds__(256) void kmc_init_lists_kernel(KParams kp)
     lat.p = kp.lat_pool + kp.lat_off[r];
     const int *prog = kp.prog;
     const int *ie = prog + kp.arg0 + 3 * p;
-    const int n_exec = ie[0], arg = ie[1], cbase = ie[2];
+    const int n_exec = ie[0] & 0xFFFF, arg = ie[1], cbase = ie[2];
     const bool lat_int = kp.backend == 1;
-    if (!lat_int && arg < 0)
+    if (!lat_int && (arg < 0 || ((ie[0] >> 16) & 1)))
         return;
     const long long V1 = (long long)dd.w + 1;
     int *sp = kp.avail_pool + kp.avail_off[r] + (long long)p * V1;
@@ -1584,8 +1592,11 @@ __global__ __launch_bounds__(256) void kmc_check_lists_kernel(KParams kp, int *b
     lat.p = kp.lat_pool + kp.lat_off[r];
     const int *prog = kp.prog;
     const int *ie = prog + kp.arg0 + 3 * p;
-    const int n_exec = ie[0], arg = ie[1], cbase = ie[2];
+    const int n_exec = ie[0] & 0xFFFF, arg = ie[1], cbase = ie[2];
     const bool lat_int = kp.backend == 1;
+    // multi-layer: processes anchored on layers the initialisation does
+    // not touch up can hold while unregistered, as in the reference
+    const bool subset = (ie[0] >> 17) & 1;
     const int V = dd.w;
     const long long V1 = (long long)V + 1;
     const int *sp = kp.avail_pool + kp.avail_off[r] + (long long)p * V1;
@@ -1626,7 +1637,7 @@ __global__ __launch_bounds__(256) void kmc_check_lists_kernel(KParams kp, int *b
                 }
             }
             const bool reg = bp[kp.spuck * c + n_exec] != 0;
-            nbad += en != reg ? 1 : 0;
+            nbad += (subset ? reg && !en : en != reg) ? 1 : 0;
         }
     }
     if (nbad)
@@ -1742,6 +1753,12 @@ struct kmcg_model {
     int mrows = 1;       // mailbox rows per wave
     int merged = 0;      // 1: one apply per step
     int off_seg = 0, off_proc = 0, off_touch = 0, off_nli = 0, off_init = 0, off_nlitab = 0;
+    // a species id a lattice site may hold: 0..S-1, or the null species S of
+    // a multi-layer model (sites of layers that are not populated)
+    bool species_ok(int v) const { return (v >= 0 && v < S) || (nullsp >= 0 && v == nullsp); }
+    // lattice bits per site: 2 when every value, null included, fits
+    // species+1 in 0..3 (single layer: null -1 -> 0), else 8
+    int lattice_bits() const { return (nullsp >= 0 ? S + 1 : S) + 1 <= 4 ? 2 : 8; }
 };
 
 struct kmcg_batch {
@@ -1889,8 +1906,8 @@ int build_prog(kmcg_model *m)
             return fail(KMCG_EUNSUPPORTED, "engine supports 1..8 species");
         if (m->spuck < 1 || m->dim < 1 || m->dim > 3)
             return fail(KMCG_EINVAL, "bad spuck/dimension");
-        if (m->nullsp != -1)
-            return fail(KMCG_EUNSUPPORTED, "multi-lattice models are not supported");
+        if (m->nullsp != -1 && m->nullsp != m->S)
+            return fail(KMCG_EINVAL, "null species must be -1 (one layer) or nr_of_species");
 
         std::vector<SegH> segs;
         std::vector<std::vector<int>> proc_items(m->P);   // 4 ints per item
@@ -1904,7 +1921,7 @@ int build_prog(kmcg_model *m)
                 SegH sg;
                 const int site_type = W(base), old = W(base + 1), nw = W(base + 2);
                 if (old != NO_WRITE) {
-                    if (old < 0 || old >= m->S || nw < 0 || nw >= m->S)
+                    if (!m->species_ok(old) || !m->species_ok(nw))
                         return fail(KMCG_EINVAL, "bad species in routine");
                     sg.writes = {0, 0, 0, site_type, 1 << old, nw};
                 }
@@ -2211,7 +2228,7 @@ int build_prog(kmcg_model *m)
         for (size_t si = 0; si < segs.size(); ++si)
             for (size_t w = 0; w < segs[si].writes.size(); w += 6) {
                 const int n = segs[si].writes[w + 3], nw = segs[si].writes[w + 5];
-                if (n < 1 || n > m->spuck || nw < 0 || nw >= m->S)
+                if (n < 1 || n > m->spuck || !m->species_ok(nw))
                     return fail(KMCG_EINVAL, "bad lattice write");
             }
         // largest offset component any segment reaches from its base cell
@@ -2269,7 +2286,8 @@ int build_prog(kmcg_model *m)
                     coords.push_back(c);
                 pos += 5 + 3 * (size_t)nc;
             }
-            const int K = (int)coords.size(), S = m->S;
+            // a null site (multi-layer) is one more value a coordinate can hold
+            const int K = (int)coords.size(), S = m->nullsp >= 0 ? m->S + 1 : m->S;
             long long entries = 1;
             for (int k = 0; k < K && entries <= 4096; ++k)
                 entries *= S;
@@ -2330,7 +2348,11 @@ int build_prog(kmcg_model *m)
             pg.insert(pg.end(), 8, 0);            // coordinate reads go 8 words ahead
         }
         // per process: anchor site type, its touchup addition op (local_smart)
-        // or its group (lat_int), for the parallel initialisation
+        // or its group (lat_int), for the parallel initialisation.  A
+        // local_smart process anchored on a site type that initialize_state
+        // does not touch up (another layer than the initial one, blob word
+        // 17) starts unregistered everywhere.
+        const int off_itu = W(17);
         m->off_init = (int)pg.size();
         for (int p = 1; p <= m->P; ++p) {
             if (m->backend == 1) {
@@ -2347,7 +2369,10 @@ int build_prog(kmcg_model *m)
                     opw = pg[sb + 3] + OP_W * (int)j;
                     cb = pg[sb + 4];
                 }
-            pg.insert(pg.end(), {n_exec, opw, cb});
+            // bit 16: not registered by the initialisation; bit 17 (multi-
+            // layer): the lists only promise registered => conditions hold
+            const bool touched = off_itu == 0 || W(off_itu + n_exec - 1) != 0;
+            pg.insert(pg.end(), {n_exec | (touched ? 0 : 1 << 16) | (m->nullsp >= 0 ? 1 << 17 : 0), opw, cb});
         }
         pg.insert(pg.end(), {0, 0, 0, 0});   // padding: condition words are read four at a time
         if (pg.size() * sizeof(int) > 64 * 1024)
@@ -2647,7 +2672,7 @@ int kmcg_batch_footprint(const kmcg_model *m, int32_t n, const int32_t *sizes, i
     int rc = sizes_ok(m, n, sizes);
     if (rc)
         return rc;
-    const int bits = (m->S + 1 <= 4) ? 2 : 8;
+    const int bits = m->lattice_bits();
     long long tot = 0;
     for (int i = 0; i < n; ++i) {
         long long X = sizes[3 * i], Y = m->dim >= 2 ? sizes[3 * i + 1] : 1, Z = m->dim >= 3 ? sizes[3 * i + 2] : 1;
@@ -2709,7 +2734,7 @@ int kmcg_batch_create(const kmcg_model *m, int32_t n, const int32_t *sizes, int3
     b->m = m;
     b->device = device;
     b->n = n;
-    b->bits = (m->S + 1 <= 4) ? 2 : 8;
+    b->bits = m->lattice_bits();
     b->X.resize(n); b->Y.resize(n); b->Z.resize(n); b->V.resize(n);
     b->lat_off.resize(n); b->avail_off.resize(n);
     std::vector<int4> dims(n);
@@ -2829,7 +2854,7 @@ static int pack_config(kmcg_batch *b, int r, const int8_t *config, std::vector<u
 {
     const int V = b->V[r];
     for (int s = 0; s < V; ++s)
-        if (config[s] < 0 || config[s] >= b->m->S)
+        if (!b->m->species_ok(config[s]))
             return fail(KMCG_EINVAL, "configuration holds an unknown species");
     packed.assign(lattice_bytes(V, b->bits), 0);
     if (b->bits == 8) {
@@ -3260,7 +3285,7 @@ int kmcg_replace_species(kmcg_batch *b, int32_t r, int32_t site, int32_t old_sp,
     int rc = check_replica(b, r);
     if (rc)
         return rc;
-    if (site < 1 || site > b->V[r] || new_sp < 0 || new_sp >= b->m->S)
+    if (site < 1 || site > b->V[r] || !b->m->species_ok(new_sp))
         return fail(KMCG_EINVAL, "site or species out of range");
     std::vector<int8_t> lat(b->V[r]);
     rc = kmcg_get_lattice(b, r, lat.data());

@@ -36,6 +36,9 @@ class Model:
         _lib.check(lib.kmcg_model_info(self.h, _ptr(info)))
         self.nr_of_proc, self.nr_of_species, self.spuck, self.backend, self.dimension = \
             (int(v) for v in info)
+        # the species id of an unpopulated site: -1 for one layer (never
+        # stored), nr_of_species for several layers (kmos' null_species)
+        self.null_species = int(compiled.null_species)
 
     def __del__(self):
         if getattr(self, "h", None) is not None and _lib._lib is not None:
@@ -66,6 +69,9 @@ class Batch:
         self.n = len(self.sizes)
         self.P = model.nr_of_proc
         self.S = model.nr_of_species
+        # largest species id a lattice site may hold (the null species of
+        # a multi-layer model included)
+        self.max_species = self.S if model.null_species >= 0 else self.S - 1
         self.spuck = model.spuck
         cells = self.sizes.copy()
         if model.dimension < 3:
@@ -88,7 +94,8 @@ class Batch:
 
     def _species_array(self, configs, r0: int, n: int) -> np.ndarray:
         """Lattices of replicas r0..r0+n-1 as one int8 array, after checking
-        that every entry is a species id 0..S-1 (before the cast, so that
+        that every entry is a species id 0..S-1, or S (null) for a
+        multi-layer model (before the cast, so that
         an out-of-range id cannot wrap into a valid one) and that the
         sizes match the replicas' volumes."""
         if r0 < 0 or n < 0 or r0 + n > self.n:
@@ -103,9 +110,9 @@ class Batch:
             if c.size != int(self.volumes[r0 + i]):
                 raise ValueError("lattice of replica %d has %d sites, expected %d"
                                  % (r0 + i, c.size, int(self.volumes[r0 + i])))
-        if flat.size and (flat.min() < 0 or flat.max() >= self.S):
+        if flat.size and (flat.min() < 0 or flat.max() > self.max_species):
             raise ValueError("species ids must lie in 0..%d, got %d..%d"
-                             % (self.S - 1, int(flat.min()), int(flat.max())))
+                             % (self.max_species, int(flat.min()), int(flat.max())))
         return np.ascontiguousarray(flat, dtype=np.int8)
 
     # ---- set-up -----------------------------------------------------------

@@ -1,135 +1,190 @@
-"""Steady-state detection for kMC sampling (kmos/run/steady_state.py).
+"""Steady-state detection for kMC sampling (the role of
+kmos/run/steady_state.py:135-267, ``sample_steady_state``).
 
-The initial-bias test of Rossetti, Li and Qu (Winter Simulation Conference
-2005, doi:10.1109/WSC.2005.1574321): an exponentially weighted moving
-average (EWMA) of each sampled quantity is checked against lower and upper
-control limits (Montgomery, Introduction to Statistical Quality Control,
-7th ed.); the cutoff that keeps the most EWMA points inside the limits marks
-the end of the warm-up.  ``sample_steady_state`` drives a model until no
-quantity asks to scrap more than ``bias_threshold`` of its history; the
-batched form over many replicas is ``sweep.ModelRunner.run_steady_state``.
+Method: the initial-bias test of Rossetti, Li and Qu, "Exploring
+exponentially weighted moving average control charts to determine the
+warm-up period" (Winter Simulation Conference 2005,
+doi:10.1109/WSC.2005.1574321).  Each sampled quantity y_0..y_{n-1} is
+smoothed by the adjusted EWMA (pandas ``Series.ewm(alpha, adjust=True)``)
+
+    z_t = sum_{i<=t} (1-a)^(t-i) y_i  /  sum_{i<=t} (1-a)^(t-i),
+
+and for every candidate end of the warm-up c the points z_c..z_{n-1} are
+tested against a control band around the mean m_c of y_c..y_{n-1} with
+standard deviation s_c (population form):
+
+    |band half-width at t| = L s_c sqrt(a/(2-a) (1 - (1-a)^t)).
+
+(kmos uses the exponent t rather than the textbook 2t of Montgomery's
+EWMA chart; it is kept so that a run is judged steady after the same batch
+as under kmos.)  The cut c >= warm_up that keeps the largest fraction of
+its points strictly inside the band is the warm-up; c/n is the fraction of
+the history to scrap.  A run is steady when no quantity (keys naming a time
+or a step count are not tested) asks to scrap more than ``bias_threshold``.
+
+``sample_steady_state`` drives one model batch by batch until then; the
+batched form over a parameter grid is ``sweep.ModelRunner.run_steady_state``,
+which feeds one ``SteadyStateMonitor`` per replica.
 """
 from __future__ import annotations
 
-import itertools
-
 import numpy as np
+from scipy.signal import lfilter
 
 
+def ewma(y, alpha, adjust=True):
+    """EWMA of the series y: the adjusted (normalised-weight) form by
+    default, the plain recursion z_t = a y_t + (1-a) z_{t-1}, z_0 = y_0,
+    with adjust=False.  Both equal pandas' ``ewm(alpha=...).mean()``."""
+    y = np.asarray(y, dtype=np.float64)
+    if y.size == 0:
+        return y.copy()
+    decay = 1.0 - float(alpha)
+    if adjust:
+        # numerator and normaliser obey the same first-order recursion
+        return lfilter([1.0], [1.0, -decay], y) / lfilter([1.0], [1.0, -decay], np.ones_like(y))
+    # plain form: z_t - decay z_{t-1} = a y_t, started from z_0 = y_0
+    z, _ = lfilter([float(alpha)], [1.0, -decay], y[1:], zi=[decay * y[0]])
+    return np.concatenate([y[:1], z])
+
+
+def band_profile(n, L, alpha):
+    """Control-band half-width per point t = 0..n-1 for unit standard
+    deviation (see the module docstring)."""
+    t = np.arange(n, dtype=np.float64)
+    return L * np.sqrt(alpha / (2.0 - alpha) * (1.0 - (1.0 - alpha) ** t))
+
+
+def inside_fractions(y, L, alpha, cuts):
+    """For every cut c in `cuts`: the fraction of EWMA points z_c..z_{n-1}
+    strictly inside the band around the tail's mean."""
+    y = np.asarray(y, dtype=np.float64)
+    z = ewma(y, alpha)
+    w = band_profile(len(y), L, alpha)
+    out = np.empty(len(cuts))
+    for j, c in enumerate(cuts):
+        tail = y[c:]
+        m, s = tail.mean(), tail.std()
+        half = s * w[c:]
+        zt = z[c:]
+        out[j] = np.count_nonzero((m - half < zt) & (zt < m + half)) / float(tail.size)
+    return out
+
+
+def scrap_fraction(y, L, alpha, warm_up):
+    """Fraction of the history y to discard as warm-up: the first best cut
+    at or after `warm_up`, over len(y).  A constant series has none."""
+    y = np.asarray(y, dtype=np.float64)
+    if np.all(y == y[0]):
+        return 0.0
+    if warm_up >= len(y):
+        raise ValueError("the history (%d batches) must be longer than warm_up (%d)" % (len(y), warm_up))
+    cuts = np.arange(warm_up, len(y))
+    best = cuts[int(np.argmax(inside_fractions(y, L, alpha, cuts)))]
+    return best / float(len(y))
+
+
+# kmos' names for the two pieces users call directly
 def ewma_alpha(y, alpha, prev_ewma=None, adjust=True):
-    """EWMA with pandas' `adjust` weighting (steady_state.py:28-49)."""
-    old_wt_factor = 1.0 - alpha
-    old_wt = 1.0
-    new_wt = 1.0 if adjust else alpha
-    alpha = np.double(alpha)
+    """kmos' entry point: the EWMA of y, or with `prev_ewma` the one-step
+    update a y[-1] + (1-a) prev_ewma."""
     if prev_ewma is not None:
-        return alpha * y[-1] + (1 - alpha) * prev_ewma
-    ewma = np.zeros_like(y, dtype=np.double)
-    ewma[0] = y[0]
-    for i in range(1, len(y)):
-        old_wt *= old_wt_factor
-        ewma[i] = (new_wt * y[i] + old_wt * ewma[i - 1]) / (old_wt + new_wt)
-        if adjust:
-            old_wt += new_wt
-        else:
-            old_wt = 1.0
-    return ewma
-
-
-def lcl_ucl(y, cutoff, L, lambda_factor):
-    """Mean and the lower/upper control limits of the EWMA for data after
-    `cutoff` (steady_state.py:52-73)."""
-    mu0 = np.mean(y[cutoff:]) * np.ones_like(y)
-    sigma = np.std(y[cutoff:])
-    N = len(y)
-    delta = L * sigma * np.sqrt(lambda_factor / (2 - lambda_factor)
-                                * (1 - (1 - lambda_factor) ** np.arange(0.0, N, 1)))
-    return mu0, mu0 - delta, mu0 + delta
-
-
-def p2d(y, cutoff, L, alpha):
-    """Fraction of EWMA points after `cutoff` inside the control limits
-    (steady_state.py:76-86)."""
-    ewma = ewma_alpha(y, alpha)
-    _, lcl, ucl = lcl_ucl(y, cutoff, L, alpha)
-    return ((lcl[cutoff:] < ewma[cutoff:]) & (ewma[cutoff:] < ucl[cutoff:])).sum() / float(len(y) - cutoff)
+        return alpha * y[-1] + (1.0 - alpha) * prev_ewma
+    return ewma(y, alpha, adjust)
 
 
 def get_scrap_fraction(y, L, alpha, warm_up):
-    """Fraction of the history to discard as warm-up (steady_state.py:89-98)."""
-    if (y[0] == y).all():
-        return 0.0
-    D = np.array([p2d(y, cutoff, L, alpha) for cutoff in range(len(y))])
-    return (np.argmax(D[warm_up:]) + warm_up) / float(len(y))
+    """kmos' entry point for ``scrap_fraction``."""
+    return scrap_fraction(y, L, alpha, warm_up)
 
 
-class Detector:
-    """The per-model state of sample_steady_state (steady_state.py:186-267):
-    feed one get_std_sampled_data dict per batch; `done` turns true when the
-    history has passed the test, `result(output)` gives the averages."""
+def _tested(key):
+    return "time" not in key and "step" not in key
+
+
+class SteadyStateMonitor:
+    """Batch-by-batch state of one steady-state run.
+
+    ``add(data)`` takes one ``get_std_sampled_data(..., output='dict')``
+    result and returns whether the run is now steady; the test runs at
+    batch indices b >= warm_up with b % check_frequency == 0.  ``summary``
+    averages the history after the first ``bias_threshold`` of it: keys
+    naming a time keep their last value, keys naming a step count are
+    summed, everything else is averaged with the batches' ``kmc_time`` as
+    weights."""
 
     def __init__(self, L=4, alpha=0.05, bias_threshold=0.15, warm_up=20, check_frequency=10):
-        self.L, self.alpha, self.bias_threshold = L, alpha, bias_threshold
+        self.L, self.alpha = L, alpha
+        self.bias_threshold = bias_threshold
         self.warm_up, self.check_frequency = warm_up, check_frequency
-        self.hist = {}
-        self.batch = -1
+        self.keys = None
+        self.rows = []
         self.done = False
         self.completed_percent = 0.0
         self.critical_key = ""
 
+    @property
+    def batch(self):
+        """0-based index of the last batch added (-1 before the first)."""
+        return len(self.rows) - 1
+
     def add(self, data: dict) -> bool:
-        self.batch += 1
-        batch = self.batch
-        for key, value in data.items():
-            self.hist.setdefault(key, []).append(value)
-        if batch >= self.warm_up and batch % self.check_frequency == 0:
-            max_scrap, critical = 0.0, ""
-            for key, y in self.hist.items():
-                if "time" in key or "step" in key:
-                    continue
-                scrap = get_scrap_fraction(np.array(y), self.L, self.alpha, self.warm_up)
-                if scrap > max_scrap:
-                    max_scrap, critical = scrap, key
-            self.critical_key = critical
-            self.completed_percent = float(1 - max_scrap) / (1 - self.bias_threshold) * 100.0
-            if self.completed_percent >= 100 and batch >= self.warm_up:
-                self.done = True
+        if self.keys is None:
+            self.keys = list(data)
+        self.rows.append([data[k] for k in self.keys])
+        b = self.batch
+        if b < self.warm_up or b % self.check_frequency:
+            return self.done
+        table = np.asarray(self.rows, dtype=np.float64)
+        worst, worst_key = 0.0, ""
+        for j, key in enumerate(self.keys):
+            if not _tested(key):
+                continue
+            f = scrap_fraction(table[:, j], self.L, self.alpha, self.warm_up)
+            if f > worst:
+                worst, worst_key = f, key
+        self.critical_key = worst_key
+        self.completed_percent = (1.0 - worst) / (1.0 - self.bias_threshold) * 100.0
+        self.done = self.completed_percent >= 100.0
         return self.done
 
-    def result(self, header: str, output="str"):
-        start = int(self.batch * self.bias_threshold)
-        hist = {key: np.array(value[start:]) for key, value in self.hist.items()}
+    def summary(self, header: str, output="str"):
+        keep = self.rows[int(self.batch * self.bias_threshold):]
+        cols = {k: np.asarray([row[j] for row in keep]) for j, k in enumerate(self.keys)}
+        weights = cols["kmc_time"]
         data = {}
-        for key, values in hist.items():
-            if "time" in key:
-                data[key] = values[-1]
-            elif "step" in key:
-                data[key] = sum(values)
+        for k, v in cols.items():
+            if "time" in k:
+                data[k] = v[-1]
+            elif "step" in k:
+                data[k] = sum(v)
             else:
-                data[key] = np.average(values, weights=hist["kmc_time"])
+                data[k] = np.average(v, weights=weights)
         if output == "dict":
             return data
         if output == "str":
-            return " ".join(format(data[key.replace("#", "")], ".5e") for key in header.split()) + "\n"
+            return " ".join(format(data[name.lstrip("#")], ".5e") for name in header.split()) + "\n"
         raise UserWarning("Output format %s not defined. I only know 'str' and 'dict'" % output)
 
 
 def sample_steady_state(model, batch_size=1000000, L=4, alpha=0.05, bias_threshold=0.15,
                         tof_method="integ", warm_up=20, check_frequency=10, show_progress=True,
                         make_plots=False, output="str", seed="EWMA"):
-    """Sample `model` in batches of `batch_size` steps (100 samples each)
-    until the EWMA test finds the history free of initial bias
-    (steady_state.py:135-267).  Works with any model that has
-    get_std_sampled_data and get_std_header (kmos_amd.run.KMC_Model)."""
+    """Sample `model` in batches of `batch_size` kMC steps (100 samples per
+    batch) until the run is steady, then return the averaged sampled data
+    ('str': one line in the order of the model's std header; 'dict').
+    Works with any model offering get_std_sampled_data and get_std_header
+    (kmos_amd.run.KMC_Model).  Same parameters as kmos'
+    sample_steady_state (kmos/run/steady_state.py:135-148)."""
     if make_plots:
         raise NotImplementedError("EWMA plots need matplotlib, which this engine does not use")
-    det = Detector(L, alpha, bias_threshold, warm_up, check_frequency)
-    for _ in itertools.count():
-        det.add(model.get_std_sampled_data(100, batch_size, tof_method=tof_method, output="dict"))
-        if show_progress and det.batch >= warm_up and det.batch % check_frequency == 0:
-            print("%3d%% limited by %-40s (%d)" % (int(det.completed_percent), det.critical_key, det.batch))
-        if det.done:
+    mon = SteadyStateMonitor(L, alpha, bias_threshold, warm_up, check_frequency)
+    while True:
+        steady = mon.add(model.get_std_sampled_data(100, batch_size, tof_method=tof_method, output="dict"))
+        b = mon.batch
+        if show_progress and b >= warm_up and b % check_frequency == 0:
+            print("%3d%% limited by %-40s (%d)" % (int(mon.completed_percent), mon.critical_key, b))
+        if steady:
             if show_progress:
-                print("Done after %d batches!" % det.batch)
-            break
-    return det.result(model.get_std_header(), output)
+                print("Done after %d batches!" % b)
+            return mon.summary(model.get_std_header(), output)

@@ -37,7 +37,7 @@ from .engine import Batch, Model
 from .kmc_model import compiled_for_export, load_settings
 from .run import (adjustable_values, interval_tof, parameters_repr, rate_constants, std_header,
                   std_sampled_output, tof_matrix)
-from .steady_state import Detector
+from .steady_state import SteadyStateMonitor
 
 
 def T_grid(T_min, T_max, n):
@@ -311,7 +311,7 @@ class ModelRunner(metaclass=_ModelRunner):
                     b.run(int(init_steps))
                 base, last_tof = self._baseline(b, tofm, ncells, tof_method)
                 params = [adjustable_values(self.point_parameters(pt)) for pt in chunk]
-                dets = [Detector(L, alpha, bias_threshold, warm_up, check_frequency) for _ in chunk]
+                dets = [SteadyStateMonitor(L, alpha, bias_threshold, warm_up, check_frequency) for _ in chunk]
                 stopped = {}
                 live = list(range(len(chunk)))
                 for _ in range(max_batches):
@@ -340,7 +340,7 @@ class ModelRunner(metaclass=_ModelRunner):
                     elif not dets[r].done:
                         lines.append("# %s: no steady state after %d batches\n" % (self.input_line(pt), max_batches))
                     else:
-                        lines.append(dets[r].result(header, "str"))
+                        lines.append(dets[r].summary(header, "str"))
                     results[self.input_line(pt)] = lines[-1]
                 self._write(lines, self.point_parameters(chunk[0]))
             finally:

@@ -132,6 +132,15 @@ typedef struct {
 #define SITES(o, p, k) ((o)->sites[(int64_t)(p) * ((o)->V + 1) + (k)])
 #define BACK(o, p, s) ((o)->back[(int64_t)(p) * ((o)->V + 1) + (s)])
 
+/* Count of avail_sites entries read or written (the algorithmic bytes of a
+ * step are 4 B times this).  Built with -DORC_NO_TOUCH for timing runs, so
+ * that the counter costs nothing there. */
+#ifdef ORC_NO_TOUCH
+#define TOUCH(o, d) ((void)0)
+#else
+#define TOUCH(o, d) ((o)->touched += (d))
+#endif
+
 static int imod(int a, int m) { int r = a % m; return r < 0 ? r + m : r; }
 
 /* lattice.mpy:146-173 calculate_lattice2nr, with the periodic modulo. */
@@ -162,19 +171,19 @@ static void del_proc(orc_t *o, int proc, int site)
     int p = proc - 1;
     int m = BACK(o, p, site);
     int last = o->nr[p];
-    o->touched += 1;                          /* back(site) read */
+    TOUCH(o, 1);                          /* back(site) read */
     if (m < last) {
         int moved = SITES(o, p, last);
         SITES(o, p, m) = moved;
         SITES(o, p, last) = 0;
         BACK(o, p, moved) = m;
-        o->touched += 4;
+        TOUCH(o, 4);
     } else {
         SITES(o, p, m) = 0;
-        o->touched += 1;
+        TOUCH(o, 1);
     }
     BACK(o, p, site) = 0;
-    o->touched += 1;
+    TOUCH(o, 1);
     o->nr[p] -= 1;
 }
 
@@ -185,12 +194,12 @@ static void add_proc(orc_t *o, int proc, int site)
     o->nr[p] += 1;
     SITES(o, p, o->nr[p]) = site;
     BACK(o, p, site) = o->nr[p];
-    o->touched += 2;
+    TOUCH(o, 2);
 }
 
 static int can_do(orc_t *o, int proc, int site)
 {
-    o->touched += 1;
+    TOUCH(o, 1);
     return BACK(o, proc - 1, site) != 0;
 }
 
@@ -219,7 +228,7 @@ static int routine_run(orc_t *o, int r, const int c[4])
         const int32_t *d = dis + 5 * i;
         int s = lattice2nr(o, c[0] + d[1], c[1] + d[2], c[2] + d[3], d[4]);
         if (can_do(o, d[0], s)) {
-            o->touched -= 1;                  /* del re-uses the check's read */
+            TOUCH(o, -1);                  /* del re-uses the check's read */
             del_proc(o, d[0], s);
         }
     }
@@ -326,7 +335,7 @@ static void lat_int_touchup_cell(orc_t *o, const int cell[3])
     int site = lattice2nr(o, cell[0], cell[1], cell[2], 1);
     for (int p = 1; p <= o->P; ++p)
         if (can_do(o, p, site)) {
-            o->touched -= 1;
+            TOUCH(o, -1);
             del_proc(o, p, site);
         }
     int G = HDR(o, 13);
@@ -417,12 +426,15 @@ int orc_init_config(orc_t *o, uint32_t seed, uint32_t stream, const int8_t *conf
     memset(o->back, 0, sizeof(int32_t) * (int64_t)o->P * (o->V + 1));
     for (int s = 1; s <= o->V; ++s)
         o->lat[s] = config ? config[s - 1] : (int8_t)sdef[(s - 1) % o->spuck];
+    /* local_smart touches up the sites of the initial layer only (header
+     * word 17: one flag per site type; every site of a one-layer model) */
+    const int32_t *tinit = o->blob + HDR(o, 17);
     for (int k = 0; k < o->Z; ++k)
         for (int j = 0; j < o->Y; ++j)
             for (int i = 0; i < o->X; ++i) {
                 if (o->backend == 0) {
                     for (int n = 1; n <= o->spuck; ++n)
-                        if (orc_touchup(o, i, j, k, n)) return o->status;
+                        if (tinit[n - 1] && orc_touchup(o, i, j, k, n)) return o->status;
                 } else {
                     orc_touchup(o, i, j, k, 0);
                 }
@@ -507,7 +519,7 @@ static void determine_procsite(orc_t *o, double ran_proc, double ran_site, int *
     int k = (int)(1.0 + ran_site * (double)n);
     if (k > n) k = n;
     *site = SITES(o, *proc - 1, k);
-    o->touched += 1;
+    TOUCH(o, 1);
 }
 
 /* run_proc_nr for both backends */

@@ -11,6 +11,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libkmc_oracle.so")
+# timing build: -O3, no touched-entry counter (the CPU baseline)
+FAST_PATH = os.path.join(HERE, "libkmc_oracle_fast.so")
 SRC_PATH = os.path.join(HERE, "kmc_oracle.c")
 
 STATUS = {0: "ok", 1: "absorbing", 2: "species_mismatch", 3: "del_absent", 4: "bad_arg"}
@@ -18,23 +20,25 @@ STATUS = {0: "ok", 1: "absorbing", 2: "species_mismatch", 3: "del_absent", 4: "b
 
 def build(force=False):
     """Compile the oracle with gcc (no FMA contraction, like the reference's
-    x86-64 build)."""
-    if force or not os.path.exists(LIB_PATH) or \
-            os.path.getmtime(LIB_PATH) < os.path.getmtime(SRC_PATH):
-        subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared",
-                               SRC_PATH, "-o", LIB_PATH, "-lm"])
+    x86-64 build), and its timing variant (-O3, no entry counter)."""
+    for path, flags in ((LIB_PATH, ["-O2"]), (FAST_PATH, ["-O3", "-DORC_NO_TOUCH"])):
+        if force or not os.path.exists(path) or os.path.getmtime(path) < os.path.getmtime(SRC_PATH):
+            subprocess.check_call(["gcc"] + flags + ["-ffp-contract=off", "-fPIC", "-shared",
+                                                     SRC_PATH, "-o", path, "-lm"])
     return LIB_PATH
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def lib(fast=False):
+    """The oracle library; `fast` selects the timing build, whose
+    touched-entry counter always reads 0."""
+    if fast not in _libs:
+        path = FAST_PATH if fast else LIB_PATH
+        if not os.path.exists(path):
             build()
-        L = ctypes.CDLL(LIB_PATH)
+        L = ctypes.CDLL(path)
         vp, i32, i64, u32, u64, dbl = (ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64,
                                        ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double)
         L.orc_create.restype = vp
@@ -89,8 +93,8 @@ def lib():
         L.orc_nr2lattice.argtypes = [vp, i32, vp]
         L.orc_lattice2nr.argtypes = [vp, i32, i32, i32, i32]
         L.orc_lattice2nr.restype = i32
-        _lib = L
-    return _lib
+        _libs[fast] = L
+    return _libs[fast]
 
 
 def _ptr(a):
@@ -114,125 +118,126 @@ def step_uniforms(seed, stream, step):
 class Oracle:
     """One replica of the reference step loop on the CPU."""
 
-    def __init__(self, blob, size):
+    def __init__(self, blob, size, fast=False):
+        self.L = lib(fast)
         blob = np.ascontiguousarray(blob, dtype=np.int32)
         X, Y = int(size[0]), int(size[1]) if len(size) > 1 else 1
         Z = int(size[2]) if len(size) > 2 else 1
         self._blob = blob
-        self.h = lib().orc_create(_ptr(blob), blob.size, X, Y, Z)
+        self.h = self.L.orc_create(_ptr(blob), blob.size, X, Y, Z)
         if not self.h:
             raise ValueError("bad model blob")
-        self.P = lib().orc_nr_of_proc(self.h)
-        self.V = lib().orc_volume(self.h)
+        self.P = self.L.orc_nr_of_proc(self.h)
+        self.V = self.L.orc_volume(self.h)
         self.S = int(blob[4])
         self.spuck = int(blob[6])
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc_free(self.h)
+            self.L.orc_free(self.h)
             self.h = None
 
     def init(self, seed, stream=0, config=None):
         if config is None:
-            return lib().orc_init(self.h, seed & 0xFFFFFFFF, stream & 0xFFFFFFFF)
+            return self.L.orc_init(self.h, seed & 0xFFFFFFFF, stream & 0xFFFFFFFF)
         c = np.ascontiguousarray(config, dtype=np.int8)
         assert c.size == self.V
-        return lib().orc_init_config(self.h, seed & 0xFFFFFFFF, stream & 0xFFFFFFFF, _ptr(c))
+        return self.L.orc_init_config(self.h, seed & 0xFFFFFFFF, stream & 0xFFFFFFFF, _ptr(c))
 
     def set_rates(self, rates):
         r = np.ascontiguousarray(rates, dtype=np.float64)
         assert r.size == self.P
-        lib().orc_set_rates(self.h, _ptr(r))
+        self.L.orc_set_rates(self.h, _ptr(r))
 
     def set_config(self, config):
         c = np.ascontiguousarray(config, dtype=np.int8)
         assert c.size == self.V
-        return lib().orc_set_config(self.h, _ptr(c))
+        return self.L.orc_set_config(self.h, _ptr(c))
 
     def run(self, n, trace=False):
         if trace:
             p = np.zeros(n, np.int32)
             s = np.zeros(n, np.int32)
             t = np.zeros(n, np.float64)
-            k = lib().orc_run(self.h, n, _ptr(p), _ptr(s), _ptr(t))
+            k = self.L.orc_run(self.h, n, _ptr(p), _ptr(s), _ptr(t))
             return k, p[:k], s[:k], t[:k]
-        return lib().orc_run(self.h, n, None, None, None)
+        return self.L.orc_run(self.h, n, None, None, None)
 
     def get_next_step(self):
         p = np.zeros(1, np.int32)
         s = np.zeros(1, np.int32)
-        lib().orc_get_next_step(self.h, _ptr(p), _ptr(s))
+        self.L.orc_get_next_step(self.h, _ptr(p), _ptr(s))
         return int(p[0]), int(s[0])
 
     def run_proc_nr(self, proc, site):
-        return lib().orc_run_proc_nr(self.h, proc, site)
+        return self.L.orc_run_proc_nr(self.h, proc, site)
 
     def touchup(self, x, y, z, n):
-        return lib().orc_touchup(self.h, x, y, z, n)
+        return self.L.orc_touchup(self.h, x, y, z, n)
 
     @property
     def status(self):
-        return lib().orc_status(self.h)
+        return self.L.orc_status(self.h)
 
     @property
     def kmc_time(self):
-        return lib().orc_kmc_time(self.h)
+        return self.L.orc_kmc_time(self.h)
 
     @property
     def kmc_step(self):
-        return lib().orc_kmc_step(self.h)
+        return self.L.orc_kmc_step(self.h)
 
     @property
     def rng_ctr(self):
-        return lib().orc_rng_ctr(self.h)
+        return self.L.orc_rng_ctr(self.h)
 
     @property
     def touched(self):
-        return lib().orc_touched(self.h)
+        return self.L.orc_touched(self.h)
 
     def reset_touched(self):
-        lib().orc_reset_touched(self.h)
+        self.L.orc_reset_touched(self.h)
 
     def lattice(self):
         out = np.zeros(self.V, np.int8)
-        lib().orc_get_lattice(self.h, _ptr(out))
+        self.L.orc_get_lattice(self.h, _ptr(out))
         return out
 
     def nr_of_sites(self):
         out = np.zeros(self.P, np.int32)
-        lib().orc_get_nr(self.h, _ptr(out))
+        self.L.orc_get_nr(self.h, _ptr(out))
         return out
 
     def procstat(self):
         out = np.zeros(self.P, np.int64)
-        lib().orc_get_procstat(self.h, _ptr(out))
+        self.L.orc_get_procstat(self.h, _ptr(out))
         return out
 
     def integ_rates(self):
         out = np.zeros(self.P, np.float64)
-        lib().orc_get_integ(self.h, _ptr(out))
+        self.L.orc_get_integ(self.h, _ptr(out))
         return out
 
     def accum_rates(self):
         out = np.zeros(self.P, np.float64)
-        lib().orc_get_accum(self.h, _ptr(out))
+        self.L.orc_get_accum(self.h, _ptr(out))
         return out
 
     def occupation(self):
         out = np.zeros(self.S * self.spuck, np.float64)
-        lib().orc_get_occupation(self.h, _ptr(out))
+        self.L.orc_get_occupation(self.h, _ptr(out))
         return out.reshape(self.S, self.spuck)
 
     def avail(self):
         sites = np.zeros((self.P, self.V), np.int32)
         back = np.zeros((self.P, self.V), np.int32)
-        lib().orc_get_avail(self.h, _ptr(sites), _ptr(back))
+        self.L.orc_get_avail(self.h, _ptr(sites), _ptr(back))
         return sites, back
 
     def get_state(self, rates):
         """The fields of a .reload snapshot (rates: the ones last set)."""
         sites, back = self.avail()
-        return {"kmc_time": self.kmc_time, "kmc_time_step": lib().orc_kmc_time_step(self.h),
+        return {"kmc_time": self.kmc_time, "kmc_time_step": self.L.orc_kmc_time_step(self.h),
                 "kmc_step": self.kmc_step, "rng_counter": self.rng_ctr, "lattice": self.lattice(),
                 "nr_of_sites": self.nr_of_sites(), "procstat": self.procstat(),
                 "integ_rates": self.integ_rates(), "rates": np.asarray(rates, np.float64),
@@ -242,16 +247,16 @@ class Oracle:
     def msd_init(self, trace_species, hops, disp):
         self._hops = np.ascontiguousarray(hops, np.int32)
         self._disp = np.ascontiguousarray(disp, np.float64)
-        lib().orc_msd_init(self.h, int(trace_species), _ptr(self._hops), _ptr(self._disp))
+        self.L.orc_msd_init(self.h, int(trace_species), _ptr(self._hops), _ptr(self._disp))
 
     def msd_trajectory(self, n):
-        lib().orc_msd_trajectory(self.h, int(n))
+        self.L.orc_msd_trajectory(self.h, int(n))
 
     def msd_zero(self):
-        lib().orc_msd_zero(self.h)
+        self.L.orc_msd_zero(self.h)
 
     def run_displacement(self, n, traj_on=False):
-        return lib().orc_run_displacement(self.h, int(n), 1 if traj_on else 0)
+        return self.L.orc_run_displacement(self.h, int(n), 1 if traj_on else 0)
 
     def msd_state(self):
         ni = np.zeros(1, np.int32)
@@ -259,15 +264,15 @@ class Oracle:
         sa = np.zeros(self.V + 1, np.int32)
         ia = np.zeros(self.V + 1, np.int32)
         dp = np.zeros((self.V + 1, 3), np.float64)
-        lib().orc_msd_get(self.h, _ptr(ni), _ptr(sa), _ptr(ia), _ptr(dp), _ptr(ne), None)
+        self.L.orc_msd_get(self.h, _ptr(ni), _ptr(sa), _ptr(ia), _ptr(dp), _ptr(ne), None)
         ev = np.zeros((max(int(ne[0]), 1), 2), np.int32)
-        lib().orc_msd_get(self.h, _ptr(ni), _ptr(sa), _ptr(ia), _ptr(dp), _ptr(ne), _ptr(ev))
+        self.L.orc_msd_get(self.h, _ptr(ni), _ptr(sa), _ptr(ia), _ptr(dp), _ptr(ne), _ptr(ev))
         return {"nr_of_ions": int(ni[0]), "site_arr": sa, "id_arr": ia, "displacement": dp,
                 "events": ev[: int(ne[0])]}
 
     # autocorrelation bins ---------------------------------------------------
     def acf_alloc(self, nr_types, t_bin, t_f, safety=0, extending=0):
-        lib().orc_acf_alloc(self.h, int(nr_types), float(t_bin), float(t_f), int(safety), int(extending))
+        self.L.orc_acf_alloc(self.h, int(nr_types), float(t_bin), float(t_f), int(safety), int(extending))
         self._nt1 = int(nr_types) + 1
         self._ext = int(float(t_f) * (safety or 2) / float(t_bin) + 1) * (extending or 2)
 
@@ -275,17 +280,17 @@ class Oracle:
         self._pa = None if property_acf is None else np.ascontiguousarray(property_acf, np.int32)
         self._pp = None if product_property is None else np.ascontiguousarray(product_property, np.float64)
         assert self._pa is None or self._pa.size == self.V + 1
-        lib().orc_acf_set(self.h, _ptr(self._pa), _ptr(self._pp))
+        self.L.orc_acf_set(self.h, _ptr(self._pa), _ptr(self._pp))
 
     def acf_init(self, trace_species, hops):
         self._hops = np.ascontiguousarray(hops, np.int32)
-        lib().orc_acf_init(self.h, int(trace_species), _ptr(self._hops))
+        self.L.orc_acf_init(self.h, int(trace_species), _ptr(self._hops))
 
     def acf_zero(self):
-        lib().orc_acf_zero(self.h)
+        self.L.orc_acf_zero(self.h)
 
     def run_acf(self, n, traj_on=False):
-        return lib().orc_run_acf(self.h, int(n), 1 if traj_on else 0)
+        return self.L.orc_run_acf(self.h, int(n), 1 if traj_on else 0)
 
     def acf_state(self):
         i5 = np.zeros(5, np.int32)
@@ -294,7 +299,7 @@ class Oracle:
         cc = np.zeros(self._ext, np.int32)
         buf = np.zeros(self.V + 1, np.float64)
         po = np.zeros(self.V + 1, np.int32)
-        lib().orc_acf_get(self.h, _ptr(cb), _ptr(cc), _ptr(d4), _ptr(i5), _ptr(buf), _ptr(po))
+        self.L.orc_acf_get(self.h, _ptr(cb), _ptr(cc), _ptr(d4), _ptr(i5), _ptr(buf), _ptr(po))
         ext = int(i5[2])
         assert ext == self._ext
         return {"config_bin": cb, "counter_write_in_bin": cc,
@@ -312,6 +317,6 @@ class Oracle:
                 c("rates", np.float64), ig, c("avail_sites", np.int32), c("avail_sites_back", np.int32)]
         assert keep[0].size == self.V and keep[5].size == self.P * self.V
         ctr = st.get("rng_counter")
-        lib().orc_set_state(self.h, float(st["kmc_time"]), float(st.get("kmc_time_step", 0.0)),
+        self.L.orc_set_state(self.h, float(st["kmc_time"]), float(st.get("kmc_time_step", 0.0)),
                             int(st["kmc_step"]), int(st["kmc_step"] if ctr is None else ctr),
                             *[_ptr(a) for a in keep])

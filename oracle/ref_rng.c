@@ -11,6 +11,11 @@
  * returns the same Philox4x32-10 uniforms as the engine and the oracle:
  * draw d belongs to step d/3, slot d%3; slots 0 and 1 are the two 64-bit
  * halves of block (step, 0), slot 2 the low half of block (step, 1).
+ *
+ * The timing variant without the per-step clock read is compiled with
+ * -DCPU_TIME=kmcg_ref_noclock: update_clocks' `call CPU_TIME(runtime)`
+ * (kmos/fortran_src/base.mpy:1144) then lands in kmcg_ref_noclock_, which
+ * returns 0 (walltime is no result of the step loop).
  */
 #include <stdint.h>
 
@@ -64,3 +69,7 @@ void kmcg_ref_uniform_(double *x)
     *x = ((double)(v >> 12) + 0.5) * 0x1p-52;
     g_draw += 1;
 }
+
+/* implicit-interface Fortran call: `call kmcg_ref_noclock(runtime)` with a
+ * default-kind real argument */
+void kmcg_ref_noclock_(float *t) { *t = 0.0f; }

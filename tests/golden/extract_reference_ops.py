@@ -6,11 +6,13 @@ Reads the Fortran that the reference ships as export fixtures (under
 operations each generated routine performs:
 
 local_smart (proclist.f90)
-  routines  put_/take_<species>_<layer>_<site> and touchup_<layer>_<site>:
+  routines  put_/take_<species>_<layer>_<site>, touchup_<layer>_<site> and
+            (multi-layer models) create_/annihilate_<layer>_<site>:
             ["del", proc, [dx,dy,dz,n]] in text order, then
             ["add", proc, [dx,dy,dz,n], [[dx,dy,dz,n,species], ...]] in text
             order, the list being the select-case path that guards the add
-  run_proc  per process: [[routine, [dx,dy,dz,n]], ...]
+  run_proc  per process: [[routine, [dx,dy,dz,n]], ...], with the species
+            argument appended for create_/annihilate_ calls
 lat_int (run_proc_*.f90, nli_*.f90, proclist.f90)
   groups    per group: dels [[group, [dx,dy,dz]]], updates
             [[[dx,dy,dz,n], [old species], new]], adds (same as dels)
@@ -36,6 +38,8 @@ FIXTURES = [
     ("ruo2_local_smart", "tests/export_test/reference_export", "ruo2_default.xml", "local_smart"),
     ("ruo2_lat_int", "tests/export_test/reference_export_lat_int", "ruo2_default.xml", "lat_int"),
     ("cli_local_smart", "tests/test_cli_generated_model", "cli_model.xml", "local_smart"),
+    ("pdopd_local_smart", "tests/export_test/reference_pdopd_local_smart", "pdopd.xml", "local_smart"),
+    ("pdopd_lat_int", "tests/export_test/reference_pdopd_lat_int", "pdopd.xml", "lat_int"),
 ]
 
 VEC = re.compile(r"\(/\s*([^/]*)/\)")
@@ -44,7 +48,10 @@ VEC = re.compile(r"\(/\s*([^/]*)/\)")
 def site_names(model_file):
     sys.path.insert(0, REPO)
     from kmos_amd.model import load_project
-    pt = load_project(os.path.join(HERE, model_file))
+    path = os.path.join(HERE, model_file)
+    if not os.path.exists(path):        # the models the package ships
+        path = os.path.join(REPO, "kmos_amd", "data", model_file)
+    pt = load_project(path)
     return ["%s_%s" % ls for ls in pt.site_list()], pt
 
 
@@ -79,7 +86,7 @@ def parse_local_smart(fdir, names):
     subs = subroutines(text)
     routines = {}
     for name, body in subs.items():
-        if not (name.startswith("put_") or name.startswith("take_") or name.startswith("touchup_")):
+        if not name.startswith(("put_", "take_", "touchup_", "create_", "annihilate_")):
             continue
         own = None
         for i, nm in enumerate(names):
@@ -129,6 +136,12 @@ def parse_local_smart(fdir, names):
             rname = m.group(1)
             own = [i + 1 for i, nm in enumerate(names) if rname.endswith("_" + nm)][0]
             run_proc[cur].append([rname, parse_coord(m.group(2), own, names)])
+            continue
+        m = re.match(r"call ((?:create|annihilate)_\w+)\((.*), (\w+)\)$", s)
+        if m and cur:
+            rname = m.group(1)
+            own = [i + 1 for i, nm in enumerate(names) if rname.endswith("_" + nm)][0]
+            run_proc[cur].append([rname, parse_coord(m.group(2), own, names), m.group(3)])
     return {"routines": routines, "run_proc": run_proc}
 
 

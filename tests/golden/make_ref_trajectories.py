@@ -19,6 +19,8 @@ REPO = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, REPO)
 sys.path.insert(0, os.path.join(REPO, "tests"))
 
+from kmos_amd.compiler import compile_model  # noqa: E402
+from paritylib import pdopd_project, random_config  # noqa: E402
 from refbin import run_reference  # noqa: E402
 
 # (name, reference build, size, seed, stream, steps, rate seed, random config)
@@ -27,26 +29,40 @@ CASES = [
     ("ruo2_local_smart_8x6_config", "ruo2_local_smart", (8, 6), 4, 9, 5000, 12, True),
     ("ruo2_lat_int_10x10", "ruo2_lat_int", (10, 10), 5, 1, 5000, 13, False),
     ("ruo2_lat_int_8x6_config", "ruo2_lat_int", (8, 6), 6, 2, 5000, 14, True),
+    # two layers, create/annihilate: from the initial layer and from a
+    # lattice whose cells hold either layer
+    ("pdopd_local_smart_5x4", "pdopd_local_smart", (5, 4), 7, 3, 4000, 15, False),
+    ("pdopd_local_smart_6x5_config", "pdopd_local_smart", (6, 5), 8, 4, 4000, 16, True),
+    ("pdopd_lat_int_5x4", "pdopd_lat_int", (5, 4), 9, 5, 4000, 17, False),
+    ("pdopd_lat_int_6x5_config", "pdopd_lat_int", (6, 5), 10, 6, 4000, 18, True),
 ]
-P = 36
 
 
-def case_inputs(size, rate_seed, use_config):
+def case_inputs(model, size, rate_seed, use_config):
+    """Rates log-uniform per process; the RuO2 configurations weight the
+    three species 0.3 / 0.4 / 0.3, the pdopd ones hold one layer per cell."""
     g = np.random.default_rng(rate_seed)
-    rates = 10.0 ** g.uniform(-2.0, 9.0, P)
-    config = None
-    if use_config:
-        config = g.choice([0, 1, 2], size=size[0] * size[1] * 2, p=[0.3, 0.4, 0.3]).astype(np.int8)
+    if model == "ruo2":
+        rates = 10.0 ** g.uniform(-2.0, 9.0, 36)
+        config = None
+        if use_config:
+            config = g.choice([0, 1, 2], size=size[0] * size[1] * 2, p=[0.3, 0.4, 0.3]).astype(np.int8)
+        return rates, config
+    pt = pdopd_project()
+    cm = compile_model(pt, "local_smart")
+    rates = 10.0 ** g.uniform(-2.0, 6.0, cm.nr_of_proc)
+    config = random_config(pt, size[0] * size[1] * cm.spuck, g) if use_config else None
     return rates, config
 
 
 def main():
     for name, build, size, seed, stream, steps, rseed, use_cfg in CASES:
-        rates, config = case_inputs(size, rseed, use_cfg)
-        res = run_reference(build, size, seed, stream, steps, rates, P, config=config)
+        model = build.split("_")[0]
+        rates, config = case_inputs(model, size, rseed, use_cfg)
+        res = run_reference(build, size, seed, stream, steps, rates, len(rates), config=config)
         out = os.path.join(HERE, "ref_traj_%s.npz" % name)
         np.savez_compressed(
-            out, size=np.array(size), seed=seed, stream=stream, steps=steps, rates=rates,
+            out, model=model, size=np.array(size), seed=seed, stream=stream, steps=steps, rates=rates,
             config=config if config is not None else np.zeros(0, np.int8),
             proc=res["proc"], site=res["site"], time=res["time"], lattice=res["lattice"],
             nr=res["nr"], procstat=res["procstat"], integ=res["integ"],

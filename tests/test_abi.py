@@ -9,6 +9,7 @@ import pytest
 
 from kmos_amd import _lib, workloads as W
 from kmos_amd.compiler import compile_model
+from paritylib import pdopd_project
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -41,6 +42,7 @@ def _load(cm):
     (W.ruo2_project, "local_smart"), (W.ruo2_project, "lat_int"),
     (W.zgb_project, "local_smart"), (W.pairwise_project, "lat_int"),
     (W.pairwise_project, "local_smart"), (W.quickstart_project, "local_smart"),
+    (pdopd_project, "local_smart"), (pdopd_project, "lat_int"),
 ])
 def test_model_load_on_host(factory, backend):
     cm = compile_model(factory(), backend)
@@ -98,12 +100,13 @@ def test_many_process_models_load_or_are_rejected_cleanly():
 
 
 def test_lattice_species_checked_before_the_int8_cast():
-    """Species ids outside 0..S-1 are refused before the cast to int8, so
-    that 300 or 256 cannot wrap into a valid id (44, 0)."""
+    """Species ids outside 0..S-1 (0..S with a multi-layer model's null
+    species) are refused before the cast to int8, so that 300 or 256
+    cannot wrap into a valid id (44, 0)."""
     import numpy as np
     from kmos_amd.engine import Batch
     b = Batch.__new__(Batch)
-    b.n, b.S, b.volumes = 2, 3, np.array([4, 4], np.int64)
+    b.n, b.S, b.max_species, b.volumes = 2, 3, 2, np.array([4, 4], np.int64)
     ok = b._species_array([np.array([0, 1, 2, 0]), np.array([2, 2, 1, 0])], 0, 2)
     assert ok.dtype == np.int8 and ok.tolist() == [0, 1, 2, 0, 2, 2, 1, 0]
     for bad in ([300, 0, 0, 0], [256, 0, 0, 0], [-1, 0, 0, 0], [3, 0, 0, 0]):
@@ -115,3 +118,7 @@ def test_lattice_species_checked_before_the_int8_cast():
         b._species_array([np.array([0, 0, 0])], 0, 1)          # wrong size
     with pytest.raises(ValueError):
         b._species_array([np.zeros(4, int)], 1, 2)             # beyond the batch
+    b.max_species = 3                                          # two layers: null = S = 3
+    assert b._species_array([np.array([3, 0, 1, 2])], 0, 1).tolist() == [3, 0, 1, 2]
+    with pytest.raises(ValueError):
+        b._species_array([np.array([4, 0, 0, 0])], 0, 1)

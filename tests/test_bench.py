@@ -1,8 +1,11 @@
 """The benchmark's host-side contract, without a GPU: more than one GPU is
 refused with a message, the committed rocprofv3 summaries parse into the
-traffic and LDS figures the JSON line carries, the autotuner summary names
-the default shape's trial, and the algorithmic bytes per kMC step of the ZGB
-workload are the figure its profile and roofline quote."""
+per-kMC-step traffic and LDS figures, the roofline scales them by the run's
+own kMC steps per launch and launch time (so the LDS bandwidth is the
+profile's bytes per step times the kernel's steps/s), the autotuner summary
+names the default shape's trial, the CPU baselines state the cores they
+used, and the algorithmic bytes per kMC step of the ZGB workload are the
+figure its profile quotes."""
 import os
 import subprocess
 import sys
@@ -24,15 +27,28 @@ def test_more_than_one_gpu_is_refused():
 
 
 @pytest.mark.parametrize("workload", WORKLOADS)
-def test_profile_figures_parse(workload):
-    traffic = bench.measured_traffic(workload)
-    lds = bench.measured_lds(workload)
+def test_profile_figures_scale_with_the_run(workload):
+    traffic = bench.profile_figure(workload, "pmc", "traffic_bytes_per_kmc_step")
+    lds = bench.profile_figure(workload, "lds", "lds_bytes_per_kmc_step")
     assert traffic is not None and traffic > 0
     assert lds is not None and lds > traffic        # the LDS moves far more bytes than HBM
-    f = bench.lds_figures(workload, 0.1)
-    assert f["unit"] == "GB/s" and f["peak"] == pytest.approx(256 * 128 * 2.4)
-    assert f["frac"] == pytest.approx(lds / 0.1 / 1e9 / f["peak"])
-    assert bench.measured_lds("no_such_workload") is None and bench.lds_figures("no_such_workload", 1.0) is None
+    for spl, avg_s in ((4.096e7, 0.1), (2.048e7, 0.06)):
+        f = bench.roofline(workload, 200.0, spl, avg_s)
+        kernel_rate = spl / avg_s                     # kMC steps/s inside the kernel
+        assert f["achieved"] == pytest.approx(200.0 * kernel_rate / 1e9)
+        assert f["frac"] == pytest.approx(f["achieved"] / 8000.0)
+        assert f["traffic"] == pytest.approx(traffic * spl)
+        assert f["lds"]["achieved"] == pytest.approx(lds * kernel_rate / 1e9)
+        assert f["lds"]["peak"] == pytest.approx(256 * 128 * 2.4)
+        assert f["lds"]["frac"] == pytest.approx(f["lds"]["achieved"] / f["lds"]["peak"])
+    assert bench.profile_figure("no_such_workload", "lds", "lds_bytes_per_kmc_step") is None
+    none = bench.roofline("no_such_workload", 100.0, 1e6, 1.0)
+    assert none["lds"] is None and none["traffic"] is None
+
+
+def test_host_cores_are_stated():
+    avail, used = bench.host_cores()
+    assert avail == len(os.sched_getaffinity(0)) and used == min(avail, 16)
 
 
 def test_launch_summary():
@@ -48,6 +64,17 @@ def test_zgb_algorithmic_bytes_match_the_profile():
     from kmos_amd import workloads as W
     from kmos_amd.compiler import compile_model
     wl = W.zgb()
-    bps = bench.algorithmic_bytes_per_step(wl, compile_model(wl.project, wl.backend))
+    wl.project_blob = compile_model(wl.project, wl.backend).to_blob()
+    bps = bench.algorithmic_bytes_per_step(wl)
     text = open(os.path.join(REPO, "profiles", "kmc_steps_kernel_zgb_128x128_x4096_pmc.md")).read()
     assert "| %.1f |" % bps in text
+
+
+def test_oracle_cpu_baseline_states_its_cores():
+    from kmos_amd import workloads as W
+    from kmos_amd.compiler import compile_model
+    wl = W.quickstart(steps=20000)
+    wl.project_blob = compile_model(wl.project, wl.backend).to_blob()
+    out = bench.oracle_cpu_baseline(wl, seconds=0.5)
+    assert out["kind"] == "port" and out["value"] > 0
+    assert out["cores_used"] == min(out["cores_available"], 16) == out["cores"]

@@ -11,7 +11,7 @@ import os
 
 import pytest
 
-from kmos_amd.compiler import compile_model, iftree_order, _most_common
+from kmos_amd.compiler import compile_model, iftree_order, multilattice_routine, _most_common
 from kmos_amd.model import load_project
 from paritylib import model_file
 
@@ -33,25 +33,63 @@ def _local_smart_ops(cm, r):
     return ops
 
 
-@pytest.mark.parametrize("name", ["ruo2_local_smart", "cli_local_smart"])
+LOCAL_SMART = ["ruo2_local_smart", "cli_local_smart", "pdopd_local_smart"]
+
+
+@pytest.mark.parametrize("name", LOCAL_SMART)
 def test_local_smart_routines_match_reference(name):
     ref = _golden(name)
     pt = load_project(model_file(ref["model"]))
     cm = compile_model(pt, "local_smart")
-    mine = {r.name: _local_smart_ops(cm, r) for r in cm.routines}
+    # put/take/touchup routines; create/annihilate ones (named with their
+    # species here) are checked below
+    mine = {r.name: _local_smart_ops(cm, r) for r in cm.routines if ":" not in r.name}
+    theirs = {k: v for k, v in ref["routines"].items() if not k.startswith(("create_", "annihilate_"))}
     # the reference only writes put/take routines for non-default species
-    assert set(ref["routines"]) == set(mine)
-    for rname, ops in ref["routines"].items():
+    assert set(theirs) == set(mine)
+    for rname, ops in theirs.items():
         assert mine[rname] == ops, rname
 
 
-@pytest.mark.parametrize("name", ["ruo2_local_smart", "cli_local_smart"])
+def test_multilattice_routines_match_reference():
+    """Every create_/annihilate_<layer>_<site> the reference writes for the
+    25-site, two-layer pdopd model (kmos/io/__init__.py:2445-2560): the op
+    list does not depend on the species argument, so it is compared for
+    each species; the compiled model carries exactly the (site, species)
+    pairs its processes' '^'/'$' actions use."""
+    ref = _golden("pdopd_local_smart")
+    pt = load_project(model_file(ref["model"]))
+    cm = compile_model(pt, "local_smart")
+    multi = {k: v for k, v in ref["routines"].items() if k.startswith(("create_", "annihilate_"))}
+    assert len(multi) == 2 * len(pt.site_list())
+    sid = pt.species_ids()
+    null = pt.null_species()
+    assert null == len(pt.species) == 4
+    for rname, ops in multi.items():
+        op, rest = rname.split("_", 1)
+        lname, sname = [ls for ls in pt.site_list() if "%s_%s" % ls == rest][0]
+        for species in sorted(sid):
+            r = multilattice_routine(pt, op, lname, sname, species)
+            assert _local_smart_ops(cm, r) == ops, (rname, species)
+            assert (r.old_species, r.new_species) == ((null, sid[species]) if op == "create" else (sid[species], null))
+    used = {(r.name.split(":")[0], r.name.split(":")[1]) for r in cm.routines if ":" in r.name}
+    calls = {(c[0], c[2]) for acts in ref["run_proc"].values() for c in acts if len(c) == 3}
+    assert used == calls
+
+
+@pytest.mark.parametrize("name", LOCAL_SMART)
 def test_local_smart_run_proc_matches_reference(name):
     ref = _golden(name)
     pt = load_project(model_file(ref["model"]))
     cm = compile_model(pt, "local_smart")
     for p, acts in enumerate(cm.proc_actions):
-        mine = [[cm.routines[r].name, list(s)] for r, s in acts]
+        mine = []
+        for r, s in acts:
+            rname = cm.routines[r].name
+            if ":" in rname:            # create/annihilate: routine, site, species
+                mine.append([rname.split(":")[0], list(s), rname.split(":")[1]])
+            else:
+                mine.append([rname, list(s)])
         assert mine == ref["run_proc"][cm.process_names[p]], cm.process_names[p]
 
 
@@ -79,8 +117,9 @@ def _nli_leaves(cm, g, sp):
     return out
 
 
-def test_lat_int_groups_match_reference():
-    ref = _golden("ruo2_lat_int")
+@pytest.mark.parametrize("name", ["ruo2_lat_int", "pdopd_lat_int"])
+def test_lat_int_groups_match_reference(name):
+    ref = _golden(name)
     pt = load_project(model_file(ref["model"]))
     cm = compile_model(pt, "lat_int")
     gnames = [g.name for g in cm.groups]
@@ -88,7 +127,8 @@ def test_lat_int_groups_match_reference():
     assert gnames == ref["touchup_groups"]
     for p, g in enumerate(cm.proc_group):
         assert ref["dispatch"][cm.process_names[p]] == gnames[g]
-    sp = cm.species_names
+    # the null species of a multi-layer model is written by its name
+    sp = cm.species_names + (["null_species"] if cm.null_species >= 0 else [])
     for gi, g in enumerate(cm.groups):
         rg = ref["groups"][g.name]
         mods = [[gnames[og], list(off)] for og, off in g.modified]
@@ -98,7 +138,7 @@ def test_lat_int_groups_match_reference():
         assert ups == rg["updates"], g.name
         # nli: the non-default leaves in text order
         ref_leaves = [(path, res) for path, res in ref["nli"][g.name] if res != "0"]
-        mine = _nli_leaves(cm, gi, sp)
+        mine = _nli_leaves(cm, gi, cm.species_names)
         assert [(p, r) for p, r in mine] == [([c[:4] + [sorted(c[4])] for c in p], r)
                                              for p, r in ref_leaves], g.name
 

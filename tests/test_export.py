@@ -47,14 +47,20 @@ def test_tampered_program_is_rejected(tmp_path):
         compiled_for_export(d)
 
 
-def test_multi_layer_model_is_rejected_clearly():
-    """Multi-layer models (the reference's pdopd.xml: Pd100 and PdO layers,
-    create/annihilate actions, kmos/io/__init__.py:375-404) are outside the
-    path: compiling one fails with a clear message for both backends."""
-    from kmos_amd.compiler import compile_model
-    from kmos_amd.model import load_project
+@pytest.mark.parametrize("backend", ["local_smart", "lat_int"])
+def test_multi_layer_model_exports(tmp_path, backend):
+    """The reference's two-layer pdopd.xml (Pd100 and PdO layers, 25 sites
+    per cell, create/annihilate actions, kmos/io/__init__.py:375-404,
+    2445-2560) exports and re-compiles from its settings to the same
+    program; its null species is nr_of_species and only the initial
+    layer's sites start populated."""
     pt = load_project(os.path.join(GOLDEN, "pdopd.xml"))
     assert len(pt.layers) == 2
-    for backend in ("local_smart", "lat_int"):
-        with pytest.raises(NotImplementedError, match="multi-layer"):
-            compile_model(pt, backend)
+    d = export_source(os.path.join(GOLDEN, "pdopd.xml"), str(tmp_path / backend), backend)
+    cm = compiled_for_export(d)
+    np.testing.assert_array_equal(cm.to_blob(), compile_model(pt, backend).to_blob())
+    assert cm.null_species == len(pt.species) == 4
+    layers = [lname for lname, _ in pt.site_list()]
+    assert [n for n in range(cm.spuck) if cm.site_default[n] != cm.null_species] == \
+        [n for n in range(cm.spuck) if layers[n] == pt.default_layer] == \
+        [n for n in range(cm.spuck) if cm.site_init[n]]

@@ -3,7 +3,7 @@
 Each case runs a few replicas of different sizes, seeds and rate sets in
 one batch on the GPU and the same replicas one by one on the CPU oracle,
 then compares the complete state (lattice, nr_of_sites, both avail_sites
-tables, procstat) exactly and kmc_time / integ_rates within 1e-12 / 1e-9
+tables, procstat) exactly and kmc_time / integ_rates within 1e-12
 relative (the GPU's log may differ from glibc's by an ulp).
 """
 import numpy as np
@@ -11,14 +11,13 @@ import pytest
 
 from kmos_amd import workloads as W
 from kmos_amd.compiler import compile_model
-from paritylib import compare_clock, compare_state, oracle_replica
+from paritylib import compare_clock, compare_state, oracle_replica, pdopd_project, random_config
 
 pytestmark = pytest.mark.gpu
 
 
 def _random_config(pt, V, rng):
-    ids = sorted(pt.species_ids().values())
-    return rng.choice(ids, size=V).astype(np.int8)
+    return random_config(pt, V, rng)
 
 
 CASES = [
@@ -30,6 +29,12 @@ CASES = [
     ("pairwise_lat_int", W.pairwise_project, "lat_int", [(8, 8), (12, 10)], 3000, True),
     ("pairwise_local_smart", W.pairwise_project, "local_smart", [(8, 8), (12, 10)], 3000, True),
     ("zgb_lat_int", W.zgb_project, "lat_int", [(8, 8), (10, 6)], 3000, False),
+    # two layers, create/annihilate actions: the reference's pdopd model,
+    # from its initial layer and from mixed-layer lattices
+    ("pdopd_local_smart", pdopd_project, "local_smart", [(3, 3), (5, 4), (8, 8)], 3000, False),
+    ("pdopd_local_smart_config", pdopd_project, "local_smart", [(3, 3), (5, 4), (8, 8)], 3000, True),
+    ("pdopd_lat_int", pdopd_project, "lat_int", [(3, 3), (5, 4), (8, 8)], 3000, False),
+    ("pdopd_lat_int_config", pdopd_project, "lat_int", [(3, 3), (5, 4), (8, 8)], 3000, True),
 ]
 
 

@@ -21,7 +21,7 @@ import pytest
 from kmos_amd import workloads as W
 from kmos_amd.compiler import compile_model
 from kmos_amd.model import load_project
-from paritylib import oracle_replica
+from paritylib import oracle_replica, pdopd_project, random_config
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -64,7 +64,8 @@ GOLDEN_TRAJ = sorted(glob.glob(os.path.join(GOLDEN, "ref_traj_*.npz")))
 def test_oracle_matches_reference_golden(oracle, path):
     d = np.load(path)
     backend = "lat_int" if "lat_int" in path else "local_smart"
-    cm = compile_model(W.ruo2_project(), backend)
+    pt = pdopd_project() if str(d["model"]) == "pdopd" else W.ruo2_project()
+    cm = compile_model(pt, backend)
     config = d["config"] if d["config"].size else None
     o = oracle_replica(oracle, cm, tuple(d["size"]), int(d["seed"]), int(d["stream"]), d["rates"], config)
     k, p, s, t = o.run(int(d["steps"]), trace=True)
@@ -86,27 +87,35 @@ def _ref_available(name):
     return driver_path(name) is not None
 
 
-@pytest.mark.parametrize("backend", ["local_smart", "lat_int"])
-def test_oracle_matches_live_reference(oracle, backend):
-    name = "ruo2_%s" % backend
+@pytest.mark.parametrize("model,backend", [("ruo2", "local_smart"), ("ruo2", "lat_int"),
+                                           ("pdopd", "local_smart"), ("pdopd", "lat_int")])
+def test_oracle_matches_live_reference(oracle, model, backend):
+    """Fresh seeds, sizes, rates (some processes off) and starting lattices
+    against the reference build: RuO2 (one layer) and the two-layer pdopd
+    model with create/annihilate actions."""
+    name = "%s_%s" % (model, backend)
     if not _ref_available(name):
         pytest.skip("reference build %s absent (oracle/build_ref.py needs /root/reference)" % name)
     from refbin import run_reference
-    cm = compile_model(W.ruo2_project(), backend)
+    pt = W.ruo2_project() if model == "ruo2" else pdopd_project()
+    cm = compile_model(pt, backend)
     g = np.random.default_rng(99)
     for trial in range(3):
-        size = (int(g.integers(4, 13)), int(g.integers(4, 13)))
-        rates = 10.0 ** g.uniform(-2, 9, cm.nr_of_proc)
+        lo = 4 if model == "ruo2" else 3
+        size = (int(g.integers(lo, 13 - lo)), int(g.integers(lo, 13 - lo)))
+        rates = 10.0 ** g.uniform(-2, 9 if model == "ruo2" else 6, cm.nr_of_proc)
         rates[g.integers(0, cm.nr_of_proc, 3)] = 0.0       # some processes off
-        config = g.choice([0, 1, 2], size=size[0] * size[1] * 2).astype(np.int8) if trial else None
+        config = random_config(pt, size[0] * size[1] * cm.spuck, g) if trial else None
         seed, stream = int(g.integers(0, 2**31)), int(g.integers(0, 1000))
         ref = run_reference(name, size, seed, stream, 4000, rates, cm.nr_of_proc, config=config)
         bulk = run_reference(name, size, seed, stream, 4000, rates, cm.nr_of_proc, config=config, mode=1)
         o = oracle_replica(oracle, cm, size, seed, stream, rates, config)
         k, p, s, t = o.run(4000, trace=True)
+        assert k == len(ref["proc"])
         np.testing.assert_array_equal(p, ref["proc"])
         np.testing.assert_array_equal(s, ref["site"])
         np.testing.assert_array_equal(t, ref["time"])
+        np.testing.assert_array_equal(o.lattice(), ref["lattice"])
         sites, back = o.avail()
         np.testing.assert_array_equal(sites, ref["sites"])
         np.testing.assert_array_equal(back, ref["back"])
@@ -178,6 +187,40 @@ def test_oracle_bookkeeping_invariants(oracle, factory, backend, size):
     for step in range(300):
         reg = _check_bookkeeping(o, cm.nr_of_proc, o.V)
         assert reg == _enabled_bruteforce(pt, cm, o.lattice(), size), "step %d" % step
+        if o.run(1) == 0:
+            break
+
+
+@pytest.mark.parametrize("backend", ["local_smart", "lat_int"])
+def test_oracle_bookkeeping_invariants_two_layers(oracle, backend):
+    """The two-layer pdopd model.  initialize_state touches up the sites of
+    the initial layer only (proclist_generic_subroutines.mpy:279-300), so
+    after init a local_smart process anchored on the other layer is
+    registered nowhere, even where its conditions hold; every other
+    (process, site) is registered exactly where the conditions hold.  After
+    every later step the lists keep their invariant and register only
+    pairs whose conditions hold."""
+    pt = pdopd_project()
+    cm = compile_model(pt, backend)
+    size = (4, 3)
+    g = np.random.default_rng(8)
+    rates = 10.0 ** g.uniform(-1, 3, cm.nr_of_proc)
+    o = oracle_replica(oracle, cm, size, 2, 0, rates)
+    reg = _check_bookkeeping(o, cm.nr_of_proc, o.V)
+    en = _enabled_bruteforce(pt, cm, o.lattice(), size)
+    if backend == "local_smart":
+        init_layer = {p + 1 for p, proc in enumerate(pt.processes)
+                      if proc.executing_coord().layer == pt.default_layer}
+        assert init_layer != set(range(1, cm.nr_of_proc + 1))
+        assert reg == {(p, s) for p, s in en if p in init_layer}
+    else:
+        assert reg == en
+    assert reg
+    config = random_config(pt, o.V, g)
+    o.set_config(config)
+    for step in range(300):
+        reg = _check_bookkeeping(o, cm.nr_of_proc, o.V)
+        assert reg <= _enabled_bruteforce(pt, cm, o.lattice(), size), "step %d" % step
         if o.run(1) == 0:
             break
 

@@ -9,9 +9,10 @@ averaging and the steady-state test.
 * the parameters block of the .dat header in the reference's layout
   (kmos/run/__init__.py:1670-1692);
 * std_sampled_output against a direct weighted mean;
-* ewma_alpha against pandas' adjusted EWM, the scrap fraction on constant
-  and on step signals, and Detector against sample_steady_state on the same
-  data stream.
+* the EWMA against pandas' EWM (adjusted and plain), the control band and
+  the per-cut inside fractions against a hand-worked 4-point series, the
+  scrap fraction on constant and on step signals, and SteadyStateMonitor
+  against sample_steady_state on the same data stream.
 """
 import numpy as np
 import pytest
@@ -98,8 +99,27 @@ def test_ewma_matches_pandas():
     pd = pytest.importorskip("pandas")
     y = np.random.default_rng(3).normal(size=200).cumsum()
     for alpha in (0.05, 0.3):
-        ref = pd.Series(y).ewm(alpha=alpha, adjust=True).mean().to_numpy()
-        np.testing.assert_allclose(SS.ewma_alpha(y, alpha), ref, rtol=1e-12, atol=1e-12)
+        for adjust in (True, False):
+            ref = pd.Series(y).ewm(alpha=alpha, adjust=adjust).mean().to_numpy()
+            np.testing.assert_allclose(SS.ewma(y, alpha, adjust), ref, rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(SS.ewma_alpha(y, alpha), SS.ewma(y, alpha), rtol=0, atol=0)
+
+
+def test_band_and_inside_fractions_by_hand():
+    # half-width L sqrt(a/(2-a) (1-(1-a)^t)) at a = 0.5, L = 1: 0, sqrt(1/6), sqrt(1/4), sqrt(7/24)
+    np.testing.assert_allclose(SS.band_profile(4, 1.0, 0.5), [0.0, 6 ** -0.5, 0.5, (7 / 24) ** 0.5],
+                               rtol=1e-15)
+    # y = 0,1,0,1 at a = 0.5: z = 0, 2/3, 2/7, 2/3 (weights 1, 1/2, 1/4, 1/8 normalised)
+    y = np.array([0.0, 1.0, 0.0, 1.0])
+    np.testing.assert_allclose(SS.ewma(y, 0.5), [0.0, 2 / 3, 2 / 7, 2 / 3], rtol=1e-15)
+    # cut 0: m = s = 1/2, bands 0, .204, .25, .270 -> z1, z2, z3 inside, z0 not (strict)
+    # cut 1: m = 2/3, s = .471 -> z1 and z3 inside, z2 (.381 away) outside the .236 band
+    # cut 2: m = s = 1/2 -> both inside; cut 3: s = 0 -> nothing inside
+    np.testing.assert_allclose(SS.inside_fractions(y, 1.0, 0.5, [0, 1, 2, 3]), [0.75, 2 / 3, 1.0, 0.0])
+    assert SS.scrap_fraction(y, 1.0, 0.5, 0) == 0.5          # best cut 2 of 4
+    assert SS.scrap_fraction(y, 1.0, 0.5, 3) == 0.75         # only cut 3 allowed
+    with pytest.raises(ValueError):
+        SS.scrap_fraction(y, 1.0, 0.5, 4)
 
 
 def test_scrap_fraction():
@@ -127,14 +147,14 @@ class _FakeModel:
                 "simulated_time": 2.0 * self.n, "kmc_steps": sample_size}
 
 
-def test_detector_is_sample_steady_state():
+def test_monitor_is_sample_steady_state():
     a = SS.sample_steady_state(_FakeModel(), batch_size=100, warm_up=5, check_frequency=3, show_progress=False)
     m = _FakeModel()
-    det = SS.Detector(warm_up=5, check_frequency=3)
+    det = SS.SteadyStateMonitor(warm_up=5, check_frequency=3)
     while not det.add(m.get_std_sampled_data(100, 100)):
         pass
-    assert det.result(m.get_std_header()) == a
+    assert det.summary(m.get_std_header()) == a
     assert det.batch >= 5 and det.batch % 3 == 0
-    d = det.result(m.get_std_header(), "dict")
+    d = det.summary(m.get_std_header(), "dict")
     start = int(det.batch * 0.15)
     assert d["kmc_steps"] == 100 * (det.batch + 1 - start)
