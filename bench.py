@@ -456,6 +456,37 @@ def serving_saturated(batched_rate, n=4096, rate=1e5, seed=3, ipl=8):
     return out
 
 
+PHASES = ("select", "site_read", "post", "replay", "rest")
+
+
+def phase_profile(engine, wl, K, warmup):
+    """The contract workload run by the phase-counting build of the engine
+    (libkmcgpu_phases.so): shader-clock ticks per kMC step in each phase of
+    the step loop, over all replicas and per lattice size, and the ticks
+    per microsecond implied by the kernel time."""
+    cm = compile_model(wl.project, wl.backend)
+    model = engine.Model(cm)
+    b, wall, executed, kernel_ms, chunk, _ = run_batched(engine, model, wl, K, warmup, tune=False)
+    ph = b.phase_profile().astype(np.float64)
+    b.close()
+    steps = ph[:, 5]
+    out = {"workload": wl.name, "replicas": wl.n, "kmc_steps_per_launch": executed / K,
+           "avg_launch_ms": float(np.mean(kernel_ms)), "gpu_steps_per_s": executed / wall}
+    ticks = ph[:, :5].sum(axis=1)
+    # a wave runs for about the whole launch: its ticks over the launch time
+    out["ticks_per_us"] = float(np.median(ticks / K / (np.mean(kernel_ms) * 1e3)))
+
+    def table(sel):
+        tot = ph[sel, :5].sum(axis=0) / max(steps[sel].sum(), 1.0)
+        return {name: {"ticks_per_step": float(v), "frac": float(v / tot.sum())} for name, v in zip(PHASES, tot)}
+
+    out["all"] = table(np.ones(wl.n, bool))
+    vol = wl.sizes.prod(axis=1)
+    out["by_size"] = {"%dx%d" % tuple(wl.sizes[np.nonzero(vol == v)[0][0]][:2]): table(vol == v)
+                      for v in np.unique(vol)}
+    return out
+
+
 def calibrate_copy(nbytes=1 << 30, reps=5):
     """A plain device-to-device copy of known size: what FETCH_SIZE and
     WRITE_SIZE read for it calibrates the step kernel's counters."""
@@ -492,9 +523,14 @@ def main():
                     help="default launch shape and serving chunk length instead of the autotuner's")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of 1 GiB five times (calibrates HBM byte counters)")
+    ap.add_argument("--phases", action="store_true",
+                    help="only the phase profile of the step loop on the workload (loads the "
+                         "phase-counting build libkmcgpu_phases.so), one JSON line")
     a = ap.parse_args()
     if a.calibrate_copy:
         return calibrate_copy()
+    if a.phases:
+        os.environ["KMCG_PHASES"] = "1"
     if a.gpus != 1:
         print("bench.py measures one GPU: replicas are independent and the engine runs "
               "them all on a single device; run with --gpus 1", file=sys.stderr)
@@ -514,6 +550,9 @@ def main():
         print("no GPU visible", file=sys.stderr)
         return 1
     K = max(1, a.steps)
+    if a.phases:
+        print(json.dumps(phase_profile(engine, wl, K, a.warmup)))
+        return 0
     head, model, cm, wall, executed, chunk, choice = batched_leg(engine, wl, K, a.warmup, not a.no_tune,
                                                                  cpu=False)
     _progress("batched %s: %d launches timed" % (wl.name, K))

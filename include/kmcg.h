@@ -8,7 +8,9 @@
  * codes (KMCG_REPLICA_*).  No entry point aborts the process.
  *
  * Numbering follows the reference Fortran: processes 1..P, sites 1..V,
- * species 0..S-1 in name order, null species -1.
+ * species 0..S-1 in name order, null species -1 (one layer) or S (several
+ * layers: the sites of layers that are not populated,
+ * kmos/fortran_src/proclist_constants.mpy:61-68).
  *
  * Reference interfaces these replace (paths relative to the kmos tree):
  *   f2py module kmc_model.{base,lattice,proclist}  kmos/utils/__init__.py:406-521
@@ -20,7 +22,9 @@
  *   1 lat_int), 3 nr_of_proc, 4 nr_of_species, 5 null_species, 6 spuck,
  *   7 dimension, 8 off_site_default, 9 off_proc_table, 10 n_routines,
  *   11 off_routine_table, 12 off_touchup, 13 n_groups, 14 off_group_table,
- *   15 min_size (cells per periodic dimension), 16 total words.  Section layouts are documented in
+ *   15 min_size (cells per periodic dimension), 16 total words,
+ *   17 off_site_init (per site type: 1 = initialize_state touches it up,
+ *   the sites of the initial layer).  Section layouts are documented in
  *   kmos_amd/compiler.py (_to_blob).
  */
 #ifndef KMCG_H
@@ -133,6 +137,14 @@ int kmcg_set_graph(kmcg_batch *b, int32_t enable);
 /* Milliseconds the last kmcg_run took on the device (HIP events around the
  * kernel on the batch's stream). */
 int kmcg_last_run_ms(kmcg_batch *b, float *ms);
+/* Phase profile of the step loop, for replicas r0..r0+n-1: out[n][8] holds
+ * the shader-clock ticks each replica's wave spent selecting (accumulate,
+ * clock, search), reading its site, posting the process's ops, replaying
+ * the lists and on the rest of its steps, then the steps counted; the
+ * counters are cleared after the read.  Only a library built with
+ * -DKMCG_PHASES=1 (libkmcgpu_phases.so) counts; the plain library returns
+ * KMCG_EUNSUPPORTED.  No counterpart in the reference (a profiling aid). */
+int kmcg_phase_profile(kmcg_batch *b, int32_t r0, int32_t n, uint64_t *out);
 
 /* proclist.get_next_kmc_step: draws one step's uniforms and returns the
  * process and site it would pick, passing ran_time as the site uniform

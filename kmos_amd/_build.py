@@ -7,6 +7,8 @@ REPO = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc", "kmcg_engine.hip")
 DEPS = [SRC, os.path.join(HERE, "csrc", "kmcg_device.h"), os.path.join(REPO, "include", "kmcg.h")]
 LIB = os.path.join(HERE, "libkmcgpu.so")
+# the same source with the step loop's phase counters compiled in
+PHASES_LIB = os.path.join(HERE, "libkmcgpu_phases.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -20,19 +22,29 @@ FLAGS = [
 
 
 def build(force=True, verbose=False):
-    """Compile the engine; always recompiles unless force=False and the
-    library is newer than its sources."""
-    if not force and os.path.exists(LIB) and \
-            all(os.path.getmtime(LIB) >= os.path.getmtime(d) for d in DEPS):
-        return LIB
-    tmp = LIB + ".tmp%d" % os.getpid()
-    cmd = [HIPCC] + FLAGS + [SRC, "-o", tmp]
-    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stdout)
-    if verbose:
-        print(r.stdout)
-    os.replace(tmp, LIB)
+    """Compile the engine and its phase-profiling variant (in parallel);
+    always recompiles unless force=False and both libraries are newer than
+    their sources.  Returns the engine's path."""
+    jobs = []
+    for lib, extra in ((LIB, []), (PHASES_LIB, ["-DKMCG_PHASES=1"])):
+        if not force and os.path.exists(lib) and \
+                all(os.path.getmtime(lib) >= os.path.getmtime(d) for d in DEPS):
+            continue
+        tmp = lib + ".tmp%d" % os.getpid()
+        cmd = [HIPCC] + FLAGS + extra + [SRC, "-o", tmp]
+        jobs.append((lib, tmp, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                                text=True)))
+    failed = []
+    for lib, tmp, proc in jobs:
+        out = proc.communicate()[0]
+        if proc.returncode != 0:
+            failed.append(out)
+            continue
+        if verbose:
+            print(out)
+        os.replace(tmp, lib)
+    if failed:
+        raise RuntimeError("hipcc failed:\n" + "\n".join(failed))
     return LIB
 
 

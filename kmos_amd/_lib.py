@@ -8,6 +8,9 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libkmcgpu.so")
+# the same engine built with its step-loop phase counters (KMCG_PHASES=1);
+# loaded instead when the environment sets KMCG_PHASES=1
+PHASES_PATH = os.path.join(HERE, "libkmcgpu_phases.so")
 
 KMCG_OK = 0
 ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "EUNSUPPORTED"}
@@ -36,6 +39,7 @@ SIGNATURES = [
     ("kmcg_set_graph", _i32, [_vp, _i32]),
     ("kmcg_check_lists", _i32, [_vp, _i32, _i32, _vp]),
     ("kmcg_last_run_ms", _i32, [_vp, _vp]),
+    ("kmcg_phase_profile", _i32, [_vp, _i32, _i32, _vp]),
     ("kmcg_get_next_step", _i32, [_vp, _i32, _vp, _vp]),
     ("kmcg_run_proc_nr", _i32, [_vp, _i32, _i32, _i32]),
     ("kmcg_touchup", _i32, [_vp, _i32, _i32, _i32, _i32, _i32]),
@@ -81,10 +85,11 @@ def load():
     """Load libkmcgpu.so; raise if it has not been built."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise KmcgError("libkmcgpu.so is not built (run __graft_entry__.build() "
-                            "or python -m kmos_amd._build)")
-        lib = ctypes.CDLL(LIB_PATH)
+        path = PHASES_PATH if os.environ.get("KMCG_PHASES") == "1" else LIB_PATH
+        if not os.path.exists(path):
+            raise KmcgError("%s is not built (run __graft_entry__.build() "
+                            "or python -m kmos_amd._build)" % os.path.basename(path))
+        lib = ctypes.CDLL(path)
         for name, rt, args in SIGNATURES:
             f = getattr(lib, name)
             f.restype = rt

@@ -149,7 +149,13 @@ struct KParams {
     double *acf_d;                 // [8] per replica: sum(buffer_acf), t_o, wrap_count, t_bin
     int *acf_i;                    // [8] per replica: bin_index, nr_of_bins, extended bins,
                                    // nr_of_ions_before_kmc_step_acf, nr_of_ions
+    // phase profile (library built with KMCG_PHASES, else null): per
+    // replica 8 counters, shader-clock ticks spent in select, site read,
+    // post, replay and the rest of the step, then the steps counted
+    unsigned long long *phase;
 };
+
+constexpr int PHASE_W = 8;
 
 constexpr int MSD_HOP_W = 9;
 

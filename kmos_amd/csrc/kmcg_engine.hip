@@ -32,6 +32,27 @@ using namespace kmcg;
 // ===========================================================================
 namespace {
 
+// Phase profile of the step loop (a build with -DKMCG_PHASES=1 only): each
+// wave adds the shader-clock ticks between marks to one of five counters
+// (0 select, 1 site read, 2 post, 3 replay, 4 the rest of the step).  The
+// clock reads wait for outstanding scalar and LDS work, so the instrumented
+// loop runs a little slower than the plain one.
+#ifndef KMCG_PHASES
+#define KMCG_PHASES 0
+#endif
+#if KMCG_PHASES
+#define PH_MARK(W, i)                                                  \
+    do {                                                               \
+        const unsigned long long t_ = __builtin_amdgcn_s_memtime();     \
+        (W).tph[i] += t_ - (W).tmark;                                  \
+        (W).tmark = t_;                                                \
+    } while (0)
+#else
+#define PH_MARK(W, i) \
+    do {              \
+    } while (0)
+#endif
+
 constexpr int MODE_INIT = 0;       // touchups in k, j, i order (initialize_state)
 constexpr int MODE_ADJUST = 1;     // touchups with x outermost (_adjust_database)
 constexpr int MODE_TOUCHUP = 2;    // one touchup at (arg0..arg3)
@@ -60,6 +81,10 @@ struct Wave {
     double rate[PPL];
     double integ[PPL];
     long long pstat[PPL];
+#if KMCG_PHASES
+    unsigned long long tph[5];
+    unsigned long long tmark;
+#endif
 
     __device__ __forceinline__ int *sites(int p) const { return avail + (long long)p * V1; }
     __device__ __forceinline__ int *back(int p) const { return avail + (long long)(P + p) * V1; }
@@ -162,6 +187,7 @@ struct NoMid {
     bool armed;
     template <class Wv>
     __device__ __forceinline__ void operator()(const Wv &, int) {}
+    __device__ __forceinline__ void wrote(int) {}
 };
 
 template <int PPL, int BITS, int RB, class Mid>
@@ -228,6 +254,7 @@ __device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, const 
                 } else {
                     cur += 1;
                     L.s(cur) = site;
+                    mid.wrote(cur);
                     L.b(site) = cur;
                     w[cur - base] = site;
 #pragma unroll
@@ -244,6 +271,7 @@ __device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, const 
                     if (mpos < last) {
                         const int moved = w[last - base];
                         L.s(mpos) = moved;
+                        mid.wrote(mpos);
                         L.b(moved) = mpos;
                         if (mpos >= base)
                             w[mpos - base] = moved;
@@ -253,6 +281,7 @@ __device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, const 
                             bv[j] = (st[j] == moved) ? mpos : bv[j];
                     }
                     L.s(last) = 0;
+                    mid.wrote(last);
                     L.b(site) = 0;
 #pragma unroll
                     for (int j = i + 1; j < RB; ++j)
@@ -714,8 +743,12 @@ __device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, bool m
             const int bz2 = W.d.dim >= 3 ? wrap1(z + unpack_z(o2), W.d.Z) : 0;
             // near the faces the segments run one by one
             if (W.d.interior(bx1, by1, bz1) && W.d.interior(bx2, by2, bz2) &&
-                post_pair<PPL, BITS, true>(W, cur, q2, bx1, by1, bz1, bx2, by2, bz2) >= 0)
-                return apply_queues<PPL, BITS, RB2>(W, mid);
+                post_pair<PPL, BITS, true>(W, cur, q2, bx1, by1, bz1, bx2, by2, bz2) >= 0) {
+                PH_MARK(W, 2);
+                const int a = apply_queues<PPL, BITS, RB2>(W, mid);
+                PH_MARK(W, 3);
+                return a;
+            }
         }
     }
     for (int i = 0; i < nit; ++i, it += IT_W) {
@@ -730,12 +763,16 @@ __device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, bool m
             break;
         if (!merged) {
             NoMid none;
+            PH_MARK(W, 2);
             e = apply_queues<PPL, BITS, RB2>(W, none);
+            PH_MARK(W, 3);
             if (e)
                 return e;
         }
     }
+    PH_MARK(W, 2);
     const int a = merged ? apply_queues<PPL, BITS, RB2>(W, mid) : ST_OK;
+    PH_MARK(W, 3);
     return e ? e : a;
 }
 
@@ -835,12 +872,18 @@ struct NextSel {
     bool armed = false;
     bool on;             // the next step's u_proc is at hand
     double u1;           // the next step's u_proc
+    double u2;           // the next step's u_site
     double &acc;         // this lane's prefix (PPL == 1)
     int nrs = 0;         // predicted count of this lane's process
     double total = 0.0;
     int pick = 0;
     bool made = false;   // prefixes were changed
-    __device__ NextSel(bool on_, double u1_, double &acc_) : on(on_), u1(u1_), acc(acc_) {}
+    // the next step's site, read ahead by the picked process's owner lane
+    // from the predicted count; stale if this replay writes that slot
+    int pf_slot = -1;    // the slot read (owner lane), else -1
+    int pf_val = 0;
+    bool pf_stale = false;
+    __device__ NextSel(bool on_, double u1_, double u2_, double &acc_) : on(on_), u1(u1_), u2(u2_), acc(acc_) {}
     template <class Wv>
     __device__ __forceinline__ void operator()(const Wv &W, int nr_after)
     {
@@ -853,10 +896,22 @@ struct NextSel {
         double a[1] = {acc};
         total = accumulate<1>(c, a, W.P, W.lane, fs);
         acc = a[0];
-        if (total > 0.0)
+        if (total > 0.0) {
             pick = interval_search<1>(a, u1 * total, W.P, W.lane);
+            // determine_procsite's slot from the predicted count; the load
+            // is issued now and waited for at the next step
+            const int nrp = readlane_i(nr_after, pick);
+            int kk = (int)(1.0 + u2 * (double)nrp);
+            kk = kk > nrp ? nrp : kk;
+            // a wrong prediction can overstate a count: stay inside the list
+            if (W.lane == pick && kk >= 1 && kk <= W.d.V) {
+                pf_slot = kk;
+                pf_val = W.sites(pick)[kk];
+            }
+        }
         made = true;
     }
+    __device__ __forceinline__ void wrote(int slot) { pf_stale |= slot == pf_slot; }
 };
 
 // Copy the op program to LDS (whole workgroup), then set up this wave.
@@ -919,6 +974,11 @@ __device__ __forceinline__ int wave_setup(Wave<PPL, BITS> &W, const KParams &kp,
     W.track = false;
     W.chk = false;
     W.local_smart = kp.backend == 0;
+#if KMCG_PHASES
+    for (int i = 0; i < 5; ++i)
+        W.tph[i] = 0;
+    W.tmark = __builtin_amdgcn_s_memtime();
+#endif
     for (int i = W.lane; i < W.mstride; i += 64)
         W.cnt[i] = 0;
 #pragma unroll
@@ -1245,7 +1305,17 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
     bool have = false;
     double s_total = 0.0;
     int s_pick = 0;
+    // the site of that pick, read during the replay (OV): valid while the
+    // replay left the predicted counts and did not write the slot read
+    bool pf_ok = false;
+    int pf_val = 0;
+#if KMCG_PHASES
+    for (int i = 0; i < 5; ++i)
+        W.tph[i] = 0;
+    W.tmark = __builtin_amdgcn_s_memtime();
+#endif
     while (done < n) {
+        PH_MARK(W, 4);
         if (uj == 64) {
             ubase += 64;
             uj = 0;
@@ -1280,6 +1350,7 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
             W.integ[k] = W.integ[k] + c[k] * dt;
         // determine_procsite (base.mpy:1075-1120)
         const int pick = have ? s_pick : interval_search<PPL>(acc, u[1] * total, P, lane);
+        PH_MARK(W, 0);
         const int pk = pick >> 6, pl = pick & 63;
         const int nrp = readlane_i(slot_get<PPL>(W.nr, pk), pl);
         int kk = (int)(1.0 + u[2] * (double)nrp);
@@ -1290,7 +1361,7 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
         }
         int sv = 0;
         if (lane == pl) {
-            sv = W.sites(pick)[kk];
+            sv = pf_ok ? pf_val : W.sites(pick)[kk];
 #pragma unroll
             for (int k = 0; k < PPL; ++k)
                 if (k == pk) W.pstat[k] += 1;
@@ -1300,6 +1371,7 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
         proc_prefetch(W.prog, W.off_proc, W.off_seg, pick + 1, lane, pre);
         uj += 1;
         const int site = readlane_i(sv, pl);
+        PH_MARK(W, 1);
         done += 1;
         if (MSD) {
             if (kp.acf_pa)
@@ -1313,12 +1385,14 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
             nr_before[k] = W.nr[k];
         bool redo = false;
         if constexpr (OV && PPL == 1 && !MSD) {
-            NextSel nx(uj < 64, readlane_d(lu1, uj & 63), acc[0]);
+            NextSel nx(uj < 64, readlane_d(lu1, uj & 63), readlane_d(lu2, uj & 63), acc[0]);
             st = run_proc<PPL, BITS, true>(W, pre, site, kp.merged, nx);
             have = nx.made && !st && !__ballot(W.nr[0] != nx.nrs);
             redo = nx.made && !have;
             s_total = nx.total;
             s_pick = nx.pick;
+            pf_ok = have && __ballot(nx.pf_slot > 0) && !__ballot(nx.pf_stale);
+            pf_val = nx.pf_val;
         } else {
             NoMid none;
             st = run_proc<PPL, BITS, !MSD>(W, pre, site, kp.merged, none);
@@ -1342,6 +1416,14 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
     if (W.track)
         write_record<PPL, BITS>(W, kp, slot, done == 0 ? 0 : (done - 1) / kp.sample_every, t, dt_last, ks, st);
     wave_store<PPL, BITS, LDS_LAT>(W, kp, r, true);
+#if KMCG_PHASES
+    PH_MARK(W, 4);
+    if (lane == 0 && kp.phase) {
+        for (int i = 0; i < 5; ++i)
+            kp.phase[PHASE_W * r + i] += W.tph[i];
+        kp.phase[PHASE_W * r + 5] += (unsigned long long)done;
+    }
+#endif
     if (lane == 0) {
         kp.ktime[r] = t;
         kp.kdt[r] = dt_last;
@@ -1794,6 +1876,7 @@ struct kmcg_batch {
     long long *budget = nullptr;
     int *rep_ids = nullptr;
     int *out = nullptr;
+    unsigned long long *phase = nullptr;   // phase counters (KMCG_PHASES builds)
     double *occ = nullptr;
     double *samples = nullptr;
     // pinned staging of a run's replica list and budgets; one run in flight
@@ -2412,6 +2495,7 @@ KParams make_params(kmcg_batch *b)
     kp.lists_ok = b->lists_ok;
     kp.executed = b->executed;
     kp.out = b->out;
+    kp.phase = b->phase;
     kp.P = m->P;
     kp.S = m->S;
     kp.spuck = m->spuck;
@@ -2696,7 +2780,7 @@ void kmcg_batch_free(kmcg_batch *b)
                     b->rngctr, b->status, b->executed, b->budget, b->rep_ids, b->out, b->occ,
                     b->samples, b->msd_hop, b->msd_d, b->msd_id, b->msd_site, b->msd_disp,
                     b->msd_off_d, b->msd_ev, b->msd_evn, b->lists_ok, b->acf_pa, b->acf_po, b->acf_buf,
-                    b->acf_pp, b->acf_bin, b->acf_cnt, b->acf_boff, b->acf_d, b->acf_i};
+                    b->acf_pp, b->acf_bin, b->acf_cnt, b->acf_boff, b->acf_d, b->acf_i, b->phase};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -2793,6 +2877,9 @@ int kmcg_batch_create(const kmcg_model *m, int32_t n, const int32_t *sizes, int3
     BALLOC(b->budget, n);
     BALLOC(b->rep_ids, n);
     BALLOC(b->out, 16);
+#if KMCG_PHASES
+    BALLOC(b->phase, (size_t)n * PHASE_W);
+#endif
     BALLOC(b->occ, (size_t)n * m->S * m->spuck);
     BALLOC(b->samples, (size_t)n * (4 + m->S * m->spuck + 2 * P));
     if (hipHostMalloc((void **)&b->h_ids, n * sizeof(int), hipHostMallocDefault) != hipSuccess ||
@@ -3239,6 +3326,27 @@ int kmcg_last_run_ms(kmcg_batch *b, float *ms)
         return rc;
     *ms = b->last_ms;
     return KMCG_OK;
+}
+
+int kmcg_phase_profile(kmcg_batch *b, int32_t r0, int32_t n, uint64_t *out)
+{
+#if KMCG_PHASES
+    if (!b || !out || r0 < 0 || n < 1 || r0 + n > b->n)
+        return fail(KMCG_EINVAL, "bad replica range");
+    int rc = settle(b);
+    if (rc)
+        return rc;
+    rc = copy_out(b, (unsigned long long *)out, b->phase + (size_t)r0 * PHASE_W, (size_t)n * PHASE_W);
+    if (rc)
+        return rc;
+    HIPCHK(hipMemsetAsync(b->phase + (size_t)r0 * PHASE_W, 0, (size_t)n * PHASE_W * sizeof(unsigned long long),
+                          b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return KMCG_OK;
+#else
+    (void)b; (void)r0; (void)n; (void)out;
+    return fail(KMCG_EUNSUPPORTED, "phase profile needs the library built with KMCG_PHASES");
+#endif
 }
 
 int kmcg_get_next_step(kmcg_batch *b, int32_t r, int32_t *proc, int32_t *site)

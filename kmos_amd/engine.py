@@ -196,6 +196,16 @@ class Batch:
         _lib.check(_lib.load().kmcg_last_run_ms(self.h, ctypes.byref(v)))
         return float(v.value)
 
+    def phase_profile(self, r0: int = 0, n: Optional[int] = None) -> np.ndarray:
+        """Step-loop phase counters of replicas r0.. ([n][8]: ticks in
+        select, site read, post, replay, rest; steps), cleared after the
+        read.  Needs the library built with KMCG_PHASES (loaded when the
+        environment sets KMCG_PHASES=1)."""
+        n = self.n - r0 if n is None else n
+        out = np.zeros((n, 8), np.uint64)
+        _lib.check(_lib.load().kmcg_phase_profile(self.h, r0, n, _ptr(out)))
+        return out
+
     def get_next_step(self, r: int):
         p, s = ctypes.c_int32(0), ctypes.c_int32(0)
         _lib.check(_lib.load().kmcg_get_next_step(self.h, r, ctypes.byref(p), ctypes.byref(s)))
