@@ -87,6 +87,8 @@ def test_graph_replays_match_direct_launches(engine, oracle):
             for k in x:
                 np.testing.assert_array_equal(x[k], y[k], err_msg="records %s" % k)
         _same(a, b, "recorded %d" % it)
+        o.run(int(bud[0]))
+        compare_state(b, 0, o, where="graph replica 0 after recorded %d" % it)
     # the select-overlapped kernel form is part of the graph key: toggling
     # it between replays captures (or reuses) the graph of that form
     for it, ov in enumerate((1, 0, 1, 0)):
