@@ -14,6 +14,7 @@ PHASES_PATH = os.path.join(HERE, "libkmcgpu_phases.so")
 
 KMCG_OK = 0
 ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "EUNSUPPORTED"}
+ERRORS_BY_NAME = {v: k for k, v in ERRORS.items()}
 REPLICA_STATUS = {0: "ok", 1: "absorbing", 2: "species_mismatch", 3: "del_absent", 4: "corrupt"}
 
 # (name, restype, argtypes) of every entry point of include/kmcg.h

@@ -139,7 +139,8 @@ def test_rejected_configuration_changes_nothing(engine, oracle):
     set_config and init fail before any replica is touched: every replica
     keeps its lattice, lists, counters and clock, and keeps stepping like
     the oracle."""
-    from kmos_amd._lib import KmcgError
+    import ctypes
+    from kmos_amd import _lib
     pt = W.zgb_project()
     cm = compile_model(pt, "local_smart")
     m = engine.Model(cm)
@@ -159,12 +160,21 @@ def test_rejected_configuration_changes_nothing(engine, oracle):
     rng = np.random.default_rng(3)
     configs = [_random_config(pt, int(v), rng) for v in b.volumes]
     configs[1][5] = 7
+    # the host wrapper refuses the id before any call, and so does the
+    # library itself when called directly through the C ABI
     print("bad set_config", flush=True)
-    with pytest.raises(KmcgError):
+    with pytest.raises(ValueError):
         b.set_config(configs)
+    flat = np.ascontiguousarray(np.concatenate(configs), dtype=np.int8)
+    seeds_c = np.ascontiguousarray(seeds, dtype=np.uint32)
+    lib = _lib.load()
+    assert lib.kmcg_set_config(b.h, 0, 3, flat.ctypes.data_as(ctypes.c_void_p)) == _lib.ERRORS_BY_NAME["EINVAL"]
     print("bad init", flush=True)
-    with pytest.raises(KmcgError):
+    with pytest.raises(ValueError):
         b.init(seeds, None, configs=configs)
+    assert lib.kmcg_init(b.h, 0, 3, seeds_c.ctypes.data_as(ctypes.c_void_p), None,
+                         flat.ctypes.data_as(ctypes.c_void_p)) == _lib.ERRORS_BY_NAME["EINVAL"]
+    assert b"unknown species" in lib.kmcg_last_error()
     for r in range(3):
         for x, y in zip(before[r], state(r)):
             np.testing.assert_array_equal(np.asarray(x), np.asarray(y), err_msg="replica %d" % r)
