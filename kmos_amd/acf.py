@@ -26,8 +26,9 @@ particle's first site, property at its site now):
     acf.calc_product_property(model); acf.initialize_acf(model, "ion")
     acf.do_kmc_steps_acf(model, 100000); acf.get_acf(model, normalization=True)
 
-``model`` is a kmos_amd.run.KMC_Model (replica 0 of its batch).  Only the
-local_smart backend's routine is restated.  A process is a *hop* when, in
+``model`` is a kmos_amd.run.KMC_Model (replica 0 of its batch).  The
+local_smart and lat_int backends' routines are restated (the otf backend is
+not built).  A process is a *hop* when, in
 the generated routine, one action takes the particle away (the site becomes
 the default species) and one puts a particle down on a site that held the
 default species, with no action that leaves its species unchanged (kmos's
@@ -40,7 +41,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .compiler import BACKEND_LOCAL_SMART, CompiledModel
+from .compiler import BACKEND_LAT_INT, BACKEND_LOCAL_SMART, CompiledModel
 
 HOP_W = 9
 
@@ -55,9 +56,17 @@ def displacement_table(cm: CompiledModel):
     """(hops int32[P][9], disp float64[P][3]) for kmcg_msd_init: per process
     the hop flag, the left site (dx, dy, dz, n) and the reached site
     relative to the chosen site's cell, and the displacement
-    matmul(unit_cell_size, cell difference + site position difference)."""
-    if cm.backend != BACKEND_LOCAL_SMART:
-        raise NotImplementedError("displacement tracking is restated for the local_smart backend only")
+    matmul(unit_cell_size, cell difference + site position difference).
+
+    local_smart chooses a process at its executing coordinate, so the sites
+    are taken relative to it (the smart writer, kmos/io/acf.py:443-671);
+    lat_int chooses it at site 1 of its own cell, so they are the actions'
+    coordinates themselves (the otf/lat_int writer, kmos/io/acf.py:674-1000,
+    `lsite = nr2lattice(nr_site) + (/0,0,0,-1/)`).  In both the particle
+    leaves the site of the action that takes it and reaches the site of the
+    action that puts it."""
+    if cm.backend not in (BACKEND_LOCAL_SMART, BACKEND_LAT_INT):
+        raise NotImplementedError("displacement tracking covers the local_smart and lat_int backends")
     pt = cm.project
     default = pt.default_species
     cell = [[_f32(v) for v in row] for row in pt.cell]
@@ -66,7 +75,7 @@ def displacement_table(cm: CompiledModel):
     hops = np.zeros((P, HOP_W), np.int32)
     disp = np.zeros((P, 3), np.float64)
     for p, proc in enumerate(pt.processes):
-        ex = proc.executing_coord()
+        ex_off = proc.executing_coord().offset if cm.backend == BACKEND_LOCAL_SMART else (0, 0, 0)
 
         def previous(action):
             return [c.species for c in proc.conditions if c.coord == action.coord][0]
@@ -76,8 +85,8 @@ def displacement_table(cm: CompiledModel):
         ok = True
         for a in proc.actions:
             prev = previous(a)
-            rel = (a.coord.offset[0] - ex.offset[0], a.coord.offset[1] - ex.offset[1],
-                   a.coord.offset[2] - ex.offset[2], pt.site_index(a.coord.layer, a.coord.name))
+            rel = (a.coord.offset[0] - ex_off[0], a.coord.offset[1] - ex_off[1],
+                   a.coord.offset[2] - ex_off[2], pt.site_index(a.coord.layer, a.coord.name))
             if a.species[:1] in ("^", "$"):
                 ok = False
             elif a.species == default and a.species != prev:

@@ -187,7 +187,6 @@ struct NoMid {
     bool armed;
     template <class Wv>
     __device__ __forceinline__ void operator()(const Wv &, int) {}
-    __device__ __forceinline__ void wrote(int) {}
 };
 
 template <int PPL, int BITS, int RB, class Mid>
@@ -254,7 +253,6 @@ __device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, const 
                 } else {
                     cur += 1;
                     L.s(cur) = site;
-                    mid.wrote(cur);
                     L.b(site) = cur;
                     w[cur - base] = site;
 #pragma unroll
@@ -271,7 +269,6 @@ __device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, const 
                     if (mpos < last) {
                         const int moved = w[last - base];
                         L.s(mpos) = moved;
-                        mid.wrote(mpos);
                         L.b(moved) = mpos;
                         if (mpos >= base)
                             w[mpos - base] = moved;
@@ -281,7 +278,6 @@ __device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, const 
                             bv[j] = (st[j] == moved) ? mpos : bv[j];
                     }
                     L.s(last) = 0;
-                    mid.wrote(last);
                     L.b(site) = 0;
 #pragma unroll
                     for (int j = i + 1; j < RB; ++j)
@@ -872,18 +868,12 @@ struct NextSel {
     bool armed = false;
     bool on;             // the next step's u_proc is at hand
     double u1;           // the next step's u_proc
-    double u2;           // the next step's u_site
     double &acc;         // this lane's prefix (PPL == 1)
     int nrs = 0;         // predicted count of this lane's process
     double total = 0.0;
     int pick = 0;
     bool made = false;   // prefixes were changed
-    // the next step's site, read ahead by the picked process's owner lane
-    // from the predicted count; stale if this replay writes that slot
-    int pf_slot = -1;    // the slot read (owner lane), else -1
-    int pf_val = 0;
-    bool pf_stale = false;
-    __device__ NextSel(bool on_, double u1_, double u2_, double &acc_) : on(on_), u1(u1_), u2(u2_), acc(acc_) {}
+    __device__ NextSel(bool on_, double u1_, double &acc_) : on(on_), u1(u1_), acc(acc_) {}
     template <class Wv>
     __device__ __forceinline__ void operator()(const Wv &W, int nr_after)
     {
@@ -896,22 +886,10 @@ struct NextSel {
         double a[1] = {acc};
         total = accumulate<1>(c, a, W.P, W.lane, fs);
         acc = a[0];
-        if (total > 0.0) {
+        if (total > 0.0)
             pick = interval_search<1>(a, u1 * total, W.P, W.lane);
-            // determine_procsite's slot from the predicted count; the load
-            // is issued now and waited for at the next step
-            const int nrp = readlane_i(nr_after, pick);
-            int kk = (int)(1.0 + u2 * (double)nrp);
-            kk = kk > nrp ? nrp : kk;
-            // a wrong prediction can overstate a count: stay inside the list
-            if (W.lane == pick && kk >= 1 && kk <= W.d.V) {
-                pf_slot = kk;
-                pf_val = W.sites(pick)[kk];
-            }
-        }
         made = true;
     }
-    __device__ __forceinline__ void wrote(int slot) { pf_stale |= slot == pf_slot; }
 };
 
 // Copy the op program to LDS (whole workgroup), then set up this wave.
@@ -1305,10 +1283,6 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
     bool have = false;
     double s_total = 0.0;
     int s_pick = 0;
-    // the site of that pick, read during the replay (OV): valid while the
-    // replay left the predicted counts and did not write the slot read
-    bool pf_ok = false;
-    int pf_val = 0;
 #if KMCG_PHASES
     for (int i = 0; i < 5; ++i)
         W.tph[i] = 0;
@@ -1361,7 +1335,7 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
         }
         int sv = 0;
         if (lane == pl) {
-            sv = pf_ok ? pf_val : W.sites(pick)[kk];
+            sv = W.sites(pick)[kk];
 #pragma unroll
             for (int k = 0; k < PPL; ++k)
                 if (k == pk) W.pstat[k] += 1;
@@ -1385,14 +1359,12 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
             nr_before[k] = W.nr[k];
         bool redo = false;
         if constexpr (OV && PPL == 1 && !MSD) {
-            NextSel nx(uj < 64, readlane_d(lu1, uj & 63), readlane_d(lu2, uj & 63), acc[0]);
+            NextSel nx(uj < 64, readlane_d(lu1, uj & 63), acc[0]);
             st = run_proc<PPL, BITS, true>(W, pre, site, kp.merged, nx);
             have = nx.made && !st && !__ballot(W.nr[0] != nx.nrs);
             redo = nx.made && !have;
             s_total = nx.total;
             s_pick = nx.pick;
-            pf_ok = have && __ballot(nx.pf_slot > 0) && !__ballot(nx.pf_stale);
-            pf_val = nx.pf_val;
         } else {
             NoMid none;
             st = run_proc<PPL, BITS, !MSD>(W, pre, site, kp.merged, none);

@@ -296,8 +296,8 @@ class Batch:
             return a
         lat = np.asarray(state["lattice"])
         if lat.shape != (V,) or not np.issubdtype(lat.dtype, np.integer) or \
-                (V and (lat.min() < 0 or lat.max() >= self.S)):
-            raise ValueError("state['lattice'] must hold %d species ids in 0..%d" % (V, self.S - 1))
+                (V and (lat.min() < 0 or lat.max() > self.max_species)):
+            raise ValueError("state['lattice'] must hold %d species ids in 0..%d" % (V, self.max_species))
         lat = np.ascontiguousarray(lat, dtype=np.int8)
         nr = arr("nr_of_sites", np.int32, (P,))
         ps = arr("procstat", np.int64, (P,))

@@ -1,13 +1,15 @@
 #!/usr/bin/env python3
 """Extract the hop table of the reference's generated displacement routine.
 
-Reads get_diff_sites_displacement from the proclist_acf.f90 the reference
-ships as a test fixture (tests/test_acf/_tmp_export_local_smart/ref_src,
-generated for tests/test_acf/2d_grid.xml, copied here as
+Reads get_diff_sites_displacement from the proclist_acf.f90 files the
+reference ships as test fixtures (tests/test_acf/_tmp_export_<backend>/
+ref_src, generated for tests/test_acf/2d_grid.xml, copied here as
 tests/golden/2d_grid.xml) and writes, as JSON data, per process the site
 the particle leaves (lsite_old) and the site it reaches (lsite_new), each
-as [dx, dy, dz, site] relative to the chosen site; `site` is the site name
-or "exec" for the chosen site's own type.
+as [dx, dy, dz, site]: for local_smart relative to the chosen site, `site`
+being the site name or "exec" for the chosen site's own type
+(ref_hops_2d_grid.json); for lat_int relative to the chosen site's cell,
+whose site 1 holds the process (ref_hops_2d_grid_lat_int.json).
 
 Usage: python tests/golden/extract_reference_hops.py [--ref /root/reference]
 """
@@ -17,7 +19,8 @@ import os
 import re
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = "tests/test_acf/_tmp_export_local_smart/ref_src/proclist_acf.f90"
+SRC = "tests/test_acf/_tmp_export_%s/ref_src/proclist_acf.f90"
+OUT = {"local_smart": "ref_hops_2d_grid.json", "lat_int": "ref_hops_2d_grid_lat_int.json"}
 LAYER = "default"
 
 
@@ -33,7 +36,8 @@ def parse(text):
             proc = m.group(1)
             out[proc] = {}
             continue
-        m = re.match(r"lsite_(new|old) = \(lsite(?: \+ \(/(.*)/\))?\)$", line)
+        # local_smart writes "(lsite + (/.../))", lat_int "lsite + (/.../)"
+        m = re.match(r"lsite_(new|old) = \(?lsite(?: \+ \(/(.*)/\))?\)?$", line)
         if m and proc:
             which = "fin" if m.group(1) == "new" else "init"
             if m.group(2) is None:
@@ -56,12 +60,13 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
     a = ap.parse_args()
-    with open(os.path.join(a.ref, SRC)) as f:
-        table = parse(f.read())
-    out = os.path.join(HERE, "ref_hops_2d_grid.json")
-    with open(out, "w") as f:
-        json.dump(table, f, indent=1, sort_keys=True)
-    print(out, len(table))
+    for backend, name in OUT.items():
+        with open(os.path.join(a.ref, SRC % backend)) as f:
+            table = parse(f.read())
+        out = os.path.join(HERE, name)
+        with open(out, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+        print(out, len(table))
 
 
 if __name__ == "__main__":

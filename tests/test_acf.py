@@ -1,10 +1,12 @@
 """Displacement tracking (MSD) on the CPU: the hop table and the oracle.
 
 * the hop table of the 2d_grid model equals the site pairs of the
-  reference's generated get_diff_sites_displacement (tests/golden/
-  ref_hops_2d_grid.json, extracted from the reference's fixture by
-  extract_reference_hops.py), and its displacements are the site-to-site
-  vectors of the model's geometry;
+  reference's generated get_diff_sites_displacement for both backends
+  (tests/golden/ref_hops_2d_grid.json and ref_hops_2d_grid_lat_int.json,
+  extracted from the reference's fixtures by extract_reference_hops.py):
+  local_smart's sites relative to the executing site, lat_int's relative to
+  the chosen site's cell; its displacements are the site-to-site vectors of
+  the model's geometry;
 * on a pure-hop model with hops across cell borders, every tracked
   particle's start position plus its accumulated displacement is its end
   position up to whole lattice periods, particle ids and sites stay a
@@ -129,6 +131,36 @@ def test_hop_table_matches_reference_routine():
         np.testing.assert_array_equal(disp[cm.process_names.index(name)], v)
 
 
+def test_hop_table_matches_reference_routine_lat_int():
+    pt = load_project(os.path.join(GOLDEN, "2d_grid.xml"))
+    cm = compile_model(pt, "lat_int")
+    hops, disp = displacement_table(cm)
+    ref = json.load(open(os.path.join(GOLDEN, "ref_hops_2d_grid_lat_int.json")))
+    assert sorted(ref) == sorted(cm.process_names)
+    for p, name in enumerate(cm.process_names):
+        for key, cols in (("init", slice(1, 5)), ("fin", slice(5, 9))):
+            dx, dy, dz, site = ref[name][key]
+            assert list(hops[p, cols]) == [dx, dy, dz, pt.site_index("default", site)], (name, key)
+        assert hops[p, 0] == 1
+    np.testing.assert_array_equal(disp, displacement_table(compile_model(pt, "local_smart"))[1])
+
+
+def test_hop_table_across_cells_lat_int():
+    """lat_int takes the sites relative to the cell (the actions'
+    coordinates themselves); the displacements are local_smart's."""
+    cm = compile_model(hop_project(), "lat_int")
+    hops, disp = displacement_table(cm)
+    row = dict(zip(cm.process_names, hops.tolist()))
+    assert row["b_a_right"] == [1, 0, 0, 0, 2, 1, 0, 0, 1]
+    assert row["b_a_down"] == [1, 0, 0, 0, 2, 0, -1, 0, 1]
+    ls = compile_model(hop_project(), "local_smart")
+    _, disp_ls = displacement_table(ls)
+    order = [ls.process_names.index(n) for n in cm.process_names]
+    np.testing.assert_array_equal(disp, disp_ls[order])
+    hops2, _ = displacement_table(compile_model(hop_project(True), "lat_int"))
+    assert hops2[-2:, 0].tolist() == [0, 0]
+
+
 def test_hop_table_across_cells():
     cm = compile_model(hop_project(), "local_smart")
     hops, disp = displacement_table(cm)
@@ -157,9 +189,9 @@ def _positions(cm, sites, size):
     return np.stack([x + pos[n, 0], y + pos[n, 1]], axis=1)
 
 
-@pytest.mark.parametrize("seed", [1, 2])
-def test_oracle_tracking_invariants(oracle, seed):
-    cm = compile_model(hop_project(), "local_smart")
+@pytest.mark.parametrize("seed,backend", [(1, "local_smart"), (2, "local_smart"), (3, "lat_int")])
+def test_oracle_tracking_invariants(oracle, seed, backend):
+    cm = compile_model(hop_project(), backend)
     hops, disp = displacement_table(cm)
     size = (7, 5)
     g = np.random.default_rng(seed)

@@ -3,7 +3,7 @@ sites, displacements and trajectory events bit for bit, and the ACF bins
 (counters, buffers, properties exact; bin contents, which carry kmc_time,
 to 1e-12), on a pure-hop model with hops across cell borders and on the
 same model with adsorption and desorption (no-hop processes), ragged sizes
-in one batch; and the kmos.run.acf-style functions on a KMC_Model of the
+in one batch, for both backends; and the kmos.run.acf-style functions on a KMC_Model of the
 reference's 2d_grid test model.
 """
 import os
@@ -29,10 +29,11 @@ def _compare_msd(b, r, o, where):
     np.testing.assert_array_equal(g["events"], c["events"], err_msg=where)
 
 
-@pytest.mark.parametrize("with_adsorption", [False, True])
-def test_tracking_matches_oracle(engine, oracle, with_adsorption):
-    print("case adsorption=%s" % with_adsorption, flush=True)
-    cm = compile_model(hop_project(with_adsorption), "local_smart")
+@pytest.mark.parametrize("with_adsorption,backend", [(False, "local_smart"), (True, "local_smart"),
+                                                     (False, "lat_int"), (True, "lat_int")])
+def test_tracking_matches_oracle(engine, oracle, with_adsorption, backend):
+    print("case adsorption=%s %s" % (with_adsorption, backend), flush=True)
+    cm = compile_model(hop_project(with_adsorption), backend)
     hops, disp = displacement_table(cm)
     P = cm.nr_of_proc
     sizes = [(7, 5), (12, 9), (20, 20)]
@@ -134,12 +135,13 @@ def _compare_acf(b, r, o, where):
         assert g[k] == c[k], "%s %s: %s vs %s" % (where, k, g[k], c[k])
 
 
-@pytest.mark.parametrize("with_adsorption", [False, True])
-def test_acf_matches_oracle(engine, oracle, with_adsorption):
+@pytest.mark.parametrize("with_adsorption,backend", [(False, "local_smart"), (True, "local_smart"),
+                                                     (True, "lat_int")])
+def test_acf_matches_oracle(engine, oracle, with_adsorption, backend):
     """Two property types, bins short enough for several wraps and steps
     that spill into the extended bins, ragged sizes in one batch."""
-    print("case acf adsorption=%s" % with_adsorption, flush=True)
-    cm = compile_model(hop_project(with_adsorption), "local_smart")
+    print("case acf adsorption=%s %s" % (with_adsorption, backend), flush=True)
+    cm = compile_model(hop_project(with_adsorption), backend)
     hops, _ = displacement_table(cm)
     P = cm.nr_of_proc
     sizes = [(8, 6), (12, 9), (20, 16)]

@@ -120,11 +120,13 @@ def test_set_state_rejects_broken_lists(engine):
         (lambda s: s["nr_of_sites"].__setitem__(p, 10**6), "nr_of_sites out of range"),
         (lambda s: s["nr_of_sites"].__setitem__(p, s["nr_of_sites"][p] - 1), "beyond nr_of_sites"),
         (lambda s: s["avail_sites"].__setitem__((p, 0), 10**6), "disagree"),
-        (lambda s: s["lattice"].__setitem__(0, 9), "unknown species"),
     ]
     for edit, msg in bad:
         with pytest.raises(KmcgError, match=msg):
             b.set_state(0, broken(edit))
+    # an unknown species is refused by the host wrapper before the call
+    with pytest.raises(ValueError, match="species ids"):
+        b.set_state(0, broken(lambda s: s["lattice"].__setitem__(0, 9)))
     after = b.get_state(0)
     for k in ("lattice", "nr_of_sites", "avail_sites", "avail_sites_back", "procstat"):
         np.testing.assert_array_equal(after[k], good[k], err_msg=k)
