@@ -19,6 +19,17 @@ lat_int (run_proc_*.f90, nli_*.f90, proclist.f90)
   nli       per group: leaves [[[[dx,dy,dz,n,[species]]...], result]] in
             text order (result 0 for the default branches)
   dispatch  process -> group
+otf (run_proc_*.f90, proclist.f90, proclist_pars.f90)
+  run_proc  per process, in text order: ["del", proc, [dx,dy,dz]],
+            ["write", [dx,dy,dz,n], old, new], ["upd", proc, [dx,dy,dz]] and
+            ["add", proc, [dx,dy,dz], [[dx,dy,dz,n,species], ...]] (the
+            select-case path guarding the add); processes sit on site 1 of
+            the cell, so del/upd/add carry the cell offset only
+  touchup   the adds of touchup_cell (its generic delete loop aside)
+  rates     per process: "vars", the nr_<species>_<flag> names in nr_vars
+            order (byst_<proc>); "bystanders", [[dx,dy,dz,n], {species:
+            [1-based nr_vars indices]}] in text order; "expr", the rate_<proc>
+            right-hand side with continuation lines joined
 
 Coordinates are made absolute: n is the 1-based site index in layer order.
 Usage: python tests/golden/extract_reference_ops.py [--ref /root/reference]
@@ -40,6 +51,8 @@ FIXTURES = [
     ("cli_local_smart", "tests/test_cli_generated_model", "cli_model.xml", "local_smart"),
     ("pdopd_local_smart", "tests/export_test/reference_pdopd_local_smart", "pdopd.xml", "local_smart"),
     ("pdopd_lat_int", "tests/export_test/reference_pdopd_lat_int", "pdopd.xml", "lat_int"),
+    ("ruo2_otf", "tests/export_test/reference_export_otf", "ruo2_default.xml", "otf"),
+    ("intzgb_otf", "tests/export_test/reference_export_intZGB_otf", "intzgb_otf.xml", "otf"),
 ]
 
 VEC = re.compile(r"\(/\s*([^/]*)/\)")
@@ -205,6 +218,90 @@ def parse_lat_int(fdir, names):
     return {"groups": groups, "nli": nli, "dispatch": dispatch, "touchup_groups": touch_groups}
 
 
+def _otf_ops(body, names):
+    """The ordered bookkeeping ops of one otf run_proc_<proc> or touchup_cell."""
+    ops = []
+    path, pending = [], []
+    for line in body.replace("&\n", " ").splitlines():
+        s = re.sub(r"\s+", " ", line.strip())
+        if not s or s.startswith("!"):
+            continue
+        m = re.match(r"call del_proc\((\w+), ?cell \+ \(/([^/]*)/\)\)$", s)
+        if m:
+            ops.append(["del", m.group(1), [int(v) for v in m.group(2).split(",")[:3]]])
+            continue
+        m = re.match(r"call replace_species\(cell(.*?), ?(\w+), ?(\w+)\)$", s)
+        if m:
+            ops.append(["write", parse_coord(m.group(1), 1, names), m.group(2), m.group(3)])
+            continue
+        m = re.match(r"call update_rates_matrix\((\w+), ?cell \+ \(/([^/]*)/\), ?gr_", s)
+        if m:
+            ops.append(["upd", m.group(1), [int(v) for v in m.group(2).split(",")[:3]]])
+            continue
+        m = re.match(r"select case\(get_species\(cell(.*)\)\)$", s)
+        if m:
+            pending.append(parse_coord(m.group(1), 1, names))
+            path.append(None)
+            continue
+        m = re.match(r"case\((\w+)\)$", s)
+        if m and pending:
+            path[-1] = pending[-1] + [m.group(1)]
+            continue
+        if s == "end select":
+            pending.pop()
+            path.pop()
+            continue
+        m = re.match(r"call add_proc\((\w+), ?cell \+ \(/([^/]*)/\), ?gr_", s)
+        if m:
+            ops.append(["add", m.group(1), [int(v) for v in m.group(2).split(",")[:3]],
+                        [list(p) for p in path if p is not None]])
+    return ops
+
+
+def _functions(text):
+    out = {}
+    for m in re.finditer(r"^function (\w+)\((.*?)\)\s*$(.*?)^end function \1", text, re.M | re.S):
+        out[m.group(1)] = m.group(3)
+    return out
+
+
+def parse_otf(fdir, names):
+    run_proc = {}
+    for path in sorted(glob.glob(os.path.join(fdir, "run_proc_*.f90"))):
+        for name, body in subroutines(open(path).read()).items():
+            if name.startswith("run_proc_"):
+                run_proc[name[len("run_proc_"):]] = _otf_ops(body, names)
+    text = open(os.path.join(fdir, "proclist.f90")).read()
+    touchup = _otf_ops(subroutines(text)["touchup_cell"], names)
+    pars = open(os.path.join(fdir, "proclist_pars.f90")).read()
+    joined = re.sub(r"&\s*\n\s*&", "", pars)
+    funcs = _functions(joined)
+    rates = {}
+    for m in re.finditer(r'byst_(\w+) = "([^"]*)"', joined):
+        proc = m.group(1)
+        entry = {"vars": m.group(2).split(), "bystanders": []}
+        cur = None
+        for line in funcs["gr_" + proc].splitlines():
+            s = re.sub(r"\s+", " ", line.strip())
+            mm = re.match(r"select case\(get_species\(cell(.*)\)\)$", s)
+            if mm:
+                entry["bystanders"].append([parse_coord(mm.group(1), 1, names), {}])
+                continue
+            mm = re.match(r"case\((\w+)\)$", s)
+            if mm:
+                cur = mm.group(1)
+                entry["bystanders"][-1][1][cur] = []
+                continue
+            mm = re.match(r"nr_vars\((\d+)\) = nr_vars\(\d+\) \+ 1$", s)
+            if mm:
+                entry["bystanders"][-1][1][cur].append(int(mm.group(1)))
+        body = funcs["rate_" + proc]
+        mm = re.search(r"^\s*rate_%s = (.*)$" % proc, body, re.M)
+        entry["expr"] = re.sub(r"\s+", " ", mm.group(1)).strip()
+        rates[proc] = entry
+    return {"run_proc": run_proc, "touchup": touchup, "rates": rates}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
@@ -212,7 +309,8 @@ def main():
     for out, fdir, model, backend in FIXTURES:
         names, pt = site_names(model)
         src = os.path.join(a.ref, fdir)
-        data = parse_local_smart(src, names) if backend == "local_smart" else parse_lat_int(src, names)
+        parse = {"local_smart": parse_local_smart, "lat_int": parse_lat_int, "otf": parse_otf}[backend]
+        data = parse(src, names)
         data["model"] = model
         data["backend"] = backend
         data["site_names"] = names
