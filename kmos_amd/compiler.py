@@ -26,6 +26,11 @@ lat_int (kmos/io/__init__.py:1497-2057)
       the nli case tree that maps a cell to a process of the group or 0
       (write_proclist_lat_int_run_proc, write_proclist_lat_int_nli_casetree).
 
+otf (kmos/io/__init__.py:2657-3723)
+    * per process the ordered del/write/update/add program of its run_proc
+      routine, the adds of touchup_cell and per process its rate table
+      description (kmos_amd/otf.py).
+
 ``CompiledModel.to_blob()`` flattens everything into one int32 array; its
 layout is documented in ``include/kmcg.h`` and read by the HIP library.
 """
@@ -41,7 +46,8 @@ from .model import CondAct, Coord, Process, Project, cmp_coords
 
 BACKEND_LOCAL_SMART = 0
 BACKEND_LAT_INT = 1
-BACKENDS = {"local_smart": BACKEND_LOCAL_SMART, "lat_int": BACKEND_LAT_INT}
+BACKEND_OTF = 2
+BACKENDS = {"local_smart": BACKEND_LOCAL_SMART, "lat_int": BACKEND_LAT_INT, "otf": BACKEND_OTF}
 
 BLOB_MAGIC = 0x4B4D4347  # 'KMCG'
 BLOB_VERSION = 1
@@ -107,6 +113,11 @@ class CompiledModel:
     # lat_int
     proc_group: List[int] = dataclasses.field(default_factory=list)
     groups: List[LatIntGroup] = dataclasses.field(default_factory=list)
+    # otf
+    otf_programs: List[list] = dataclasses.field(default_factory=list)
+    otf_touchup: list = dataclasses.field(default_factory=list)
+    otf_rates: list = dataclasses.field(default_factory=list)
+    param_names: List[str] = dataclasses.field(default_factory=list)
 
     @property
     def backend_name(self) -> str:
@@ -123,11 +134,21 @@ class CompiledModel:
                 for e in r.enables:
                     out.append(e.site[:3])
                     out += [c[:3] for c, _ in e.conds]
-        else:
+        elif self.backend == BACKEND_LAT_INT:
             for g in self.groups:
                 out += [tuple(off) for _, off in g.modified]
                 out += [s[:3] for s, _, _ in g.updates]
                 out += [tuple(nd[0:3]) for nd in g.nli_nodes if nd[4] != -1]
+        else:
+            # an op's cell offset plus the offset of each site it reads
+            for prog in self.otf_programs + [self.otf_touchup]:
+                for op in prog:
+                    out.append(op.off)
+                    for c, _ in op.conds:
+                        out.append(c[:3])
+                    if op.proc >= 0:
+                        for b, _ in self.otf_rates[op.proc].bystanders:
+                            out.append(tuple(op.off[i] + b[i] for i in range(3)))
         return out
 
     def min_size(self) -> int:
@@ -530,9 +551,21 @@ def compile_model(pt: Project, backend: str = "local_smart") -> CompiledModel:
         raise NotImplementedError("at most 8 species are supported")
     if backend == "local_smart":
         _compile_local_smart(pt, cm)
-    else:
+    elif backend == "lat_int":
         _compile_lat_int(pt, cm)
+    else:
+        _compile_otf(pt, cm)
     return cm
+
+
+def _compile_otf(pt: Project, cm: CompiledModel) -> None:
+    from . import otf
+    from .model import UNITS
+    otf.check_otf_model(pt)
+    cm.param_names = sorted(pt.parameters)
+    cm.otf_rates = [otf.compile_rate(pt, proc, UNITS) for proc in pt.processes]
+    cm.otf_programs = [otf.run_proc_program(pt, proc) for proc in pt.processes]
+    cm.otf_touchup = otf.touchup_program(pt)
 
 
 # ---------------------------------------------------------------------------
@@ -657,6 +690,71 @@ def _to_blob(cm: CompiledModel) -> np.ndarray:
             words[base + 6] = len(g.procs)
             words[base + 7] = here()
             words.extend(g.procs)
+    if cm.backend == BACKEND_OTF:
+        _otf_blob(cm, words, hdr)
     hdr[15] = cm.min_size()
     hdr[16] = len(words)
     return np.asarray(words, dtype=np.int32)
+
+
+def _otf_ops_words(ops) -> List[int]:
+    from . import otf
+    out: List[int] = []
+    for op in ops:
+        if op.kind == otf.OP_WRITE:
+            out += [op.kind, op.off[0], op.off[1], op.off[2], op.site, op.old, op.new]
+        elif op.kind == otf.OP_ADD:
+            out += [op.kind, op.proc, op.off[0], op.off[1], op.off[2], len(op.conds)]
+            for c, sp in op.conds:
+                out += [c[0], c[1], c[2], c[3], sp]
+        else:
+            out += [op.kind, op.proc, op.off[0], op.off[1], op.off[2]]
+    return out
+
+
+def _otf_blob(cm: CompiledModel, words: List[int], hdr: List[int]) -> None:
+    """otf sections (header words 9-13, 18):
+      9  proc table [P][2]: n_ops, off_ops -> run_proc program
+      10 n_params (userpar entries, names in sorted order)
+      11 rate table [P][9]: table_size, table_base, n_byst, off_byst, n_code,
+         off_code, n_aux, n_used, off_maxc
+           byst: [dx, dy, dz, n, inc[S]]*   code: rate program words
+           maxc: [largest count]* per used nr var
+      12 off_touchup program, 18 its n_ops
+      13 table entries per replica
+    program ops: DEL [0, q, dx, dy, dz], WRITE [1, dx, dy, dz, n, old, new],
+      UPD [2, q, dx, dy, dz], ADD [3, q, dx, dy, dz, n_cond, (dx, dy, dz, n, species)*]
+    """
+    P = cm.nr_of_proc
+    hdr[9] = len(words)
+    ptab = len(words)
+    words.extend([0] * (2 * P))
+    for p, prog in enumerate(cm.otf_programs):
+        words[ptab + 2 * p] = len(prog)
+        words[ptab + 2 * p + 1] = len(words)
+        words.extend(_otf_ops_words(prog))
+    hdr[10] = len(cm.param_names)
+    hdr[11] = len(words)
+    rtab = len(words)
+    words.extend([0] * (9 * P))
+    base = 0
+    for p, r in enumerate(cm.otf_rates):
+        rec = rtab + 9 * p
+        words[rec + 0] = r.table_size
+        words[rec + 1] = base
+        base += r.table_size
+        words[rec + 2] = len(r.bystanders)
+        words[rec + 3] = len(words)
+        for site, inc in r.bystanders:
+            words.extend(list(site) + list(inc))
+        words[rec + 4] = len(r.code)
+        words[rec + 5] = len(words)
+        words.extend(r.code)
+        words[rec + 6] = r.n_aux
+        words[rec + 7] = len(r.maxc)
+        words[rec + 8] = len(words)
+        words.extend(r.maxc)
+    hdr[12] = len(words)
+    hdr[18] = len(cm.otf_touchup)
+    words.extend(_otf_ops_words(cm.otf_touchup))
+    hdr[13] = base

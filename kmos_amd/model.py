@@ -83,6 +83,17 @@ class CondAct:
     implicit: bool = False
 
 
+@dataclasses.dataclass(frozen=True)
+class Bystander:
+    """A site whose species enters an otf rate without being a condition
+    (kmos/types.py Bystander): each species of ``allowed_species`` found
+    on ``coord`` counts once towards ``nr_<species>_<flag>`` for every flag
+    of the space-separated ``flag``."""
+    allowed_species: Tuple[str, ...]
+    coord: Coord
+    flag: str = ""
+
+
 @dataclasses.dataclass
 class Process:
     name: str
@@ -91,6 +102,10 @@ class Process:
     actions: List[CondAct] = dataclasses.field(default_factory=list)
     enabled: bool = True
     tof_count: Optional[Dict[str, float]] = None
+    # otf backend: the rate expression in the local environment and the
+    # bystanders it counts (kmos/types.py Process.otf_rate, bystander_list)
+    otf_rate: Optional[str] = None
+    bystanders: List[Bystander] = dataclasses.field(default_factory=list)
 
     def executing_coord(self) -> Coord:
         """Lowest ``sort_key`` among the actions (kmos/types.py:2094-2097)."""
@@ -143,6 +158,25 @@ class Project:
     def site_index(self, layer: str, site: str) -> int:
         """1-based site index within the unit cell."""
         return self.site_list().index((layer, site)) + 1
+
+    def shorten_names(self, max_length: int = 15) -> Dict[str, str]:
+        """Rename processes longer than ``max_length - 4`` characters to
+        their first ``max_length - 4`` characters plus a 4-digit counter per
+        stub (kmos/types.py:559-588; the CLI's export does this with
+        ``--variable-length``, default 95).  Returns short -> long names."""
+        if max_length < 5:
+            raise ValueError("Max variable length has to be at least 5.")
+        digits = 4
+        stubs: Dict[str, int] = {}
+        out = {}
+        for proc in self.processes:
+            if len(proc.name) > max_length - digits:
+                stub = proc.name[:max_length - digits]
+                short = "%s%04d" % (stub, stubs.get(stub, 0))
+                stubs[stub] = stubs.get(stub, 0) + 1
+                out[short] = proc.name
+                proc.name = short
+        return out
 
     def process_ids(self) -> Dict[str, int]:
         return {p.name: i + 1 for i, p in enumerate(self.processes)}
@@ -273,14 +307,20 @@ def parse_xml(path_or_text: str) -> Project:
                     rate_constant=pe.attrib.get("rate_constant", "0."),
                     enabled=(enabled.strip() != "False"),
                     tof_count=_parse_tof(pe.attrib.get("tof_count")),
+                    otf_rate=pe.attrib.get("otf_rate"),
                 )
                 for sub in pe:
-                    if sub.tag not in ("condition", "action"):
+                    if sub.tag not in ("condition", "action", "bystander"):
                         continue
                     coord = Coord(
                         name=sub.attrib["coord_name"],
                         offset=tuple(int(i) for i in sub.attrib["coord_offset"].split()),
                         layer=sub.attrib["coord_layer"])
+                    if sub.tag == "bystander":
+                        proc.bystanders.append(Bystander(
+                            allowed_species=tuple(sub.attrib["allowed_species"].split()),
+                            coord=coord, flag=sub.attrib.get("flag", "")))
+                        continue
                     ca = CondAct(species=sub.attrib["species"], coord=coord,
                                  implicit=sub.attrib.get("implicit", "") == "True")
                     (proc.actions if sub.tag == "action" else proc.conditions).append(ca)
@@ -367,6 +407,18 @@ def parse_ini(path_or_text: str) -> Project:
                 enabled=opts.getboolean("enabled", True),
                 tof_count=_parse_tof(opts.get("tof_count")) if "tof_count" in opts else None,
             )
+            otf = opts.get("otf_rate")
+            proc.otf_rate = None if otf is None or otf.strip() == "None" else otf
+            # bystanders = ['A', 'B']@site.(dx, dy, dz)|flag + ...
+            # (kmos/types.py:855-890)
+            for item in (opts.get("bystanders") or "").split("+"):
+                if not item.strip():
+                    continue
+                allowed, rest = item.strip().split("@")
+                coord_text, flag = rest.split("|")
+                proc.bystanders.append(Bystander(
+                    allowed_species=tuple(ast.literal_eval(allowed.strip())),
+                    coord=_parse_ini_coord(coord_text, first_layer), flag=flag.strip()))
             for item in opts["actions"].split("+"):
                 sp, co = item.strip().split("@")
                 proc.actions.append(CondAct(sp.strip(), _parse_ini_coord(co, first_layer)))
@@ -459,12 +511,21 @@ def to_xml(pt: Project) -> str:
         attrs = dict(name=p.name, rate_constant=p.rate_constant, enabled=str(bool(p.enabled)))
         if p.tof_count:
             attrs["tof_count"] = repr(p.tof_count)
+        if p.otf_rate:
+            attrs["otf_rate"] = p.otf_rate
         pe = ET.SubElement(prl, "process", **attrs)
         for tag, lst in (("condition", p.conditions), ("action", p.actions)):
             for ca in lst:
                 ET.SubElement(pe, tag, species=ca.species, coord_layer=ca.coord.layer,
                               coord_name=ca.coord.name,
                               coord_offset=" ".join(str(int(v)) for v in ca.coord.offset))
+        for b in p.bystanders:
+            attrs = dict(allowed_species=" ".join(b.allowed_species), coord_layer=b.coord.layer,
+                         coord_name=b.coord.name,
+                         coord_offset=" ".join(str(int(v)) for v in b.coord.offset))
+            if b.flag:
+                attrs["flag"] = b.flag
+            ET.SubElement(pe, "bystander", **attrs)
     ET.SubElement(root, "output_list")
     ET.indent(root)
     return '<?xml version="1.0" ?>\n' + ET.tostring(root, encoding="unicode") + "\n"
