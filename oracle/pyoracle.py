@@ -320,3 +320,150 @@ class Oracle:
         self.L.orc_set_state(self.h, float(st["kmc_time"]), float(st.get("kmc_time_step", 0.0)),
                             int(st["kmc_step"]), int(st["kmc_step"] if ctr is None else ctr),
                             *[_ptr(a) for a in keep])
+
+
+# ---------------------------------------------------------------------------
+# otf (oracle/otf_oracle.c)
+# ---------------------------------------------------------------------------
+
+OTF_SRC = os.path.join(HERE, "otf_oracle.c")
+OTF_LIB = os.path.join(HERE, "libotf_oracle.so")
+OTF_FAST = os.path.join(HERE, "libotf_oracle_fast.so")
+OTF_STATUS = {0: "ok", 1: "absorbing", 2: "species_mismatch", 3: "del_absent", 4: "bad_arg",
+              5: "search"}
+
+
+def build_otf(force=False):
+    for path, opt in ((OTF_LIB, "-O2"), (OTF_FAST, "-O3")):
+        if force or not os.path.exists(path) or os.path.getmtime(path) < os.path.getmtime(OTF_SRC):
+            subprocess.check_call(["gcc", opt, "-ffp-contract=off", "-fPIC", "-shared",
+                                   OTF_SRC, "-o", path, "-lm"])
+    return OTF_LIB
+
+
+_otf_libs = {}
+
+
+def otf_lib(fast=False):
+    if fast not in _otf_libs:
+        path = OTF_FAST if fast else OTF_LIB
+        if not os.path.exists(path):
+            build_otf()
+        L = ctypes.CDLL(path)
+        vp, i32, i64, u32, u64, dbl = (ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64,
+                                       ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double)
+        L.otf_create.restype = vp
+        L.otf_create.argtypes = [vp, i64, i32, i32, i32]
+        L.otf_free.argtypes = [vp]
+        L.otf_init.argtypes = [vp, u32, u32]
+        L.otf_set_params.argtypes = [vp, vp, vp]
+        L.otf_set_config.argtypes = [vp, vp]
+        L.otf_set_config.restype = i32
+        L.otf_update_accum_rate.argtypes = [vp]
+        L.otf_run.argtypes = [vp, i64, vp, vp, vp]
+        L.otf_run.restype = i64
+        for name, rt in (("otf_status", i32), ("otf_volume", i32), ("otf_kmc_time", dbl),
+                         ("otf_kmc_step", i64), ("otf_rng_ctr", u64)):
+            getattr(L, name).argtypes = [vp]
+            getattr(L, name).restype = rt
+        for name in ("otf_get_lattice", "otf_get_nr", "otf_get_procstat", "otf_get_integ",
+                     "otf_get_accum", "otf_get_table"):
+            getattr(L, name).argtypes = [vp, vp]
+        L.otf_get_avail.argtypes = [vp, vp, vp]
+        L.otf_set_clock.argtypes = [vp, dbl, i64, u64]
+        _otf_libs[fast] = L
+    return _otf_libs[fast]
+
+
+class OtfOracle:
+    """One replica of an otf model on the CPU (oracle/otf_oracle.c)."""
+
+    def __init__(self, blob, size, fast=False):
+        self.L = otf_lib(fast)
+        self.blob = np.ascontiguousarray(blob, dtype=np.int32)
+        X, Y = int(size[0]), int(size[1])
+        Z = int(size[2]) if len(size) > 2 else 1
+        self.h = self.L.otf_create(self.blob.ctypes.data_as(ctypes.c_void_p), self.blob.size, X, Y, Z)
+        if not self.h:
+            raise ValueError("not an otf model blob")
+        self.P = int(self.blob[3])
+        self.K = int(self.blob[10])
+        self.V = self.L.otf_volume(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.otf_free(self.h)
+            self.h = None
+
+    def init(self, seed, stream=0):
+        self.L.otf_init(self.h, seed, stream)
+
+    def set_params(self, rates, userpar=()):
+        r = np.ascontiguousarray(rates, dtype=np.float64)
+        u = np.ascontiguousarray(userpar if len(userpar) else [0.0], dtype=np.float64)
+        assert r.size == self.P and (len(userpar) == self.K)
+        self.L.otf_set_params(self.h, r.ctypes.data_as(ctypes.c_void_p), u.ctypes.data_as(ctypes.c_void_p))
+
+    def set_config(self, config):
+        c = np.ascontiguousarray(config, dtype=np.int8)
+        assert c.size == self.V
+        return self.L.otf_set_config(self.h, c.ctypes.data_as(ctypes.c_void_p))
+
+    def run(self, n, trace=False):
+        if trace:
+            tp = np.zeros(n, np.int32)
+            ts = np.zeros(n, np.int32)
+            tt = np.zeros(n, np.float64)
+            k = self.L.otf_run(self.h, n, tp.ctypes.data_as(ctypes.c_void_p),
+                               ts.ctypes.data_as(ctypes.c_void_p), tt.ctypes.data_as(ctypes.c_void_p))
+            return k, tp[:k], ts[:k], tt[:k]
+        return self.L.otf_run(self.h, n, None, None, None)
+
+    def update_accum_rate(self):
+        self.L.otf_update_accum_rate(self.h)
+
+    @property
+    def status(self):
+        return OTF_STATUS.get(self.L.otf_status(self.h), "?")
+
+    @property
+    def kmc_time(self):
+        return self.L.otf_kmc_time(self.h)
+
+    @property
+    def kmc_step(self):
+        return self.L.otf_kmc_step(self.h)
+
+    @property
+    def rng_ctr(self):
+        return self.L.otf_rng_ctr(self.h)
+
+    def _get(self, name, dtype, n):
+        a = np.zeros(n, dtype)
+        getattr(self.L, name)(self.h, a.ctypes.data_as(ctypes.c_void_p))
+        return a
+
+    def lattice(self):
+        return self._get("otf_get_lattice", np.int8, self.V)
+
+    def nr_of_sites(self):
+        return self._get("otf_get_nr", np.int32, self.P)
+
+    def procstat(self):
+        return self._get("otf_get_procstat", np.int64, self.P)
+
+    def integ_rates(self):
+        return self._get("otf_get_integ", np.float64, self.P)
+
+    def accum_rates(self):
+        return self._get("otf_get_accum", np.float64, self.P)
+
+    def table(self):
+        return self._get("otf_get_table", np.float64, int(self.blob[13]))
+
+    def avail(self):
+        """(sites [P][V], back [P][V], rates_matrix [P][V+1])"""
+        av = np.zeros((self.P, self.V, 2), np.int32)
+        rm = np.zeros((self.P, self.V + 1), np.float64)
+        self.L.otf_get_avail(self.h, av.ctypes.data_as(ctypes.c_void_p), rm.ctypes.data_as(ctypes.c_void_p))
+        return av[:, :, 0].copy(), av[:, :, 1].copy(), rm

@@ -21,7 +21,7 @@ sys.path.insert(0, os.path.join(REPO, "tests"))
 
 from kmos_amd.compiler import compile_model  # noqa: E402
 from paritylib import pdopd_project, random_config  # noqa: E402
-from refbin import run_reference  # noqa: E402
+from refbin import run_reference, run_reference_otf  # noqa: E402
 
 # (name, reference build, size, seed, stream, steps, rate seed, random config)
 CASES = [
@@ -35,6 +35,12 @@ CASES = [
     ("pdopd_local_smart_6x5_config", "pdopd_local_smart", (6, 5), 8, 4, 4000, 16, True),
     ("pdopd_lat_int_5x4", "pdopd_lat_int", (5, 4), 9, 5, 4000, 17, False),
     ("pdopd_lat_int_6x5_config", "pdopd_lat_int", (6, 5), 10, 6, 4000, 18, True),
+]
+# otf (oracle/ref_driver_otf.f90): the RuO2 fixture, whose names the
+# exporter shortened to 35 characters; also stores rates_matrix
+OTF_CASES = [
+    ("ruo2_otf_10x10", "ruo2_otf", (10, 10), 21, 7, 5000, 31, False),
+    ("ruo2_otf_8x6_config", "ruo2_otf", (8, 6), 22, 8, 5000, 32, True),
 ]
 
 
@@ -67,6 +73,19 @@ def main():
             proc=res["proc"], site=res["site"], time=res["time"], lattice=res["lattice"],
             nr=res["nr"], procstat=res["procstat"], integ=res["integ"],
             sites=res["sites"], back=res["back"], kmc_step=res["kmc_step"])
+        print(out, os.path.getsize(out))
+    for name, build, size, seed, stream, steps, rseed, use_cfg in OTF_CASES:
+        rates, config = case_inputs("ruo2", size, rseed, use_cfg)
+        userpar = np.random.default_rng(rseed + 1000).uniform(0.5, 1.5, 14)   # 14 parameters
+        res = run_reference_otf(build, size, seed, stream, steps, rates, len(rates), userpar, config=config)
+        out = os.path.join(HERE, "ref_traj_%s.npz" % name)
+        np.savez_compressed(
+            out, model="ruo2", size=np.array(size), seed=seed, stream=stream, steps=steps, rates=rates,
+            userpar=userpar, config=config if config is not None else np.zeros(0, np.int8),
+            proc=res["proc"], site=res["site"], time=res["time"], lattice=res["lattice"],
+            nr=res["nr"], procstat=res["procstat"], integ=res["integ"],
+            sites=res["sites"], back=res["back"], kmc_step=res["kmc_step"],
+            rates_matrix=res["rates_matrix"])
         print(out, os.path.getsize(out))
 
 
