@@ -1,0 +1,93 @@
+/*
+ * kmcg_otf.h -- C ABI of the engine's otf backend (libkmcgpu.so).
+ *
+ * The otf backend keeps, per process, the cells where it can run and each
+ * entry's own rate, computed from the species around the cell (the
+ * process's bystanders) by its otf_rate expression.  Numbering and return
+ * codes are those of kmcg.h; replica status codes are KMCG_REPLICA_*.
+ *
+ * Reference interfaces these replace (paths relative to the kmos tree):
+ *   f2py module kmc_model.{base,lattice,proclist,proclist_pars} of an otf
+ *   export (kmos/utils/__init__.py:406-521, the otf writer
+ *   kmos/io/__init__.py:2657-3723, kmos/fortran_src/base_otf.f90).
+ *
+ * Model blob: kmos_amd/compiler.py to_blob() with backend 2 (header and
+ * otf sections documented in _otf_blob there).
+ *
+ * Sizes: one entry of `sizes` per replica, (X, Y, Z) cells; each extent at
+ * least the model's largest offset, at most 65536 sites per replica; at
+ * most 64 processes and one layer.
+ */
+#ifndef KMCG_OTF_H
+#define KMCG_OTF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct kmcg_otf_model kmcg_otf_model;
+typedef struct kmcg_otf_batch kmcg_otf_batch;
+
+/* Parse and check an otf model blob (the generated run_proc_<proc>,
+ * touchup_cell and gr_<proc> routines as data). */
+int kmcg_otf_model_load(const int32_t *blob, int64_t nwords, kmcg_otf_model **out);
+void kmcg_otf_model_free(kmcg_otf_model *m);
+
+/* info4: nr_of_proc, nr_of_species, parameters (userpar entries), rate
+ * table entries per replica. */
+int kmcg_otf_model_info(const kmcg_otf_model *m, int32_t *info4);
+
+/* Device state for n replicas of possibly different sizes (allocate_system,
+ * base_otf.f90:741-866). */
+int kmcg_otf_batch_create(const kmcg_otf_model *m, int32_t n, const int32_t *sizes, int32_t device,
+                          kmcg_otf_batch **out);
+void kmcg_otf_batch_free(kmcg_otf_batch *b);
+
+/* proclist.init / initialize_state for replicas r0..r0+n-1: default species
+ * everywhere, touchup_cell over the cells (x fastest), clocks and counters
+ * zero, the uniform stream of each replica at its start. */
+int kmcg_otf_init(kmcg_otf_batch *b, int32_t r0, int32_t n, const uint32_t *seeds, const uint32_t *streams);
+
+/* base.set_rate_const for every process (rates[n][P]) and
+ * proclist_pars.update_user_parameter for every parameter (userpar[n][K],
+ * parameters in name order), then recalculate_rates_matrix: every listed
+ * entry takes the rate of its cell.  Before init only the values are kept
+ * (init uses them). */
+int kmcg_otf_set_params(kmcg_otf_batch *b, int32_t r0, int32_t n, const double *rates, const double *userpar);
+
+/* The lattice of replicas r0..r0+n-1 replaced by `config` (their site
+ * arrays concatenated, species 0..S-1), then touchup_cell over the cells
+ * with x outermost (kmos/run/__init__.py:1439-1457 _adjust_database). */
+int kmcg_otf_set_config(kmcg_otf_batch *b, int32_t r0, int32_t n, const int8_t *config);
+
+/* proclist.do_kmc_steps(steps) on replicas r0..r0+n-1 (asynchronous; a
+ * replica whose status is not KMCG_REPLICA_OK does not step). */
+int kmcg_otf_run(kmcg_otf_batch *b, int32_t r0, int32_t n, int64_t steps);
+int kmcg_otf_sync(kmcg_otf_batch *b);
+/* Kernel time of the last kmcg_otf_run, in milliseconds (HIP events on the
+ * batch's stream). */
+int kmcg_otf_last_run_ms(kmcg_otf_batch *b, float *ms);
+
+/* base.get_kmc_time, get_kmc_step and the replica's status. */
+int kmcg_otf_get_scalars(kmcg_otf_batch *b, int32_t r, double *kmc_time, int64_t *kmc_step, int32_t *status);
+/* lattice(1:V) */
+int kmcg_otf_get_lattice(kmcg_otf_batch *b, int32_t r, int8_t *out);
+/* get_nrofsites, get_procstat, get_integ_rate and get_accum_rate of every
+ * process (accum as of the last step's update_accum_rate). */
+int kmcg_otf_get_counters(kmcg_otf_batch *b, int32_t r, int32_t *nr_of_sites, int64_t *procstat,
+                          double *integ, double *accum);
+/* avail_sites(p, 1:V, 1) as sites[P][V], avail_sites(p, 1:V, 2) as
+ * back[P][V] and rates_matrix(p, 1:V+1) as rates_matrix[P][V+1]
+ * (get_avail_site, get_rate). */
+int kmcg_otf_get_avail(kmcg_otf_batch *b, int32_t r, int32_t *sites, int32_t *back, double *rates_matrix);
+/* The replica's rate tables (every process's otf_rate on every bystander
+ * count combination); no counterpart in the reference (a test aid). */
+int kmcg_otf_get_table(kmcg_otf_batch *b, int32_t r, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -5,7 +5,8 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc", "kmcg_engine.hip")
-DEPS = [SRC, os.path.join(HERE, "csrc", "kmcg_device.h"), os.path.join(REPO, "include", "kmcg.h")]
+DEPS = [SRC, os.path.join(HERE, "csrc", "kmcg_device.h"), os.path.join(HERE, "csrc", "kmcg_otf.hip"),
+        os.path.join(REPO, "include", "kmcg.h"), os.path.join(REPO, "include", "kmcg_otf.h")]
 LIB = os.path.join(HERE, "libkmcgpu.so")
 # the same source with the step loop's phase counters compiled in
 PHASES_LIB = os.path.join(HERE, "libkmcgpu_phases.so")

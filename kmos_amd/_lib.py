@@ -1,4 +1,5 @@
-"""ctypes binding of libkmcgpu.so (the C ABI declared in include/kmcg.h).
+"""ctypes binding of libkmcgpu.so (the C ABI declared in include/kmcg.h and
+include/kmcg_otf.h).
 
 There is no CPU fallback: if the library is missing or no GPU is visible,
 every engine call raises.
@@ -17,7 +18,7 @@ ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "EUNSUPPORTED"}
 ERRORS_BY_NAME = {v: k for k, v in ERRORS.items()}
 REPLICA_STATUS = {0: "ok", 1: "absorbing", 2: "species_mismatch", 3: "del_absent", 4: "corrupt"}
 
-# (name, restype, argtypes) of every entry point of include/kmcg.h
+# (name, restype, argtypes) of every entry point of include/*.h
 _vp, _i32, _i64, _u32, _d = (ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64,
                              ctypes.c_uint32, ctypes.c_double)
 SIGNATURES = [
@@ -73,6 +74,23 @@ SIGNATURES = [
     ("kmcg_run_sampled", _i32, [_vp, _vp, _i32, _i64, _vp, _vp]),
     ("kmcg_query", _i32, [_vp, _vp]),
     ("kmcg_run_recorded", _i32, [_vp, _vp, _i32, _i64, _vp, _i32, _i32, _vp]),
+    # include/kmcg_otf.h
+    ("kmcg_otf_model_load", _i32, [_vp, _i64, _vp]),
+    ("kmcg_otf_model_free", None, [_vp]),
+    ("kmcg_otf_model_info", _i32, [_vp, _vp]),
+    ("kmcg_otf_batch_create", _i32, [_vp, _i32, _vp, _i32, _vp]),
+    ("kmcg_otf_batch_free", None, [_vp]),
+    ("kmcg_otf_init", _i32, [_vp, _i32, _i32, _vp, _vp]),
+    ("kmcg_otf_set_params", _i32, [_vp, _i32, _i32, _vp, _vp]),
+    ("kmcg_otf_set_config", _i32, [_vp, _i32, _i32, _vp]),
+    ("kmcg_otf_run", _i32, [_vp, _i32, _i32, _i64]),
+    ("kmcg_otf_sync", _i32, [_vp]),
+    ("kmcg_otf_last_run_ms", _i32, [_vp, _vp]),
+    ("kmcg_otf_get_scalars", _i32, [_vp, _i32, _vp, _vp, _vp]),
+    ("kmcg_otf_get_lattice", _i32, [_vp, _i32, _vp]),
+    ("kmcg_otf_get_counters", _i32, [_vp, _i32, _vp, _vp, _vp, _vp]),
+    ("kmcg_otf_get_avail", _i32, [_vp, _i32, _vp, _vp, _vp]),
+    ("kmcg_otf_get_table", _i32, [_vp, _i32, _vp]),
 ]
 
 _lib = None

@@ -24,6 +24,7 @@
 
 #include "kmcg.h"
 #include "kmcg_device.h"
+#include "kmcg_otf.h"
 
 using namespace kmcg;
 
@@ -4045,3 +4046,6 @@ int kmcg_sample(kmcg_batch *b, const int32_t *replicas, int32_t n, double *out)
 }
 
 } // extern "C"
+
+// the otf backend (same translation unit: error helpers, uniform stream)
+#include "kmcg_otf.hip"

@@ -15,7 +15,8 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _declared():
-    text = open(os.path.join(REPO, "include", "kmcg.h")).read()
+    import glob
+    text = "".join(open(f).read() for f in sorted(glob.glob(os.path.join(REPO, "include", "*.h"))))
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(kmcg_\w+)\s*\(", text)))
 

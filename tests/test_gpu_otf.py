@@ -1,0 +1,161 @@
+"""The otf engine against the otf oracle (itself bit-exact against the
+reference's generated otf Fortran, tests/test_otf_oracle.py).
+
+Ragged batches of the RuO2 otf model (36 processes, two sites per cell)
+and of the intZGB model (bystanders, otf_rate expressions with integer
+powers of the parameters), some replicas started from a random lattice:
+after chunks of 1, 99 and 1900 steps every replica's lattice, list sizes,
+avail_sites (both tables), rates_matrix (every entry and the running sum
+in position V+1), procstat and accum_rates equal the oracle's bit for bit;
+kmc_time and integ_rates agree to 1e-12 relative (the device's log);
+the rate tables equal the oracle's bit for bit.  A larger intZGB batch
+(64 x 64 lattices) is checked on sampled replicas after 3000 steps.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from kmos_amd import workloads as W
+from kmos_amd.compiler import compile_model
+from kmos_amd.model import load_project
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def otf_engine(engine):
+    from kmos_amd import otf_engine
+    return otf_engine
+
+
+@pytest.fixture(scope="module")
+def otf_oracle():
+    from oracle import pyoracle
+    pyoracle.build_otf()
+    return pyoracle
+
+
+def _ruo2():
+    pt = W.ruo2_project()
+    pt.shorten_names(max_length=35)
+    cm = compile_model(pt, "otf")
+    return cm, 10.0 ** np.random.default_rng(5).uniform(-2, 6, (1, cm.nr_of_proc))[0]
+
+
+def _intzgb():
+    pt = load_project(os.path.join(GOLDEN, "intzgb_otf.xml"))
+    cm = compile_model(pt, "otf")
+    y = 0.45
+    rates = np.array([y, 0.05] + [0.25 * y] * 4 + [0.05] * 2 + [0.5 * (1 - y)] * 2)
+    return cm, rates
+
+
+def _compare(b, r, o, where, tables=False):
+    t, k, st = b.scalars(r)
+    assert st == "ok" and o.status == "ok", (where, st, o.status)
+    assert k == o.kmc_step, where
+    assert abs(t - o.kmc_time) <= 1e-12 * max(abs(o.kmc_time), 1e-300), (where, t, o.kmc_time)
+    np.testing.assert_array_equal(b.lattice(r), o.lattice(), err_msg=where)
+    c = b.counters(r)
+    np.testing.assert_array_equal(c["nr_of_sites"], o.nr_of_sites(), err_msg=where)
+    np.testing.assert_array_equal(c["procstat"], o.procstat(), err_msg=where)
+    np.testing.assert_allclose(c["integ_rates"], o.integ_rates(), rtol=1e-12, atol=0, err_msg=where)
+    if o.kmc_step:
+        np.testing.assert_array_equal(c["accum_rates"], o.accum_rates(), err_msg=where)
+    s, bk, rm = b.avail(r)
+    s2, bk2, rm2 = o.avail()
+    np.testing.assert_array_equal(s, s2, err_msg=where)
+    np.testing.assert_array_equal(bk, bk2, err_msg=where)
+    np.testing.assert_array_equal(rm, rm2, err_msg=where)
+    if tables:
+        np.testing.assert_array_equal(b.table(r), o.table(), err_msg=where)
+
+
+@pytest.mark.parametrize("model", ["ruo2", "intzgb"])
+def test_otf_matches_oracle(otf_engine, otf_oracle, model):
+    print("case otf %s" % model, flush=True)
+    cm, base = _ruo2() if model == "ruo2" else _intzgb()
+    P, K = cm.nr_of_proc, len(cm.param_names)
+    sizes = [(10, 10), (8, 6), (20, 20), (7, 9)] if model == "ruo2" else [(6, 5), (12, 9), (20, 20), (9, 7)]
+    g = np.random.default_rng(11 if model == "ruo2" else 12)
+    rates = base[None, :] * 10.0 ** g.uniform(-0.5, 0.5, (len(sizes), P))
+    userpar = g.uniform(0.6, 1.6, (len(sizes), K))
+    seeds = np.array([3, 4, 5, 6], np.uint32)
+    streams = np.array([0, 1, 2, 3], np.uint32)
+    m = otf_engine.OtfModel(cm)
+    b = otf_engine.OtfBatch(m, sizes)
+    b.set_params(rates, userpar)           # before init: init uses them
+    b.init(seeds, streams)
+    orcs = []
+    for r, s in enumerate(sizes):
+        o = otf_oracle.OtfOracle(cm.to_blob(), s)
+        o.init(int(seeds[r]), int(streams[r]))
+        o.set_params(rates[r], userpar[r])
+        orcs.append(o)
+    # replicas 1 and 3 from a random lattice
+    for r in (1, 3):
+        cfg = g.integers(0, cm.nr_of_species, int(b.volumes[r])).astype(np.int8)
+        b.set_config([cfg], r0=r)
+        assert orcs[r].set_config(cfg) == 0
+    for r, o in enumerate(orcs):
+        _compare(b, r, o, "r%d after set-up" % r, tables=True)
+    done = 0
+    for chunk in (1, 99, 1900):
+        b.run(chunk)
+        for o in orcs:
+            assert o.run(chunk) == chunk
+        done += chunk
+        for r, o in enumerate(orcs):
+            _compare(b, r, o, "%s r%d after %d" % (model, r, done))
+    # new parameters mid-run (recalculate_rates_matrix), then more steps
+    rates2 = rates * 1.7
+    b.set_params(rates2, userpar * 0.9)
+    for r, o in enumerate(orcs):
+        o.set_params(rates2[r], userpar[r] * 0.9)
+    b.run(500)
+    for r, o in enumerate(orcs):
+        o.run(500)
+        _compare(b, r, o, "%s r%d after new parameters" % (model, r), tables=True)
+
+
+def test_otf_full_lattices_sampled(otf_engine, otf_oracle):
+    """64 x 64 intZGB lattices, 128 replicas, 3000 steps; 4 replicas checked."""
+    print("case otf 64x64 x128", flush=True)
+    cm, base = _intzgb()
+    n = 128
+    g = np.random.default_rng(21)
+    rates = base[None, :] * 10.0 ** g.uniform(-0.3, 0.3, (n, cm.nr_of_proc))
+    userpar = g.uniform(0.7, 1.4, (n, len(cm.param_names)))
+    seeds = np.arange(n, dtype=np.uint32) + 100
+    m = otf_engine.OtfModel(cm)
+    b = otf_engine.OtfBatch(m, [(64, 64)] * n)
+    b.set_params(rates, userpar)
+    b.init(seeds, np.zeros(n, np.uint32))
+    b.run(3000)
+    for r in (0, 37, 90, 127):
+        o = otf_oracle.OtfOracle(cm.to_blob(), (64, 64))
+        o.init(int(seeds[r]), 0)
+        o.set_params(rates[r], userpar[r])
+        assert o.run(3000) == 3000
+        _compare(b, r, o, "64x64 r%d" % r)
+
+
+def test_otf_rejects_bad_input(otf_engine):
+    from kmos_amd import _lib
+    cm, base = _intzgb()
+    m = otf_engine.OtfModel(cm)
+    with pytest.raises(_lib.KmcgError):
+        otf_engine.OtfBatch(m, [(1, 1)])            # below the model's reach (2)
+    b = otf_engine.OtfBatch(m, [(6, 6)])
+    with pytest.raises(_lib.KmcgError):
+        b.run(10)                                   # before init
+    with pytest.raises(_lib.KmcgError):
+        b.set_params(-base, [1.0] * 5)              # negative rate constants
+    b.set_params(base, [1.0] * 5)
+    b.init([1], [0])
+    with pytest.raises(ValueError):
+        b.set_config([np.full(36, 3)])              # species 3 of 0..2
+    b.run(100)
+    assert b.scalars(0)[1] == 100
