@@ -487,6 +487,117 @@ def phase_profile(engine, wl, K, warmup):
     return out
 
 
+# ---------------------------------------------------------------------------
+# otf
+# ---------------------------------------------------------------------------
+
+def _otf_worker(args):
+    blob, size, seed, stream, rates, userpar, steps, fast = args
+    from oracle import pyoracle
+    o = pyoracle.OtfOracle(blob, size, fast=fast)
+    o.init(seed, stream)
+    o.set_params(rates, userpar)
+    t0 = time.perf_counter()
+    n = o.run(steps)
+    dt = time.perf_counter() - t0
+    return n, dt, o.work()
+
+
+def _otf_job(wl, blob, r, steps, fast):
+    return (blob, tuple(int(v) for v in wl.sizes[r][:2]), int(wl.seeds[r]), int(wl.streams[r]),
+            wl.rates[r], wl.userpar[r], steps, fast)
+
+
+def otf_algorithmic_bytes(wl, blob, sample=4, steps=5000):
+    """Per kMC step, from the oracle's work counters over `sample`
+    replicas: 16 B per list operation (the entry's cell, inverse index and
+    rate), 16 B per running-sum entry that must be re-added (its rate read,
+    its sum written: every entry from the first changed position of each
+    list on) and 8 B per probe of a binary site search."""
+    from oracle import pyoracle
+    pyoracle.build_otf()
+    tot = np.zeros(3)
+    n_steps = 0
+    for r in np.linspace(0, wl.n - 1, min(sample, wl.n)).astype(int):
+        n, _, w = _otf_worker(_otf_job(wl, blob, r, steps, False))
+        tot += w
+        n_steps += n
+    per = tot / max(n_steps, 1)
+    return 16.0 * per[0] + 16.0 * per[1] + 8.0 * per[2], {
+        "list_ops_per_step": per[0], "readded_entries_per_step": per[1], "search_probes_per_step": per[2]}
+
+
+def otf_cpu_baseline(wl, blob, seconds=15.0):
+    """The otf oracle (-O3; the reference's intZGB otf fixture does not
+    build) on the host cores, one process per core, whole replicas."""
+    from oracle import pyoracle
+    pyoracle.build_otf()
+    avail, cores = host_cores()
+    n1, dt1, _ = _otf_worker(_otf_job(wl, blob, 0, min(wl.steps, 5000), True))
+    per_replica = wl.steps / (n1 / max(dt1, 1e-9))
+    nrep = int(max(cores, min(wl.n, round(seconds * cores / per_replica))))
+    nrep = max(cores, (nrep // cores) * cores)
+    idx = np.linspace(0, wl.n - 1, nrep).astype(int)
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        res = pool.map(_otf_worker, [_otf_job(wl, blob, r, wl.steps, True) for r in idx], chunksize=1)
+    wall = time.perf_counter() - t0
+    steps = sum(r[0] for r in res)
+    return {"value": steps / wall, "unit": "kMC steps/s", "cores": cores, "cores_available": avail,
+            "cores_used": cores, "kind": "port",
+            "sample": "%d replicas x %d steps of %s (otf oracle -O3, one process per core, %.1f s wall)"
+                      % (nrep, wl.steps, wl.name, wall)}
+
+
+def otf_leg(K, warmup, cpu, wl=None):
+    """The otf engine on the intZGB workload: K timed launches after
+    `warmup`, every replica its share of the step budget per launch; replica
+    0 checked against the otf oracle afterwards."""
+    from kmos_amd import otf_engine
+    from oracle import pyoracle
+    wl = wl or W.intzgb_otf()
+    cm = compile_model(wl.project, "otf")
+    blob = cm.to_blob()
+    m = otf_engine.OtfModel(cm)
+    b = otf_engine.OtfBatch(m, wl.sizes)
+    b.set_params(wl.rates, wl.userpar)
+    b.init(wl.seeds, wl.streams)
+    chunk = max(1, wl.steps // (K + warmup))
+    for _ in range(warmup):
+        b.run(chunk)
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(K):
+        b.run(chunk)
+        kernel_ms.append(b.last_run_ms())
+    wall = time.perf_counter() - t0
+    status = [b.scalars(r)[2] for r in range(wl.n)]
+    executed = sum(b.scalars(r)[1] for r in range(wl.n)) - warmup * chunk * wl.n
+    pyoracle.build_otf()
+    o = pyoracle.OtfOracle(blob, tuple(wl.sizes[0][:2]))
+    o.init(int(wl.seeds[0]), int(wl.streams[0]))
+    o.set_params(wl.rates[0], wl.userpar[0])
+    o.run((K + warmup) * chunk)
+    s, bk, _ = b.avail(0)
+    s2, bk2, _ = o.avail()
+    spot = bool(np.array_equal(b.lattice(0), o.lattice()) and np.array_equal(s, s2) and
+                np.array_equal(b.counters(0)["procstat"], o.procstat()))
+    bps, work = otf_algorithmic_bytes(wl, blob)
+    avg_s = float(np.mean(kernel_ms)) / 1e3
+    steps_per_launch = chunk * wl.n
+    rl = roofline(wl.name, bps, steps_per_launch, avg_s)
+    rl["kernel"] = "otf_steps_kernel"
+    out = {"workload": wl.name, "replicas": int(wl.n), "steps_per_replica": int((K + warmup) * chunk),
+           "gpu_steps_per_s": executed / wall, "kmc_steps_per_launch": steps_per_launch,
+           "avg_launch_ms": avg_s * 1e3, "replica_status_counts": {k: status.count(k) for k in set(status)},
+           "parity_spot_check": spot, "roofline": rl, "work_per_step": work}
+    if cpu:
+        out["cpu_baseline"] = otf_cpu_baseline(wl, blob)
+        out["gpu_over_cpu"] = out["gpu_steps_per_s"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def calibrate_copy(nbytes=1 << 30, reps=5):
     """A plain device-to-device copy of known size: what FETCH_SIZE and
     WRITE_SIZE read for it calibrates the step kernel's counters."""
@@ -515,7 +626,7 @@ def main():
     ap.add_argument("--kmc-steps", type=int, default=None, help="kMC steps per replica")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=20000)
-    ap.add_argument("--legs", default="zgb,pairwise,quickstart,serving",
+    ap.add_argument("--legs", default="zgb,pairwise,otf,quickstart,serving",
                     help="comma-separated legs beside the contract line (empty: none)")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="request arrivals per second")
     ap.add_argument("--serve-requests", type=int, default=200)
@@ -536,6 +647,17 @@ def main():
               "them all on a single device; run with --gpus 1", file=sys.stderr)
         return 2
 
+    if a.workload == "intzgb_otf":
+        # the otf backend alone: one JSON line of its leg
+        from kmos_amd import _lib
+        _lib.load()
+        kw = {}
+        if a.replicas is not None:
+            kw["n"] = a.replicas
+        if a.kmc_steps is not None:
+            kw["steps"] = a.kmc_steps
+        print(json.dumps(otf_leg(max(1, a.steps), a.warmup, not a.no_cpu, W.intzgb_otf(**kw))))
+        return 0
     kw = {}
     if a.replicas is not None:
         kw["n"] = a.replicas
@@ -601,6 +723,9 @@ def main():
     if "pairwise" in legs and a.workload != "pairwise":
         out["pairwise"] = batched_leg(engine, W.pairwise(), K, 1, not a.no_tune, not a.no_cpu)[0]
         _progress("pairwise leg done")
+    if "otf" in legs:
+        out["otf"] = otf_leg(K, 1, not a.no_cpu)
+        _progress("otf leg done")
     if "quickstart" in legs and a.workload != "quickstart":
         out["quickstart"] = quickstart_leg(engine, not a.no_cpu)
         _progress("quickstart leg done")

@@ -134,6 +134,83 @@ __device__ __forceinline__ double otf_resum(const double *rrow, double *prow, in
     return a;
 }
 
+// lane l receives v from lane l-1 of its row of 16 (DPP row_shr:1); the
+// first lane of each row receives 0
+__device__ __forceinline__ double row_shr1z(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, 0x111, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), 0x111, 0xF, 0xF, true);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// The running sums of every list a step changed, re-added from its first
+// changed position to its end, four lists at a time: each row of 16 lanes
+// walks one list in blocks of 16 entries (one coalesced read), and within
+// a block the sum is carried left to right through the row by 15 shifted
+// adds, so lane i ends with ((carry + r0) + r1) + ... + ri, the rounding of
+// update_accum_rate's loop.  `dirty` and `n` are per owner lane.
+__device__ __forceinline__ void otf_resum_rows(const double *rm, double *pre, long long base, int C, int lane,
+                                               bool want, int dirty, int n)
+{
+    unsigned long long dm = __ballot(want && dirty < n);
+    const int row = lane >> 4, i = lane & 15;
+    while (dm) {
+        int qs[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            qs[k] = dm ? __ffsll((long long)dm) - 1 : -1;
+            if (dm)
+                dm &= dm - 1;
+        }
+        int d4[4], n4[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            d4[k] = qs[k] >= 0 ? readlane_i(dirty, qs[k]) : 0;
+            n4[k] = qs[k] >= 0 ? readlane_i(n, qs[k]) : 0;
+        }
+        const int q = row == 0 ? qs[0] : (row == 1 ? qs[1] : (row == 2 ? qs[2] : qs[3]));
+        const int d = row == 0 ? d4[0] : (row == 1 ? d4[1] : (row == 2 ? d4[2] : d4[3]));
+        const int len = (row == 0 ? n4[0] : (row == 1 ? n4[1] : (row == 2 ? n4[2] : n4[3]))) - d;
+        const int maxlen = max(max(n4[0] - d4[0], n4[1] - d4[1]), max(n4[2] - d4[2], n4[3] - d4[3]));
+        const double *rr = rm + base + (long long)(q < 0 ? 0 : q) * C;
+        double *pr = pre + base + (long long)(q < 0 ? 0 : q) * C;
+        double carry = (q >= 0 && d > 0) ? pr[d - 1] : 0.0;
+        // eight blocks per trip, the next eight read while these are summed
+        constexpr int NB = 8;
+        double v[NB], vn[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+            v[b] = (16 * b + i < len) ? rr[d + 16 * b + i] : 0.0;
+        for (int off = 0; off < maxlen; off += 16 * NB) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int jn = off + 16 * (NB + b) + i;
+                vn[b] = (jn < len) ? rr[d + jn] : 0.0;
+            }
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int j = off + 16 * b;
+                if (j >= maxlen)
+                    break;
+                const double ce = (i == 0) ? carry + v[b] : v[b];
+                double a = ce;
+#pragma unroll
+                for (int k = 1; k < 16; ++k)
+                    a = row_shr1z(a) + ce;
+                if (j + i < len)
+                    pr[d + j + i] = a;
+                const double c0 = readlane_d(a, 15), c1 = readlane_d(a, 31), c2 = readlane_d(a, 47),
+                             c3 = readlane_d(a, 63);
+                carry = row == 0 ? c0 : (row == 1 ? c1 : (row == 2 ? c2 : c3));
+            }
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+                v[b] = vn[b];
+        }
+    }
+}
+
 __device__ __forceinline__ bool otf_conds(const OtfParams &p, const OtfCell &g, const signed char *slat,
                                           const int *cd, int ncond, int x, int y, int z)
 {
@@ -383,8 +460,56 @@ __global__ __launch_bounds__(64) void otf_steps_kernel(OtfParams p)
             slat[site] = (signed char)ww[5];
         }
         __syncthreads();
-        // the owner lane's rate updates and adds, in program order
-        for (int k = 0; k < nrest; ++k) {
+        // the owner lane's rate updates, which come before its adds: in
+        // batches of up to eight, their inverse indices read together, then
+        // their old rates, then the new ones applied in program order
+        int k = 0;
+        for (;;) {
+            int m = 0;
+            while (m < 8 && k + m < nrest && w[4 * m] == 2)
+                ++m;
+            if (m == 0)
+                break;
+            int cx8[8], cy8[8], cz8[8], c8[8], bk8[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                if (j < m) {
+                    cx8[j] = otf_wrap(x + w[4 * j + 1], g.X);
+                    cy8[j] = otf_wrap(y + w[4 * j + 2], g.Y);
+                    cz8[j] = otf_wrap(z + w[4 * j + 3], g.Z);
+                    c8[j] = g.idx(cx8[j], cy8[j], cz8[j]);
+                    bk8[j] = brow[c8[j]];
+                }
+            }
+            double old8[8], new8[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j < m && bk8[j] > 0 && bk8[j] <= n)
+                    old8[j] = rrow[bk8[j] - 1];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                if (j < m) {
+                    if (bk8[j] > n) {
+                        corrupt = true;
+                    } else if (bk8[j]) {
+                        const int at = bk8[j] - 1;
+                        double old = old8[j];
+#pragma unroll
+                        for (int j2 = 0; j2 < j; ++j2)       // the same cell twice in one batch
+                            if (c8[j2] == c8[j] && bk8[j2] == bk8[j])
+                                old = new8[j2];
+                        new8[j] = otf_gr(p, g, slat, tab, q, cx8[j], cy8[j], cz8[j]);
+                        run = run + new8[j] - old;
+                        rrow[at] = new8[j];
+                        dirty = min(dirty, at);
+                    }
+                }
+            }
+            w += 4 * m;
+            k += m;
+        }
+        // then its adds, in program order
+        for (; k < nrest; ++k) {
             const int cx = otf_wrap(x + w[1], g.X), cy = otf_wrap(y + w[2], g.Y), cz = otf_wrap(z + w[3], g.Z);
             const int c = g.idx(cx, cy, cz);
             if (w[0] == 2) {
@@ -414,9 +539,8 @@ __global__ __launch_bounds__(64) void otf_steps_kernel(OtfParams p)
                 w += 5 + 5 * ncond;
             }
         }
-        if (dirty < n)
-            tot = otf_resum(rrow, prow, dirty, n);
-        else if (dirty != 0x7fffffff)
+        otf_resum_rows(p.rm, p.pre, base, g.C, lane, act, dirty, n);
+        if (dirty != 0x7fffffff)
             tot = n > 0 ? prow[n - 1] : 0.0;
         if (__ballot(bad)) {
             st = KMCG_REPLICA_SPECIES_MISMATCH;

@@ -48,6 +48,11 @@ typedef struct {
     uint32_t seed, stream;
     uint64_t ctr;
     int status;
+    /* work counters (the bench's algorithmic bytes): list operations, list
+     * entries whose running sum a step must re-add (from the first changed
+     * position of each list on) and interval-search probes of a binary search */
+    int32_t *dmin;
+    uint64_t n_ops, n_readd, n_probe;
 } otf_t;
 
 #define HDR(o, i) ((o)->blob[(i)])
@@ -255,6 +260,13 @@ static double gr(const otf_t *o, int q, const int c[3])
 #define A2(o, p, s) ((o)->a2[(size_t)(p) * ((o)->V + 1) + (s)])
 #define RM(o, p, i) ((o)->rm[(size_t)(p) * ((o)->V + 2) + (i)])
 
+static void touch(otf_t *o, int p, int pos)
+{
+    o->n_ops += 1;
+    if (pos - 1 < o->dmin[p])
+        o->dmin[p] = pos - 1;
+}
+
 static void del_proc(otf_t *o, int p, int site)
 {
     int n = o->nsites[p];
@@ -263,6 +275,7 @@ static void del_proc(otf_t *o, int p, int site)
         o->status = OTF_DEL_ABSENT;
         return;
     }
+    touch(o, p, mem);
     if (mem < n) {
         A1(o, p, mem) = A1(o, p, n);
         A1(o, p, n) = 0;
@@ -282,6 +295,7 @@ static void del_proc(otf_t *o, int p, int site)
 static void add_proc(otf_t *o, int p, int site, double rate)
 {
     int n = ++o->nsites[p];
+    touch(o, p, n);
     A1(o, p, n) = site;
     A2(o, p, site) = n;
     RM(o, p, o->V + 1) = RM(o, p, o->V + 1) + rate;
@@ -291,6 +305,7 @@ static void add_proc(otf_t *o, int p, int site, double rate)
 static void update_rates_matrix(otf_t *o, int p, int site, double rate)
 {
     int mem = A2(o, p, site);
+    touch(o, p, mem);
     RM(o, p, o->V + 1) = RM(o, p, o->V + 1) + rate - RM(o, p, mem);
     RM(o, p, mem) = rate;
 }
@@ -403,6 +418,7 @@ otf_t *otf_create(const int32_t *blob, int64_t nwords, int32_t X, int32_t Y, int
     o->integ = calloc(P, sizeof(double));
     o->arr = calloc(V + 1, sizeof(double));
     o->procstat = calloc(P, sizeof(int64_t));
+    o->dmin = calloc(P, sizeof(int32_t));
     return o;
 }
 
@@ -412,7 +428,7 @@ void otf_free(otf_t *o)
         return;
     free(o->blob); free(o->lat); free(o->a1); free(o->a2); free(o->rm); free(o->nsites);
     free(o->rates); free(o->userpar); free(o->table); free(o->accum); free(o->integ);
-    free(o->arr); free(o->procstat); free(o);
+    free(o->arr); free(o->procstat); free(o->dmin); free(o);
 }
 
 /* initialize_state: every site its default species, touchup_cell over the
@@ -534,8 +550,15 @@ int64_t otf_run(otf_t *o, int64_t n, int32_t *tproc, int32_t *tsite, double *tti
         o->procstat[proc - 1] += 1;                                 /* run_proc_nr */
         int c[3];
         cell_of(o, site, c);
+        for (int p = 0; p < o->P; ++p)
+            o->dmin[p] = 0x7fffffff;
+        for (int m = ns; m > 1; m = (m + 1) / 2)
+            o->n_probe += 1;
         const int32_t *pt = o->blob + HDR(o, 9) + 2 * (proc - 1);
         run_ops(o, o->blob + pt[1], pt[0], c);
+        for (int p = 0; p < o->P; ++p)
+            if (o->dmin[p] < o->nsites[p])
+                o->n_readd += (uint64_t)(o->nsites[p] - o->dmin[p]);
     }
     return done;
 }
@@ -544,6 +567,13 @@ int otf_status(const otf_t *o) { return o->status; }
 double otf_kmc_time(const otf_t *o) { return o->kmc_time; }
 int64_t otf_kmc_step(const otf_t *o) { return o->kmc_step; }
 uint64_t otf_rng_ctr(const otf_t *o) { return o->ctr; }
+/* list operations, re-added running-sum entries, search probes so far */
+void otf_work(const otf_t *o, uint64_t out[3])
+{
+    out[0] = o->n_ops;
+    out[1] = o->n_readd;
+    out[2] = o->n_probe;
+}
 int otf_volume(const otf_t *o) { return o->V; }
 void otf_get_lattice(const otf_t *o, int8_t *out) { memcpy(out, o->lat + 1, o->V); }
 void otf_get_nr(const otf_t *o, int32_t *out) { memcpy(out, o->nsites, sizeof(int32_t) * o->P); }

@@ -371,6 +371,7 @@ def otf_lib(fast=False):
             getattr(L, name).argtypes = [vp, vp]
         L.otf_get_avail.argtypes = [vp, vp, vp]
         L.otf_set_clock.argtypes = [vp, dbl, i64, u64]
+        L.otf_work.argtypes = [vp, vp]
         _otf_libs[fast] = L
     return _otf_libs[fast]
 
@@ -460,6 +461,11 @@ class OtfOracle:
 
     def table(self):
         return self._get("otf_get_table", np.float64, int(self.blob[13]))
+
+    def work(self):
+        """(list operations, re-added running-sum entries, binary-search
+        probes) since creation."""
+        return tuple(int(v) for v in self._get("otf_work", np.uint64, 3))
 
     def avail(self):
         """(sites [P][V], back [P][V], rates_matrix [P][V+1])"""
