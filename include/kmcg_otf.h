@@ -15,8 +15,10 @@
  * otf sections documented in _otf_blob there).
  *
  * Sizes: one entry of `sizes` per replica, (X, Y, Z) cells; each extent at
- * least the model's largest offset, at most 65536 sites per replica; at
- * most 64 processes and one layer.
+ * least the model's largest offset; the sites of a replica (one byte each)
+ * and the model's packed program share 64 KiB of LDS (the program of the
+ * RuO2 model takes about 20 KiB, of the intZGB model about 6 KiB); at most
+ * 64 processes, 32 sites per cell, 64 species and one layer.
  */
 #ifndef KMCG_OTF_H
 #define KMCG_OTF_H

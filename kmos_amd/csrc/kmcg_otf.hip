@@ -32,11 +32,24 @@ namespace {
 
 constexpr int OTF_MAX_P = 64;
 constexpr int OTF_MAX_V = 65536;     // sites per replica (one byte each in LDS)
+constexpr int OTF_LDS = 65536;       // LDS bytes a replica's wavefront may use
 
+// The device program, compact int32 words (built by otf_build):
+//   [0, spuck)        site defaults
+//   owner [P][P]      run_proc_<p>'s ops for process q: off | n_del << 16 | n_rest << 24
+//   writes [P]        off | count << 16 -> 2 words per write: packed site, old | new << 16
+//   byst [P]          off | count << 16 -> per bystander: packed site, then S strides
+//   tbase [P]         first table entry of process q
+//   touch [P]         off | count << 16 -> touchup_cell's add ops of process q
+// op word:   kind (2 bits: 0 del, 2 update, 3 add) | dx, dy, dz (8 bits each) << 2, 10, 18
+//            | ncond << 26 (adds)
+// cond word: dx, dy, dz (7 bits each) | site - 1 (5 bits) << 21 | species << 26
+// site word: dx, dy, dz (8 bits each) | site - 1 << 24
 struct OtfParams {
-    int P, S, spuck, T;
-    const int *prog;
+    int P, S, spuck, T, W;
+    const int *prog;                 // global copy of the program (W words)
     int off_owner, off_writes, off_byst, off_tbase, off_touch;
+    int tab_lds;                     // 1: the replica's rate table is staged in LDS too
     const int *dims;                 // [n][4]: X, Y, Z, C
     const long long *lat_off;        // [n]: first site of the replica in `lat`
     const long long *list_off;       // [n]: first entry of the replica's rows
@@ -58,6 +71,8 @@ struct OtfParams {
 };
 
 __device__ __forceinline__ int otf_wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+__device__ __forceinline__ int sx8(int w, int sh) { return (int)(signed char)((w >> sh) & 0xFF); }
+__device__ __forceinline__ int sx7(int w, int sh) { return (((w >> sh) & 0x7F) ^ 0x40) - 0x40; }
 
 struct OtfCell {
     int X, Y, Z, C;
@@ -69,32 +84,40 @@ struct OtfCell {
         y = yz % Y;
         z = yz / Y;
     }
+    // the cell at (x, y, z) + (dx, dy, dz), each offset at most one period
+    __device__ __forceinline__ int at(int x, int y, int z, int dx, int dy, int dz) const
+    {
+        return idx(otf_wrap(x + dx, X), otf_wrap(y + dy, Y), otf_wrap(z + dz, Z));
+    }
 };
 
 // gr_<q>(cell): the table entry of the bystander counts around (x, y, z)
-__device__ __forceinline__ double otf_gr(const OtfParams &p, const OtfCell &g, const signed char *slat,
-                                         const double *tab, int q, int x, int y, int z)
+__device__ __forceinline__ double otf_gr(const int *prog, const OtfParams &p, const OtfCell &g,
+                                         const signed char *slat, const double *tab, int q, int x, int y, int z)
 {
-    const int *bt = p.prog + p.off_byst + 2 * q;
-    const int *b = p.prog + bt[0];
+    const int bt = prog[p.off_byst + q];
+    const int *b = prog + (bt & 0xFFFF);
+    const int nb = bt >> 16;
     int idx = 0;
-    for (int k = 0; k < bt[1]; ++k, b += 4 + p.S) {
-        const int c = g.idx(otf_wrap(x + b[0], g.X), otf_wrap(y + b[1], g.Y), otf_wrap(z + b[2], g.Z));
-        const int sp = slat[p.spuck * c + b[3] - 1];
+    for (int k = 0; k < nb; ++k, b += 1 + p.S) {
+        const int w = b[0];
+        const int c = g.at(x, y, z, sx8(w, 0), sx8(w, 8), sx8(w, 16));
+        const int sp = slat[p.spuck * c + ((w >> 24) & 0xFF)];
         if (sp >= 0 && sp < p.S)
-            idx += b[4 + sp];
+            idx += b[1 + sp];
     }
-    return tab[p.prog[p.off_tbase + q] + idx];
+    return tab[prog[p.off_tbase + q] + idx];
 }
 
 // del_proc (base_otf.f90:221-282) on the owner lane's rows: the last entry
-// moves into the freed position
+// moves into the freed position.  Returns the freed position, -1 when the
+// inverse index points outside the list.
 __device__ __forceinline__ int otf_del(int *srow, int *brow, double *rrow, int &n, double &run, int c)
 {
     const int pos = brow[c] - 1;
     const int last = n - 1;
     if (pos < 0 || pos > last)
-        return -1;                       // inconsistent list: the caller stops
+        return -1;
     run = run - rrow[pos];
     if (pos < last) {
         const int mv = srow[last];
@@ -111,7 +134,7 @@ __device__ __forceinline__ int otf_del(int *srow, int *brow, double *rrow, int &
 }
 
 // the running sum of a row from position `from` on (update_accum_rate's
-// order: left to right); returns the row's total
+// order: left to right), one lane alone; returns the row's total
 __device__ __forceinline__ double otf_resum(const double *rrow, double *prow, int from, int n)
 {
     double a = from > 0 ? prow[from - 1] : 0.0;
@@ -214,9 +237,10 @@ __device__ __forceinline__ void otf_resum_rows(const double *rm, double *pre, lo
 __device__ __forceinline__ bool otf_conds(const OtfParams &p, const OtfCell &g, const signed char *slat,
                                           const int *cd, int ncond, int x, int y, int z)
 {
-    for (int j = 0; j < ncond; ++j, cd += 5) {
-        const int c = g.idx(otf_wrap(x + cd[0], g.X), otf_wrap(y + cd[1], g.Y), otf_wrap(z + cd[2], g.Z));
-        if (slat[p.spuck * c + cd[3] - 1] != cd[4])
+    for (int j = 0; j < ncond; ++j) {
+        const int w = cd[j];
+        const int c = g.at(x, y, z, sx7(w, 0), sx7(w, 7), sx7(w, 14));
+        if (slat[p.spuck * c + ((w >> 21) & 0x1F)] != ((w >> 26) & 0x3F))
             return false;
     }
     return true;
@@ -227,7 +251,8 @@ __device__ __forceinline__ bool otf_conds(const OtfParams &p, const OtfCell &g, 
 // (touchup_cell with x outermost: delete the cell's entry, then add it if
 // the conditions hold); MODE 2: recalculate_rates_matrix (every listed
 // entry takes its cell's rate under new parameters).  Each ends with the
-// rows' running sums.
+// rows' running sums.  The program and tables are read from global memory
+// (a one-off pass).
 __global__ __launch_bounds__(64) void otf_setup_kernel(OtfParams p)
 {
     extern __shared__ signed char slat[];
@@ -235,12 +260,11 @@ __global__ __launch_bounds__(64) void otf_setup_kernel(OtfParams p)
     const int lane = threadIdx.x;
     const OtfCell g{p.dims[4 * r], p.dims[4 * r + 1], p.dims[4 * r + 2], p.dims[4 * r + 3]};
     const int V = g.C * p.spuck;
+    const int *prog = p.prog;
     signed char *glat = p.lat + p.lat_off[r];
     if (p.mode == 0) {
-        const int *def = p.prog;          // site defaults head the program
-        for (int i = lane; i < V; i += 64) {
-            glat[i] = (signed char)def[i % p.spuck];
-        }
+        for (int i = lane; i < V; i += 64)
+            glat[i] = (signed char)prog[i % p.spuck];     // site defaults head the program
         __syncthreads();
     }
     for (int i = lane; i < V; i += 64)
@@ -273,14 +297,14 @@ __global__ __launch_bounds__(64) void otf_setup_kernel(OtfParams p)
                 continue;
             int x, y, z;
             g.xyz(srow[j] - 1, x, y, z);
-            rrow[j] = otf_gr(p, g, slat, tab, q, x, y, z);
+            rrow[j] = otf_gr(prog, p, g, slat, tab, q, x, y, z);
         }
         // reaccumulate_rates_matrix (base_otf.f90:366-387): memory order
         run = 0.0;
         for (int j = 0; j < n; ++j)
             run = run + rrow[j];
     } else {
-        const int *tt = p.prog + p.off_touch + 2 * q;
+        const int tt = prog[p.off_touch + q];
         for (int k = 0; k < g.C; ++k) {
             int x, y, z;
             if (p.mode == 0) {
@@ -297,18 +321,18 @@ __global__ __launch_bounds__(64) void otf_setup_kernel(OtfParams p)
                 p.status[r] = KMCG_REPLICA_CORRUPT;
                 break;
             }
-            const int *op = p.prog + tt[0];
-            for (int m = 0; m < tt[1]; ++m) {
-                const int ncond = op[4];
-                if (otf_conds(p, g, slat, op + 5, ncond, x, y, z) && n < g.C) {
-                    const double rate = otf_gr(p, g, slat, tab, q, x, y, z);
+            const int *op = prog + (tt & 0xFFFF);
+            for (int m = 0; m < (tt >> 16); ++m) {
+                const int ncond = (op[0] >> 26) & 0x3F;
+                if (otf_conds(p, g, slat, op + 1, ncond, x, y, z) && n < g.C) {
+                    const double rate = otf_gr(prog, p, g, slat, tab, q, x, y, z);
                     srow[n] = c + 1;
                     brow[c] = n + 1;
                     rrow[n] = rate;
                     run = run + rate;
                     n += 1;
                 }
-                op += 5 + 5 * ncond;
+                op += 1 + ncond;
             }
         }
     }
@@ -328,19 +352,31 @@ __global__ __launch_bounds__(64) void otf_setup_kernel(OtfParams p)
     }
 }
 
-// do_kmc_steps for `steps` steps (fewer when the replica stops)
+// do_kmc_steps for `steps` steps (fewer when the replica stops).  LDS holds
+// the program, the replica's rate table (TLDS) and its lattice.
+template <bool TLDS>
 __global__ __launch_bounds__(64) void otf_steps_kernel(OtfParams p)
 {
-    extern __shared__ signed char slat[];
+    extern __shared__ __align__(16) int otf_smem[];
     const int r = p.rep_ids[blockIdx.x];
     const int lane = threadIdx.x;
     const int P = p.P;
     const OtfCell g{p.dims[4 * r], p.dims[4 * r + 1], p.dims[4 * r + 2], p.dims[4 * r + 3]};
     const int V = g.C * p.spuck;
+    int *prog = otf_smem;
+    const int W2 = (p.W + 1) & ~1;
+    double *stab = reinterpret_cast<double *>(otf_smem + W2);
+    signed char *slat = reinterpret_cast<signed char *>(otf_smem + W2 + (TLDS ? 2 * p.T : 0));
     signed char *glat = p.lat + p.lat_off[r];
+    for (int i = lane; i < p.W; i += 64)
+        prog[i] = p.prog[i];
+    if (TLDS)
+        for (int i = lane; i < p.T; i += 64)
+            stab[i] = p.table[(long long)r * p.T + i];
     for (int i = lane; i < V; i += 64)
         slat[i] = glat[i];
     __syncthreads();
+    const double *tab = TLDS ? stab : p.table + (long long)r * p.T;
     int st = p.status[r];
     if (st != KMCG_REPLICA_OK)
         return;
@@ -352,7 +388,6 @@ __global__ __launch_bounds__(64) void otf_steps_kernel(OtfParams p)
     int *brow = p.back + lo;
     double *rrow = p.rm + lo;
     double *prow = p.pre + lo;
-    const double *tab = p.table + (long long)r * p.T;
     const int slot = r * P + q;
     int n = act ? p.nsites[slot] : 0;
     double tot = (act && n > 0) ? prow[n - 1] : 0.0;
@@ -435,12 +470,12 @@ __global__ __launch_bounds__(64) void otf_steps_kernel(OtfParams p)
         // run_proc_<proc>: the owner lane's deletes
         int dirty = 0x7fffffff;
         bool corrupt = false;
-        const int *own = p.prog + p.off_owner + 3 * (proc * P + q);
-        const int *w = p.prog + own[0];
-        const int nd = act ? own[1] : 0;
-        const int nrest = act ? own[2] : 0;
-        for (int k = 0; k < nd; ++k, w += 4) {
-            const int c = g.idx(otf_wrap(x + w[1], g.X), otf_wrap(y + w[2], g.Y), otf_wrap(z + w[3], g.Z));
+        const int own = prog[p.off_owner + proc * P + q];
+        const int *w = prog + (own & 0xFFFF);
+        const int nd = act ? (own >> 16) & 0xFF : 0;
+        const int nrest = act ? (own >> 24) & 0xFF : 0;
+        for (int k = 0; k < nd; ++k, ++w) {
+            const int c = g.at(x, y, z, sx8(w[0], 2), sx8(w[0], 10), sx8(w[0], 18));
             if (brow[c]) {
                 const int d = otf_del(srow, brow, rrow, n, run, c);
                 if (d < 0)
@@ -450,14 +485,14 @@ __global__ __launch_bounds__(64) void otf_steps_kernel(OtfParams p)
             }
         }
         // lattice writes
-        const int *wt = p.prog + p.off_writes + 2 * proc;
+        const int wt = prog[p.off_writes + proc];
         bool bad = false;
-        if (lane < wt[1]) {
-            const int *ww = p.prog + wt[0] + 6 * lane;
-            const int c = g.idx(otf_wrap(x + ww[0], g.X), otf_wrap(y + ww[1], g.Y), otf_wrap(z + ww[2], g.Z));
-            const int site = p.spuck * c + ww[3] - 1;
-            bad = slat[site] != ww[4];
-            slat[site] = (signed char)ww[5];
+        if (lane < (wt >> 16)) {
+            const int *ww = prog + (wt & 0xFFFF) + 2 * lane;
+            const int c = g.at(x, y, z, sx8(ww[0], 0), sx8(ww[0], 8), sx8(ww[0], 16));
+            const int site = p.spuck * c + ((ww[0] >> 24) & 0xFF);
+            bad = slat[site] != (ww[1] & 0xFFFF);
+            slat[site] = (signed char)(ww[1] >> 16);
         }
         __syncthreads();
         // the owner lane's rate updates, which come before its adds: in
@@ -466,7 +501,7 @@ __global__ __launch_bounds__(64) void otf_steps_kernel(OtfParams p)
         int k = 0;
         for (;;) {
             int m = 0;
-            while (m < 8 && k + m < nrest && w[4 * m] == 2)
+            while (m < 8 && k + m < nrest && (w[m] & 3) == 2)
                 ++m;
             if (m == 0)
                 break;
@@ -474,9 +509,9 @@ __global__ __launch_bounds__(64) void otf_steps_kernel(OtfParams p)
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 if (j < m) {
-                    cx8[j] = otf_wrap(x + w[4 * j + 1], g.X);
-                    cy8[j] = otf_wrap(y + w[4 * j + 2], g.Y);
-                    cz8[j] = otf_wrap(z + w[4 * j + 3], g.Z);
+                    cx8[j] = otf_wrap(x + sx8(w[j], 2), g.X);
+                    cy8[j] = otf_wrap(y + sx8(w[j], 10), g.Y);
+                    cz8[j] = otf_wrap(z + sx8(w[j], 18), g.Z);
                     c8[j] = g.idx(cx8[j], cy8[j], cz8[j]);
                     bk8[j] = brow[c8[j]];
                 }
@@ -498,37 +533,39 @@ __global__ __launch_bounds__(64) void otf_steps_kernel(OtfParams p)
                         for (int j2 = 0; j2 < j; ++j2)       // the same cell twice in one batch
                             if (c8[j2] == c8[j] && bk8[j2] == bk8[j])
                                 old = new8[j2];
-                        new8[j] = otf_gr(p, g, slat, tab, q, cx8[j], cy8[j], cz8[j]);
+                        new8[j] = otf_gr(prog, p, g, slat, tab, q, cx8[j], cy8[j], cz8[j]);
                         run = run + new8[j] - old;
                         rrow[at] = new8[j];
                         dirty = min(dirty, at);
                     }
                 }
             }
-            w += 4 * m;
+            w += m;
             k += m;
         }
         // then its adds, in program order
         for (; k < nrest; ++k) {
-            const int cx = otf_wrap(x + w[1], g.X), cy = otf_wrap(y + w[2], g.Y), cz = otf_wrap(z + w[3], g.Z);
+            const int op = w[0];
+            const int cx = otf_wrap(x + sx8(op, 2), g.X), cy = otf_wrap(y + sx8(op, 10), g.Y),
+                      cz = otf_wrap(z + sx8(op, 18), g.Z);
             const int c = g.idx(cx, cy, cz);
-            if (w[0] == 2) {
+            if ((op & 3) == 2) {
                 if (brow[c] > n) {
                     corrupt = true;
                 } else if (brow[c]) {
                     const int at = brow[c] - 1;
-                    const double rate = otf_gr(p, g, slat, tab, q, cx, cy, cz);
+                    const double rate = otf_gr(prog, p, g, slat, tab, q, cx, cy, cz);
                     run = run + rate - rrow[at];
                     rrow[at] = rate;
                     dirty = min(dirty, at);
                 }
-                w += 4;
+                w += 1;
             } else {
-                const int ncond = w[4];
+                const int ncond = (op >> 26) & 0x3F;
                 if (n >= g.C) {
                     corrupt = true;             // an add to a full list
-                } else if (otf_conds(p, g, slat, w + 5, ncond, x, y, z)) {
-                    const double rate = otf_gr(p, g, slat, tab, q, cx, cy, cz);
+                } else if (otf_conds(p, g, slat, w + 1, ncond, x, y, z)) {
+                    const double rate = otf_gr(prog, p, g, slat, tab, q, cx, cy, cz);
                     srow[n] = c + 1;
                     brow[c] = n + 1;
                     rrow[n] = rate;
@@ -536,7 +573,7 @@ __global__ __launch_bounds__(64) void otf_steps_kernel(OtfParams p)
                     dirty = min(dirty, n);
                     n += 1;
                 }
-                w += 5 + 5 * ncond;
+                w += 1 + ncond;
             }
         }
         otf_resum_rows(p.rm, p.pre, base, g.C, lane, act, dirty, n);
@@ -717,21 +754,55 @@ bool otf_eval(const int32_t *code, int ncode, const int *counts, double base, co
     return true;
 }
 
+// packed words of the device program (see the layout above)
+int otf_site_word(int dx, int dy, int dz, int n1)
+{
+    return (dx & 0xFF) | (dy & 0xFF) << 8 | (dz & 0xFF) << 16 | (n1 - 1) << 24;
+}
+
+int otf_op_word(int kind, int dx, int dy, int dz, int ncond)
+{
+    return kind | (dx & 0xFF) << 2 | (dy & 0xFF) << 10 | (dz & 0xFF) << 18 | ncond << 26;
+}
+
+int otf_cond_word(int dx, int dy, int dz, int n1, int sp)
+{
+    return (dx & 0x7F) | (dy & 0x7F) << 7 | (dz & 0x7F) << 14 | (n1 - 1) << 21 | sp << 26;
+}
+
 int otf_build(kmcg_otf_model *m)
 {
     const std::vector<int32_t> &B = m->blob;
     const int P = m->P, S = m->S;
+    if (m->spuck > 32 || S > 64)
+        return fail(KMCG_EUNSUPPORTED, "otf engine: at most 32 sites per cell and 64 species");
     auto in = [&](long long off, long long n) { return off >= 0 && n >= 0 && off + n <= (long long)B.size(); };
+    auto small = [](int v, int lim) { return v >= -lim && v <= lim; };
     std::vector<int> &g = m->prog;
     g.assign(B.begin() + B[8], B.begin() + B[8] + m->spuck);   // site defaults first
     // flat run_proc programs -> per (exec, owner) lists and per-exec writes
     std::vector<std::vector<std::vector<int>>> own(P, std::vector<std::vector<int>>(P));
-    std::vector<std::vector<int>> ndel(P, std::vector<int>(P, 0));
+    std::vector<std::vector<int>> ndel(P, std::vector<int>(P, 0)), nrest(P, std::vector<int>(P, 0));
     std::vector<std::vector<int>> writes(P);
     auto reach = [&](int dx, int dy, int dz) {
         m->reach[0] = std::max(m->reach[0], std::abs(dx));
         m->reach[1] = std::max(m->reach[1], std::abs(dy));
         m->reach[2] = std::max(m->reach[2], std::abs(dz));
+    };
+    // an add op and its conditions, packed; false when out of the packing's range
+    auto add_op = [&](std::vector<int> &out, const int32_t *op) {
+        if (op[5] > 63 || !small(op[2], 127) || !small(op[3], 127) || !small(op[4], 127))
+            return false;
+        out.push_back(otf_op_word(3, op[2], op[3], op[4], op[5]));
+        for (int j = 0; j < op[5]; ++j) {
+            const int32_t *c = op + 6 + 5 * j;
+            if (c[3] < 1 || c[3] > m->spuck || c[4] < 0 || c[4] >= S || !small(c[0], 63) ||
+                !small(c[1], 63) || !small(c[2], 63))
+                return false;
+            reach(c[0], c[1], c[2]);
+            out.push_back(otf_cond_word(c[0], c[1], c[2], c[3], c[4]));
+        }
+        return true;
     };
     auto parse = [&](long long off, int nops, auto &&emit) -> bool {
         long long k = off;
@@ -756,83 +827,69 @@ int otf_build(kmcg_otf_model *m)
         const bool ok = parse(B[off + 1], B[off], [&](const int32_t *op) {
             const int kind = op[0];
             if (kind == 1) {
-                if (op[5] < 0 || op[5] >= S || op[6] < 0 || op[6] >= S || op[4] < 1 || op[4] > m->spuck)
+                if (op[5] < 0 || op[5] >= S || op[6] < 0 || op[6] >= S || op[4] < 1 || op[4] > m->spuck ||
+                    !small(op[1], 127) || !small(op[2], 127) || !small(op[3], 127))
                     return false;
-                writes[p].insert(writes[p].end(), op + 1, op + 7);
+                writes[p].push_back(otf_site_word(op[1], op[2], op[3], op[4]));
+                writes[p].push_back(op[5] | op[6] << 16);
                 reach(op[1], op[2], op[3]);
                 after_write = true;
                 return true;
             }
             const int q = op[1];
-            if (q < 0 || q >= P)
+            if (q < 0 || q >= P || !small(op[2], 127) || !small(op[3], 127) || !small(op[4], 127))
                 return false;
             reach(op[2], op[3], op[4]);
             if (kind == 0) {
-                if (after_write || own[p][q].size() != (size_t)4 * ndel[p][q])
+                if (after_write || nrest[p][q] > 0)
                     return false;                    // deletes come first
-                own[p][q].insert(own[p][q].end(), {0, op[2], op[3], op[4]});
+                own[p][q].push_back(otf_op_word(0, op[2], op[3], op[4], 0));
                 ndel[p][q] += 1;
             } else if (kind == 2) {
-                own[p][q].insert(own[p][q].end(), {2, op[2], op[3], op[4]});
+                own[p][q].push_back(otf_op_word(2, op[2], op[3], op[4], 0));
+                nrest[p][q] += 1;
             } else {
-                own[p][q].insert(own[p][q].end(), {3, op[2], op[3], op[4], op[5]});
-                for (int j = 0; j < op[5]; ++j) {
-                    const int32_t *c = op + 6 + 5 * j;
-                    if (c[3] < 1 || c[3] > m->spuck || c[4] < 0 || c[4] >= S)
-                        return false;
-                    reach(c[0], c[1], c[2]);
-                    own[p][q].insert(own[p][q].end(), c, c + 5);
-                }
+                if (!add_op(own[p][q], op))
+                    return false;
+                nrest[p][q] += 1;
             }
             return true;
         });
         if (!ok)
             return fail(KMCG_EINVAL, "bad otf program of process " + std::to_string(p + 1));
-        if (writes[p].size() / 6 > 64)
+        if (writes[p].size() / 2 > 64)
             return fail(KMCG_EUNSUPPORTED, "more than 64 lattice writes in one process");
     }
     // touchup: per process its adds
     std::vector<std::vector<int>> touch(P);
+    std::vector<int> ntouch(P, 0);
     const bool tok = parse(B[12], B[18], [&](const int32_t *op) {
         if (op[0] != 3 || op[1] < 0 || op[1] >= P)
             return false;
-        touch[op[1]].insert(touch[op[1]].end(), {3, op[2], op[3], op[4], op[5]});
-        for (int j = 0; j < op[5]; ++j) {
-            const int32_t *c = op + 6 + 5 * j;
-            if (c[3] < 1 || c[3] > m->spuck || c[4] < 0 || c[4] >= S)
-                return false;
-            reach(c[0], c[1], c[2]);
-            touch[op[1]].insert(touch[op[1]].end(), c, c + 5);
-        }
-        return true;
+        ntouch[op[1]] += 1;
+        return add_op(touch[op[1]], op);
     });
     if (!tok)
         return fail(KMCG_EINVAL, "bad otf touchup program");
-    // owner table [P][P][3], writes [P][2], bystanders [P][2], table bases [P], touch [P][2]
     m->off_owner = (int)g.size();
-    g.resize(g.size() + 3 * P * P);
+    g.resize(g.size() + P * P);
     m->off_writes = (int)g.size();
-    g.resize(g.size() + 2 * P);
+    g.resize(g.size() + P);
     m->off_byst = (int)g.size();
-    g.resize(g.size() + 2 * P);
+    g.resize(g.size() + P);
     m->off_tbase = (int)g.size();
     g.resize(g.size() + P);
     m->off_touch = (int)g.size();
-    g.resize(g.size() + 2 * P);
+    g.resize(g.size() + P);
     for (int p = 0; p < P; ++p)
         for (int q = 0; q < P; ++q) {
-            const int at = m->off_owner + 3 * (p * P + q);
-            g[at] = (int)g.size();
-            g[at + 1] = ndel[p][q];
-            int rest = 0;
-            for (size_t k = 4 * ndel[p][q]; k < own[p][q].size(); k += own[p][q][k] == 3 ? 5 + 5 * own[p][q][k + 4] : 4)
-                rest += 1;
-            g[at + 2] = rest;
+            if (ndel[p][q] > 255 || nrest[p][q] > 255)
+                return fail(KMCG_EUNSUPPORTED, "more than 255 ops of one process in one routine");
+            g[m->off_owner + p * P + q] = (int)g.size() | ndel[p][q] << 16 | nrest[p][q] << 24;
             g.insert(g.end(), own[p][q].begin(), own[p][q].end());
         }
     for (int p = 0; p < P; ++p) {
-        g[m->off_writes + 2 * p] = (int)g.size();
-        g[m->off_writes + 2 * p + 1] = (int)(writes[p].size() / 6);
+        g[m->off_writes + p] = (int)g.size() | (int)(writes[p].size() / 2) << 16;
         g.insert(g.end(), writes[p].begin(), writes[p].end());
     }
     long long T = 0;
@@ -846,22 +903,22 @@ int otf_build(kmcg_otf_model *m)
             size *= (long long)B[B[rec + 8] + v] + 1;
         if (size != B[rec + 0])
             return fail(KMCG_EINVAL, "otf rate table size does not match its counts");
-        g[m->off_byst + 2 * q] = (int)g.size();
-        g[m->off_byst + 2 * q + 1] = B[rec + 2];
+        g[m->off_byst + q] = (int)g.size() | B[rec + 2] << 16;
         long long reach_idx = 0;
         for (int k = 0; k < B[rec + 2]; ++k) {
             const int32_t *b = &B[B[rec + 3] + k * (4 + S)];
-            if (b[3] < 1 || b[3] > m->spuck)
+            if (b[3] < 1 || b[3] > m->spuck || !small(b[0], 127) || !small(b[1], 127) || !small(b[2], 127))
                 return fail(KMCG_EINVAL, "bad bystander site");
             reach(b[0], b[1], b[2]);
+            g.push_back(otf_site_word(b[0], b[1], b[2], b[3]));
             int most = 0;
             for (int s2 = 0; s2 < S; ++s2) {
                 if (b[4 + s2] < 0)
                     return fail(KMCG_EINVAL, "negative bystander stride");
                 most = std::max(most, (int)b[4 + s2]);
+                g.push_back(b[4 + s2]);
             }
             reach_idx += most;
-            g.insert(g.end(), b, b + 4 + S);
         }
         if (reach_idx >= size)
             return fail(KMCG_EINVAL, "bystander strides reach beyond the rate table of process " +
@@ -870,15 +927,14 @@ int otf_build(kmcg_otf_model *m)
         T += B[rec + 0];
     }
     for (int q = 0; q < P; ++q) {
-        g[m->off_touch + 2 * q] = (int)g.size();
-        int cnt = 0;
-        for (size_t k = 0; k < touch[q].size(); k += 5 + 5 * touch[q][k + 4])
-            cnt += 1;
-        g[m->off_touch + 2 * q + 1] = cnt;
+        g[m->off_touch + q] = (int)g.size() | ntouch[q] << 16;
         g.insert(g.end(), touch[q].begin(), touch[q].end());
     }
     if (T > (1LL << 24))
         return fail(KMCG_EUNSUPPORTED, "otf rate tables too large");
+    if (g.size() >= 65536 || (long long)g.size() * 4 > OTF_LDS / 2)
+        return fail(KMCG_EUNSUPPORTED, "otf program too large for the engine (" + std::to_string(g.size()) +
+                                       " words)");
     m->T = (int)T;
     return KMCG_OK;
 }
@@ -887,7 +943,7 @@ OtfParams otf_params(kmcg_otf_batch *b, long long steps, int mode)
 {
     OtfParams p{};
     const kmcg_otf_model *m = b->m;
-    p.P = m->P; p.S = m->S; p.spuck = m->spuck; p.T = m->T;
+    p.P = m->P; p.S = m->S; p.spuck = m->spuck; p.T = m->T; p.W = (int)m->prog.size();
     p.prog = b->d_prog;
     p.off_owner = m->off_owner; p.off_writes = m->off_writes; p.off_byst = m->off_byst;
     p.off_tbase = m->off_tbase; p.off_touch = m->off_touch;
@@ -928,14 +984,21 @@ int otf_launch(kmcg_otf_batch *b, int32_t r0, int32_t n, long long steps, int mo
     if (rc)
         return rc;
     HIPCHK(hipMemcpyAsync(b->d_rep, ids.data(), n * sizeof(int), hipMemcpyHostToDevice, b->stream));
-    const OtfParams p = otf_params(b, steps, mode);
-    const size_t lds = (size_t)b->maxV;
+    OtfParams p = otf_params(b, steps, mode);
+    // the steps kernel stages the program (and, when it fits, the rate
+    // table) in LDS ahead of the lattice
+    const size_t prog_bytes = (size_t)((p.W + 1) & ~1) * 4;
+    const size_t tab_bytes = (size_t)b->m->T * 8;
+    p.tab_lds = prog_bytes + tab_bytes + (size_t)b->maxV <= (size_t)OTF_LDS;
     if (timed)
         HIPCHK(hipEventRecord(b->ev0, b->stream));
-    if (mode < 0)
-        hipLaunchKernelGGL(otf_steps_kernel, dim3(n), dim3(64), lds, b->stream, p);
+    if (mode < 0 && p.tab_lds)
+        hipLaunchKernelGGL(otf_steps_kernel<true>, dim3(n), dim3(64), prog_bytes + tab_bytes + b->maxV,
+                           b->stream, p);
+    else if (mode < 0)
+        hipLaunchKernelGGL(otf_steps_kernel<false>, dim3(n), dim3(64), prog_bytes + b->maxV, b->stream, p);
     else
-        hipLaunchKernelGGL(otf_setup_kernel, dim3(n), dim3(64), lds, b->stream, p);
+        hipLaunchKernelGGL(otf_setup_kernel, dim3(n), dim3(64), (size_t)b->maxV, b->stream, p);
     HIPCHK(hipGetLastError());
     if (timed) {
         HIPCHK(hipEventRecord(b->ev1, b->stream));
@@ -1069,12 +1132,14 @@ int kmcg_otf_batch_create(const kmcg_otf_model *m, int32_t n, const int32_t *siz
     b->list_off.resize(n);
     for (int r = 0; r < n; ++r) {
         const int X = sizes[3 * r], Y = sizes[3 * r + 1], Z = sizes[3 * r + 2];
+        const long long room = OTF_LDS - (long long)((m->prog.size() + 1) & ~(size_t)1) * 4;
         if (X < 1 || Y < 1 || Z < 1 || X < m->reach[0] || Y < m->reach[1] || Z < m->reach[2] ||
-            (long long)X * Y * Z * m->spuck > OTF_MAX_V) {
+            (long long)X * Y * Z * m->spuck > std::min<long long>(OTF_MAX_V, room)) {
             delete b;
             return fail(KMCG_EINVAL, "lattice size of replica " + std::to_string(r) +
                                      " outside the engine's range (each extent at least the model's "
-                                     "largest offset, at most 65536 sites)");
+                                     "largest offset; at most " + std::to_string(std::min<long long>(OTF_MAX_V, room)) +
+                                     " sites beside this model's program in 64 KiB of LDS)");
         }
         const int C = X * Y * Z;
         b->dims[4 * r] = X; b->dims[4 * r + 1] = Y; b->dims[4 * r + 2] = Z; b->dims[4 * r + 3] = C;
