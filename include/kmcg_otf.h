@@ -72,8 +72,10 @@ int kmcg_otf_sync(kmcg_otf_batch *b);
  * batch's stream). */
 int kmcg_otf_last_run_ms(kmcg_otf_batch *b, float *ms);
 
-/* base.get_kmc_time, get_kmc_step and the replica's status. */
-int kmcg_otf_get_scalars(kmcg_otf_batch *b, int32_t r, double *kmc_time, int64_t *kmc_step, int32_t *status);
+/* base.get_kmc_time, get_kmc_time_step, get_kmc_step and the replica's
+ * status. */
+int kmcg_otf_get_scalars(kmcg_otf_batch *b, int32_t r, double *kmc_time, double *kmc_time_step, int64_t *kmc_step,
+                         int32_t *status);
 /* lattice(1:V) */
 int kmcg_otf_get_lattice(kmcg_otf_batch *b, int32_t r, int8_t *out);
 /* get_nrofsites, get_procstat, get_integ_rate and get_accum_rate of every

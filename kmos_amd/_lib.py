@@ -86,7 +86,7 @@ SIGNATURES = [
     ("kmcg_otf_run", _i32, [_vp, _i32, _i32, _i64]),
     ("kmcg_otf_sync", _i32, [_vp]),
     ("kmcg_otf_last_run_ms", _i32, [_vp, _vp]),
-    ("kmcg_otf_get_scalars", _i32, [_vp, _i32, _vp, _vp, _vp]),
+    ("kmcg_otf_get_scalars", _i32, [_vp, _i32, _vp, _vp, _vp, _vp]),
     ("kmcg_otf_get_lattice", _i32, [_vp, _i32, _vp]),
     ("kmcg_otf_get_counters", _i32, [_vp, _i32, _vp, _vp, _vp, _vp]),
     ("kmcg_otf_get_avail", _i32, [_vp, _i32, _vp, _vp, _vp]),

@@ -60,7 +60,7 @@ struct OtfParams {
     long long *procstat;             // [n][P]
     double *integ, *accum, *runsum;  // [n][P]
     double *table;                   // [n][T]
-    double *kmc_time;
+    double *kmc_time, *kmc_time_step;
     long long *kmc_step;
     unsigned long long *ctr;
     const unsigned *seed, *stream;
@@ -345,6 +345,7 @@ __global__ __launch_bounds__(64) void otf_setup_kernel(OtfParams p)
         p.accum[slot] = 0.0;
         if (q == 0) {
             p.kmc_time[r] = 0.0;
+            p.kmc_time_step[r] = 0.0;
             p.kmc_step[r] = 0;
             p.ctr[r] = 0;
             p.status[r] = KMCG_REPLICA_OK;
@@ -396,6 +397,7 @@ __global__ __launch_bounds__(64) void otf_steps_kernel(OtfParams p)
     double my_acc = act ? p.accum[slot] : 0.0;
     long long pstat = act ? p.procstat[slot] : 0;
     double t = p.kmc_time[r];
+    double t_step = p.kmc_time_step[r];
     long long kstep = p.kmc_step[r];
     unsigned long long ctr = p.ctr[r];
     const unsigned seed = p.seed[r], stream = p.stream[r];
@@ -419,6 +421,7 @@ __global__ __launch_bounds__(64) void otf_steps_kernel(OtfParams p)
         ctr += 1;
         // update_clocks, update_integ_rate
         const double dt = -log(u[0]) / total;
+        t_step = dt;
         t = t + dt;
         kstep += 1;
         integ = integ + tot * dt;
@@ -600,6 +603,7 @@ __global__ __launch_bounds__(64) void otf_steps_kernel(OtfParams p)
     }
     if (lane == 0) {
         p.kmc_time[r] = t;
+        p.kmc_time_step[r] = t_step;
         p.kmc_step[r] = kstep;
         p.ctr[r] = ctr;
         p.status[r] = st;
@@ -634,7 +638,7 @@ struct kmcg_otf_batch {
     signed char *d_lat = nullptr;
     int *d_sites = nullptr, *d_back = nullptr, *d_nsites = nullptr, *d_status = nullptr;
     double *d_rm = nullptr, *d_pre = nullptr, *d_integ = nullptr, *d_accum = nullptr, *d_runsum = nullptr;
-    double *d_table = nullptr, *d_time = nullptr;
+    double *d_table = nullptr, *d_time = nullptr, *d_dt = nullptr;
     long long *d_procstat = nullptr, *d_step = nullptr;
     unsigned long long *d_ctr = nullptr;
     unsigned *d_seed = nullptr, *d_stream = nullptr;
@@ -950,7 +954,8 @@ OtfParams otf_params(kmcg_otf_batch *b, long long steps, int mode)
     p.dims = b->d_dims; p.lat_off = b->d_lat_off; p.list_off = b->d_list_off;
     p.lat = b->d_lat; p.sites = b->d_sites; p.back = b->d_back; p.rm = b->d_rm; p.pre = b->d_pre;
     p.nsites = b->d_nsites; p.procstat = b->d_procstat; p.integ = b->d_integ; p.accum = b->d_accum;
-    p.runsum = b->d_runsum; p.table = b->d_table; p.kmc_time = b->d_time; p.kmc_step = b->d_step;
+    p.runsum = b->d_runsum; p.table = b->d_table; p.kmc_time = b->d_time; p.kmc_time_step = b->d_dt;
+    p.kmc_step = b->d_step;
     p.ctr = b->d_ctr; p.seed = b->d_seed; p.stream = b->d_stream; p.status = b->d_status;
     p.rep_ids = b->d_rep; p.steps = steps; p.mode = mode;
     return p;
@@ -1099,8 +1104,8 @@ void kmcg_otf_batch_free(kmcg_otf_batch *b)
         (void)hipStreamSynchronize(b->stream);
     void *ptrs[] = {b->d_prog, b->d_dims, b->d_rep, b->d_lat_off, b->d_list_off, b->d_lat, b->d_sites,
                     b->d_back, b->d_nsites, b->d_status, b->d_rm, b->d_pre, b->d_integ, b->d_accum,
-                    b->d_runsum, b->d_table, b->d_time, b->d_procstat, b->d_step, b->d_ctr, b->d_seed,
-                    b->d_stream};
+                    b->d_runsum, b->d_table, b->d_time, b->d_dt, b->d_procstat, b->d_step, b->d_ctr,
+                    b->d_seed, b->d_stream};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -1177,6 +1182,7 @@ int kmcg_otf_batch_create(const kmcg_otf_model *m, int32_t n, const int32_t *siz
         {(void **)&b->d_runsum, n * P * sizeof(double)},
         {(void **)&b->d_table, (size_t)n * std::max(m->T, 1) * sizeof(double)},
         {(void **)&b->d_time, (size_t)n * sizeof(double)},
+        {(void **)&b->d_dt, (size_t)n * sizeof(double)},
         {(void **)&b->d_procstat, n * P * sizeof(long long)},
         {(void **)&b->d_step, (size_t)n * sizeof(long long)},
         {(void **)&b->d_ctr, (size_t)n * sizeof(unsigned long long)},
@@ -1304,12 +1310,13 @@ int kmcg_otf_last_run_ms(kmcg_otf_batch *b, float *ms)
     return rc;
 }
 
-int kmcg_otf_get_scalars(kmcg_otf_batch *b, int32_t r, double *kmc_time, int64_t *kmc_step, int32_t *status)
+int kmcg_otf_get_scalars(kmcg_otf_batch *b, int32_t r, double *kmc_time, double *kmc_time_step, int64_t *kmc_step,
+                         int32_t *status)
 {
     int rc = otf_range(b, r, 1);
     if (rc)
         return rc;
-    if ((rc = otf_copy_out(b, kmc_time, b->d_time + r, 1)))
+    if ((rc = otf_copy_out(b, kmc_time, b->d_time + r, 1)) || (rc = otf_copy_out(b, kmc_time_step, b->d_dt + r, 1)))
         return rc;
     long long s = 0;
     if ((rc = otf_copy_out(b, &s, b->d_step + r, 1)))

@@ -12,12 +12,13 @@ into the f2py module ``kmc_model``, this backend writes
                     tof_count, xml, plus ``backend``
   kmcg_model.npy    the compiled op program (int32 blob, include/kmcg.h)
   kmc_model.py      three lines binding ``base``, ``lattice``, ``proclist``
-                    to the engine (kmos_amd.kmc_model)
+                    (and for otf ``proclist_pars``) to the engine
+                    (kmos_amd.kmc_model)
 
 so that ``from kmc_model import base, lattice, proclist`` in the export
 directory gives the f2py surface KMC_Model uses, running on the GPU.
 
-Command line:  python -m kmos_amd.export MODEL.xml|MODEL.ini EXPORT_DIR [-b local_smart|lat_int]
+Command line:  python -m kmos_amd.export MODEL.xml|MODEL.ini EXPORT_DIR [-b local_smart|lat_int|otf]
 """
 from __future__ import annotations
 
@@ -32,7 +33,7 @@ from .model import Project, load_project, to_xml
 
 KMC_MODEL_STUB = '''"""f2py-compatible kmc_model module of this export, run by kmos_amd."""
 from kmos_amd.kmc_model import module_for_export as _m
-base, lattice, proclist = _m(__file__)
+base, lattice, proclist, proclist_pars = _m(__file__)
 '''
 
 
@@ -76,7 +77,7 @@ def settings_source(pt: Project, backend: str) -> str:
 
 def export_source(project, export_dir: str, code_generator: str = "local_smart") -> str:
     """Write the export directory for ``project`` (a Project or a model file
-    path) and ``code_generator`` ('local_smart' or 'lat_int')."""
+    path) and ``code_generator`` ('local_smart', 'lat_int' or 'otf')."""
     pt = load_project(project) if isinstance(project, str) else project
     cm = compile_model(pt, code_generator)
     os.makedirs(export_dir, exist_ok=True)
@@ -92,7 +93,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description="export a kmos model for the MI355X engine")
     ap.add_argument("model")
     ap.add_argument("export_dir")
-    ap.add_argument("-b", "--backend", default="local_smart", choices=["local_smart", "lat_int"])
+    ap.add_argument("-b", "--backend", default="local_smart", choices=["local_smart", "lat_int", "otf"])
     a = ap.parse_args(argv)
     print(export_source(a.model, a.export_dir, a.backend))
     return 0

@@ -263,6 +263,11 @@ class KmcModule:
             raise RuntimeError("kmc_model: system not allocated (call proclist.init first)")
         return self.batch
 
+    def view(self):
+        """The replica-0 getters KMC_Model reads (scalars, procstat,
+        integ_rates, lattice, set_config)."""
+        return self._b()
+
     def _allocate(self, size: Sequence[int], seed: int, stream: int = 0):
         if self.batch is not None:
             raise RuntimeError("kmc_model: tried to allocate the system twice, please deallocate first")
@@ -338,12 +343,23 @@ def compiled_for_export(export_dir: str) -> CompiledModel:
 _modules = {}
 
 
+def module_for(compiled: CompiledModel, device: int = 0):
+    """The engine module of a compiled model: KmcModule, or for the otf
+    backend OtfKmcModule (which adds proclist_pars)."""
+    from .compiler import BACKEND_OTF
+    if compiled.backend == BACKEND_OTF:
+        from .kmc_model_otf import OtfKmcModule
+        return OtfKmcModule(compiled, device=device)
+    return KmcModule(compiled, device=device)
+
+
 def module_for_export(stub_file: str, device: int = 0):
-    """(base, lattice, proclist) for the export directory holding stub_file;
-    one engine module per directory and process, like an f2py import."""
+    """(base, lattice, proclist, proclist_pars) for the export directory
+    holding stub_file (proclist_pars None but for otf exports); one engine
+    module per directory and process, like an f2py import."""
     import os
     d = os.path.dirname(os.path.abspath(stub_file))
     if d not in _modules:
-        _modules[d] = KmcModule(compiled_for_export(d), device=device)
+        _modules[d] = module_for(compiled_for_export(d), device=device)
     m = _modules[d]
-    return m.base, m.lattice, m.proclist
+    return m.base, m.lattice, m.proclist, getattr(m, "proclist_pars", None)

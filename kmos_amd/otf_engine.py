@@ -130,11 +130,19 @@ class OtfBatch:
 
     # ---- state ---------------------------------------------------------------
     def scalars(self, r: int):
+        """(kmc_time, kmc_step, status name)"""
+        t, dt, k, s = self.scalars_full(r)
+        return t, k, _lib.REPLICA_STATUS.get(s, str(s))
+
+    def scalars_full(self, r: int):
+        """(kmc_time, kmc_time_step, kmc_step, status code)"""
         t = ctypes.c_double(0.0)
+        dt = ctypes.c_double(0.0)
         k = ctypes.c_int64(0)
         s = ctypes.c_int32(0)
-        _lib.check(_lib.load().kmcg_otf_get_scalars(self.h, r, ctypes.byref(t), ctypes.byref(k), ctypes.byref(s)))
-        return t.value, k.value, _lib.REPLICA_STATUS.get(s.value, str(s.value))
+        _lib.check(_lib.load().kmcg_otf_get_scalars(self.h, r, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(k),
+                                                    ctypes.byref(s)))
+        return t.value, dt.value, k.value, s.value
 
     def lattice(self, r: int) -> np.ndarray:
         out = np.zeros(int(self.volumes[r]), np.int8)

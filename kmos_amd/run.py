@@ -19,7 +19,7 @@ import fnmatch
 
 import numpy as np
 
-from .kmc_model import KmcModule, compiled_for_export, load_settings
+from .kmc_model import compiled_for_export, load_settings, module_for
 from .model import evaluate_rate_expression
 
 
@@ -179,8 +179,10 @@ class KMC_Model:
             self.settings = settings
         self.settings_parameters = copy.deepcopy(self.settings.parameters)
         self.compiled = compiled
-        self._kmc = KmcModule(compiled, device=device)
+        self._kmc = module_for(compiled, device=device)
         self.base, self.lattice, self.proclist = self._kmc.base, self._kmc.lattice, self._kmc.proclist
+        # otf exports also carry the user parameters of their otf_rate expressions
+        self.proclist_pars = getattr(self._kmc, "proclist_pars", None)
         self.system_name = system_name
         self.banner = banner
         self.print_rates = print_rates
@@ -204,7 +206,15 @@ class KMC_Model:
     def set_rate_constants(self):
         """set_rate_constants (kmos/run/__init__.py:2369-2437)."""
         rates = rate_constants(self.settings, self.settings_parameters, self.compiled)
-        self._kmc.set_rates(rates)
+        if self.proclist_pars is None:
+            self._kmc.set_rates(rates)
+            return
+        # otf: every parameter's value to update_user_parameter, then
+        # recalculate_rates_matrix (kmos/run/__init__.py:2415-2432)
+        userpar = [evaluate_rate_expression(str(self.settings_parameters[name]["value"]).split(" ")[0],
+                                            self.settings_parameters)
+                   for name in self.compiled.param_names]
+        self._kmc.set_rates(rates, userpar)
 
     def reset(self):
         """kmos/run/__init__.py:243-339 without the ASE representations."""
@@ -236,7 +246,7 @@ class KMC_Model:
             raise NotImplementedError("geometry output needs ASE; call get_atoms(geometry=False)")
         atoms = _Expando()
         atoms.calc = None
-        b = self._kmc._b()
+        b = self._kmc.view()
         t, dt, ks, _, _ = b.scalars(0)
         atoms.kmc_time = t
         atoms.kmc_step = ks
@@ -308,7 +318,7 @@ class KMC_Model:
     def _get_configuration(self):
         X, Y, Z = (int(v) for v in self.lattice.system_size)
         N = int(self.lattice.spuck)
-        lat = self._kmc._b().lattice(0)
+        lat = self._kmc.view().lattice(0)
         return lat.reshape(Z, Y, X, N).transpose(2, 1, 0, 3).copy()
 
     def _set_configuration(self, config):
@@ -320,11 +330,11 @@ class KMC_Model:
         if config.shape != (X, Y, Z, N):
             raise ValueError("config shape %s does not match model shape %s" % (config.shape, [X, Y, Z, N]))
         flat = config.transpose(2, 1, 0, 3).reshape(-1)
-        self._kmc._b().set_config(flat)
+        self._kmc.view().set_config(flat)
 
     def _adjust_database(self):
         """Touch up every site with x outermost (kmos/run/__init__.py:1439-1457)."""
-        b = self._kmc._b()
+        b = self._kmc.view()
         b.set_config(b.lattice(0))
 
     def dump_config(self, filename):

@@ -64,3 +64,19 @@ def test_multi_layer_model_exports(tmp_path, backend):
     assert [n for n in range(cm.spuck) if cm.site_default[n] != cm.null_species] == \
         [n for n in range(cm.spuck) if layers[n] == pt.default_layer] == \
         [n for n in range(cm.spuck) if cm.site_init[n]]
+
+
+def test_otf_model_exports(tmp_path):
+    """An otf export: the settings name the backend, the stored program is
+    the compiled one and survives the XML round trip (bystanders and
+    otf_rate included)."""
+    from kmos_amd.kmc_model import compiled_for_export, load_settings
+    d = export_source(os.path.join(GOLDEN, "intzgb_otf.xml"), str(tmp_path / "otf"), "otf")
+    s = load_settings(d)
+    assert s.backend == "otf"
+    cm = compiled_for_export(d)
+    assert cm.backend_name == "otf" and cm.nr_of_proc == 10
+    assert cm.param_names == ["J_CO_CO", "J_O_CO", "J_O_O", "kDes", "yCO"]
+    assert any(r.table_size > 1 for r in cm.otf_rates)
+    stub = open(os.path.join(d, "kmc_model.py")).read()
+    assert "proclist_pars" in stub

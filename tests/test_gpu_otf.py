@@ -159,3 +159,37 @@ def test_otf_rejects_bad_input(otf_engine):
         b.set_config([np.full(36, 3)])              # species 3 of 0..2
     b.run(100)
     assert b.scalars(0)[1] == 100
+
+
+def test_kmc_model_on_otf_export(otf_engine, otf_oracle, tmp_path):
+    """export -> kmc_model -> KMC_Model for the intZGB otf model: rate
+    constants and user parameters from the settings (update_user_parameter,
+    recalculate_rates_matrix), do_steps, get_atoms and sampling, a new
+    parameter value mid-run; the replica equals the oracle set up alike."""
+    from kmos_amd.export import export_source
+    from kmos_amd.run import KMC_Model
+    d = export_source(os.path.join(GOLDEN, "intzgb_otf.xml"), str(tmp_path / "otf"), "otf")
+    model = KMC_Model(export_dir=d, size=[12, 10], random_seed=7)
+    assert model.get_backend() == "otf"
+    cm = model.compiled
+    rates = model._kmc._rates.copy()
+    userpar = model._kmc._userpar[:len(cm.param_names)].copy()
+    assert userpar.tolist() == [1.0, 1.0, 1.0, 1e-10, 0.5]      # the model file's values
+    o = otf_oracle.OtfOracle(cm.to_blob(), (12, 10))
+    o.init(7, 0)
+    o.set_params(rates, userpar)
+    model.do_steps(1500)
+    o.run(1500)
+    b = model._kmc.batch
+    _compare(b, 0, o, "KMC_Model otf")
+    atoms = model.get_atoms(geometry=False)
+    assert atoms.kmc_step == 1500 and abs(atoms.kmc_time - o.kmc_time) <= 1e-12 * o.kmc_time
+    np.testing.assert_array_equal(atoms.procstat, o.procstat())
+    # a parameter change goes through update_user_parameter + recalculate
+    model.parameters.J_O_CO = 0.8
+    userpar[cm.param_names.index("J_O_CO")] = 0.8
+    o.set_params(rates, userpar)
+    out = model.get_std_sampled_data(4, 800, output="dict")
+    o.run(800)
+    _compare(b, 0, o, "KMC_Model otf after a parameter change")
+    assert out["kmc_steps"] == 800 or out.get("simulated_time", 0) > 0
