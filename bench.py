@@ -65,10 +65,10 @@ def host_cores():
     return avail, min(avail, CORE_CAP)
 
 
-def profile_figure(workload, kind, key):
+def profile_figure(workload, kind, key, kernel="kmc_steps_kernel"):
     """A per-kMC-step figure ('traffic' or 'lds' summary) from the rocprofv3
-    counter summary committed for this workload, or None."""
-    path = os.path.join(REPO, "profiles", "kmc_steps_kernel_%s_%s.md" % (workload, kind))
+    counter summary committed for this workload and kernel, or None."""
+    path = os.path.join(REPO, "profiles", "%s_%s_%s.md" % (kernel, workload, kind))
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -77,13 +77,13 @@ def profile_figure(workload, kind, key):
     return None
 
 
-def roofline(workload, bytes_per_step, steps_per_launch, avg_launch_s):
+def roofline(workload, bytes_per_step, steps_per_launch, avg_launch_s, kernel="kmc_steps_kernel"):
     """HBM roofline of the step kernel: algorithmic bytes per launch over the
     mean launch time; the measured HBM traffic and the LDS bandwidth from
     the profiles' bytes per kMC step, scaled to this run's launches."""
     achieved = bytes_per_step * steps_per_launch / avg_launch_s / 1e9
-    t_step = profile_figure(workload, "pmc", "traffic_bytes_per_kmc_step")
-    l_step = profile_figure(workload, "lds", "lds_bytes_per_kmc_step")
+    t_step = profile_figure(workload, "pmc", "traffic_bytes_per_kmc_step", kernel)
+    l_step = profile_figure(workload, "lds", "lds_bytes_per_kmc_step", kernel)
     lds = None
     if l_step is not None:
         a = l_step * steps_per_launch / avg_launch_s / 1e9
@@ -94,7 +94,7 @@ def roofline(workload, bytes_per_step, steps_per_launch, avg_launch_s):
             "traffic": None if t_step is None else t_step * steps_per_launch,
             "traffic_bytes_per_kmc_step": t_step,
             "algorithmic_bytes_per_kmc_step": bytes_per_step,
-            "kernel": "kmc_steps_kernel", "avg_launch_ms": avg_launch_s * 1e3,
+            "kernel": kernel, "avg_launch_ms": avg_launch_s * 1e3,
             "kmc_steps_per_launch": steps_per_launch, "lds": lds}
 
 
@@ -586,8 +586,7 @@ def otf_leg(K, warmup, cpu, wl=None):
     bps, work = otf_algorithmic_bytes(wl, blob)
     avg_s = float(np.mean(kernel_ms)) / 1e3
     steps_per_launch = chunk * wl.n
-    rl = roofline(wl.name, bps, steps_per_launch, avg_s)
-    rl["kernel"] = "otf_steps_kernel"
+    rl = roofline(wl.name, bps, steps_per_launch, avg_s, kernel="otf_steps_kernel")
     out = {"workload": wl.name, "replicas": int(wl.n), "steps_per_replica": int((K + warmup) * chunk),
            "gpu_steps_per_s": executed / wall, "kmc_steps_per_launch": steps_per_launch,
            "avg_launch_ms": avg_s * 1e3, "replica_status_counts": {k: status.count(k) for k in set(status)},
