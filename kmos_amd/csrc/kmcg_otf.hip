@@ -110,7 +110,9 @@ __device__ __forceinline__ double otf_gr(const int *prog, const OtfParams &p, co
 }
 
 // del_proc (base_otf.f90:221-282) on the owner lane's rows: the last entry
-// moves into the freed position.  Returns the freed position, -1 when the
+// moves into the freed position.  Returns the first list position whose
+// rate changed (the freed one, or the new end when the moved entry carries
+// the same rate, so that the running sums below it stand), -1 when the
 // inverse index points outside the list.
 __device__ __forceinline__ int otf_del(int *srow, int *brow, double *rrow, int &n, double &run, int c)
 {
@@ -118,19 +120,23 @@ __device__ __forceinline__ int otf_del(int *srow, int *brow, double *rrow, int &
     const int last = n - 1;
     if (pos < 0 || pos > last)
         return -1;
-    run = run - rrow[pos];
+    const double gone = rrow[pos];
+    run = run - gone;
+    int changed = last;
     if (pos < last) {
         const int mv = srow[last];
         const double mr = rrow[last];
         srow[pos] = mv;
         rrow[pos] = mr;
         brow[mv - 1] = pos + 1;
+        if (mr != gone)
+            changed = pos;
     }
     srow[last] = 0;
     rrow[last] = 0.0;
     brow[c] = 0;
     n = last;
-    return pos;
+    return changed;
 }
 
 // the running sum of a row from position `from` on (update_accum_rate's
@@ -167,8 +173,27 @@ __device__ __forceinline__ double row_shr1z(double v)
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
+// The pending list with the most entries to re-add (uniform; ties to the
+// lowest lane), taken off `pend`; -1 when none is left.
+__device__ __forceinline__ int otf_take_longest(unsigned long long &pend, int len)
+{
+    int best = -1, most = -1;
+    for (unsigned long long m = pend; m; m &= m - 1) {
+        const int l = __ffsll((long long)m) - 1;
+        const int v = readlane_i(len, l);
+        if (v > most) {
+            most = v;
+            best = l;
+        }
+    }
+    if (best >= 0)
+        pend &= ~(1ull << best);
+    return best;
+}
+
 // The running sums of every list a step changed, re-added from its first
-// changed position to its end, four lists at a time: each row of 16 lanes
+// changed position to its end, four lists at a time (the longest four
+// first, so that lists of like length share a pass): each row of 16 lanes
 // walks one list in blocks of 16 entries (one coalesced read), and within
 // a block the sum is carried left to right through the row by 15 shifted
 // adds, so lane i ends with ((carry + r0) + r1) + ... + ri, the rounding of
@@ -178,14 +203,12 @@ __device__ __forceinline__ void otf_resum_rows(const double *rm, double *pre, lo
 {
     unsigned long long dm = __ballot(want && dirty < n);
     const int row = lane >> 4, i = lane & 15;
+    const int mylen = (want && dirty < n) ? n - dirty : 0;
     while (dm) {
         int qs[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            qs[k] = dm ? __ffsll((long long)dm) - 1 : -1;
-            if (dm)
-                dm &= dm - 1;
-        }
+        for (int k = 0; k < 4; ++k)
+            qs[k] = otf_take_longest(dm, mylen);
         int d4[4], n4[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -539,7 +562,8 @@ __global__ __launch_bounds__(64) void otf_steps_kernel(OtfParams p)
                         new8[j] = otf_gr(prog, p, g, slat, tab, q, cx8[j], cy8[j], cz8[j]);
                         run = run + new8[j] - old;
                         rrow[at] = new8[j];
-                        dirty = min(dirty, at);
+                        if (new8[j] != old)
+                            dirty = min(dirty, at);
                     }
                 }
             }
@@ -558,9 +582,11 @@ __global__ __launch_bounds__(64) void otf_steps_kernel(OtfParams p)
                 } else if (brow[c]) {
                     const int at = brow[c] - 1;
                     const double rate = otf_gr(prog, p, g, slat, tab, q, cx, cy, cz);
-                    run = run + rate - rrow[at];
+                    const double old = rrow[at];
+                    run = run + rate - old;
                     rrow[at] = rate;
-                    dirty = min(dirty, at);
+                    if (rate != old)
+                        dirty = min(dirty, at);
                 }
                 w += 1;
             } else {

@@ -49,8 +49,9 @@ typedef struct {
     uint64_t ctr;
     int status;
     /* work counters (the bench's algorithmic bytes): list operations, list
-     * entries whose running sum a step must re-add (from the first changed
-     * position of each list on) and interval-search probes of a binary search */
+     * entries whose running sum a step must re-add (from the first position
+     * whose rate changed, in each list, on) and interval-search probes of a
+     * binary search */
     int32_t *dmin;
     uint64_t n_ops, n_readd, n_probe;
 } otf_t;
@@ -260,10 +261,10 @@ static double gr(const otf_t *o, int q, const int c[3])
 #define A2(o, p, s) ((o)->a2[(size_t)(p) * ((o)->V + 1) + (s)])
 #define RM(o, p, i) ((o)->rm[(size_t)(p) * ((o)->V + 2) + (i)])
 
-static void touch(otf_t *o, int p, int pos)
+static void touch(otf_t *o, int p, int pos, int changed)
 {
     o->n_ops += 1;
-    if (pos - 1 < o->dmin[p])
+    if (changed && pos - 1 < o->dmin[p])
         o->dmin[p] = pos - 1;
 }
 
@@ -275,7 +276,7 @@ static void del_proc(otf_t *o, int p, int site)
         o->status = OTF_DEL_ABSENT;
         return;
     }
-    touch(o, p, mem);
+    touch(o, p, mem, mem < n && RM(o, p, mem) != RM(o, p, n));
     if (mem < n) {
         A1(o, p, mem) = A1(o, p, n);
         A1(o, p, n) = 0;
@@ -295,7 +296,7 @@ static void del_proc(otf_t *o, int p, int site)
 static void add_proc(otf_t *o, int p, int site, double rate)
 {
     int n = ++o->nsites[p];
-    touch(o, p, n);
+    touch(o, p, n, 1);
     A1(o, p, n) = site;
     A2(o, p, site) = n;
     RM(o, p, o->V + 1) = RM(o, p, o->V + 1) + rate;
@@ -305,7 +306,7 @@ static void add_proc(otf_t *o, int p, int site, double rate)
 static void update_rates_matrix(otf_t *o, int p, int site, double rate)
 {
     int mem = A2(o, p, site);
-    touch(o, p, mem);
+    touch(o, p, mem, RM(o, p, mem) != rate);
     RM(o, p, o->V + 1) = RM(o, p, o->V + 1) + rate - RM(o, p, mem);
     RM(o, p, mem) = rate;
 }
