@@ -121,7 +121,8 @@ struct KParams {
     int P, S, spuck, dim, maxR, backend;
     int moff;                      // largest offset component of the program
     int mrows;                     // mailbox rows per wave
-    int merged;                    // 1: the segments of a step share one apply
+    int merged;                    // bit 0: the segments of a step share one apply;
+                                   // bit 1: a lat_int group's segments post in one pass
     int off_seg, off_proc, off_touch, off_nli, off_nlitab;
     double *srec;                  // in-loop sample records (or null)
     int sample_every, srec_cap, srec_width;

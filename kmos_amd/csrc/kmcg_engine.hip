@@ -101,6 +101,42 @@ __device__ __forceinline__ T slot_get(const T (&a)[PPL], int k)
     return v;
 }
 
+// nli_<group>(cell) of a tabulated group on the lattice before a step's
+// writes (`before`, the written sites sA and sB read as their old species
+// oA and oB) and after them (`after`), from one set of lattice reads.
+template <int PPL, int BITS, bool IN>
+__device__ __forceinline__ void nli_eval_pair(const Wave<PPL, BITS> &W, int group, int cx, int cy, int cz, int sA,
+                                              int oA, int sB, int oB, int &before, int &after)
+{
+    const int *prog = W.prog;
+    const int lc = W.d.lin(cx, cy, cz);
+    const int th = prog[W.off_nlitab + group];
+    {
+        const int hdr = prog[th], K = hdr & 0xFF, S = (hdr >> 8) & 0xFF;
+        int cw[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            cw[k] = prog[th + 1 + k];
+        int sn[8], so[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int c = (k < K) ? cw[k] : (1 << 24);
+            const int st = site_from<IN>(W.d, lc, cx, cy, cz, unpack_x(c), unpack_y(c), unpack_z(c), (c >> 24) & 0xFF);
+            sn[k] = W.lat.get(st);
+            so[k] = st == sA ? oA : (st == sB ? oB : sn[k]);
+        }
+        int i0 = 0, i1 = 0;
+#pragma unroll
+        for (int k = 7; k >= 0; --k) {
+            i0 = (k < K) ? i0 * S + so[k] : i0;
+            i1 = (k < K) ? i1 * S + sn[k] : i1;
+        }
+        const unsigned char *tab = (const unsigned char *)(prog + th + 1 + K);
+        before = tab[i0];
+        after = tab[i1];
+    }
+}
+
 // nli_<group>(cell): walk the case tree (kmos/io/__init__.py:1985-2057).
 template <int PPL, int BITS, bool IN>
 __device__ int nli_eval(const Wave<PPL, BITS> &W, int group, int cx, int cy, int cz)
@@ -673,6 +709,76 @@ __device__ __forceinline__ int post_pair(Wave<PPL, BITS> &W, const SegPre &q1, c
     return ST_OK;
 }
 
+// The two segments of a lat_int group posted in one pass
+// (run_proc_<group>, kmos/io/__init__.py:1497-1624): del_proc of every
+// modified cell's process as nli_<group> finds it before the writes, the
+// put/take writes, then add_proc as nli_<group> finds it after them.  Both
+// segments hold the same ops, so each lane evaluates its op's nli on both
+// lattices from one set of reads: the writes are done first and the
+// deletion's reads of a written site return its old species.  The queues
+// get the deletions in rounds 0..R-1 and the additions in rounds R..2R-1,
+// the order of the two passes.  Returns -1 (and changes nothing) when a
+// write would fail its species check: the caller then runs the segments
+// one by one, which reproduces the reference's behaviour on a mismatch.
+template <int PPL, int BITS>
+__device__ __forceinline__ int post_latint_pair(Wave<PPL, BITS> &W, const SegPre &q1, int nw, int woff, int bx,
+                                                int by, int bz)
+{
+    const int *prog = W.prog;
+    const int nops = q1.nops, R = q1.nrounds;
+    const int lb = W.d.lin(bx, by, bz);
+    // the writes, one lane each (at most two)
+    int s = -1, cur = 0, nsp = 0, t = 1;
+    bool bad = false;
+    if (W.lane < nw) {
+        const int w0 = prog[woff + WR_W * W.lane], w1 = prog[woff + WR_W * W.lane + 1];
+        t = (w0 >> 24) & 0xFF;
+        s = site_from<true>(W.d, lb, bx, by, bz, unpack_x(w0), unpack_y(w0), unpack_z(w0), t);
+        cur = W.lat.get(s);
+        nsp = w1 >> 16;
+        bad = cur < 0 || !((w1 >> cur) & 1);
+    }
+    const int sA = readlane_i(s, 0), oA = readlane_i(cur, 0);
+    const int sB = readlane_i(s, 1), oB = readlane_i(cur, 1);
+    if (__ballot(W.lane < nw && bad))
+        return -1;
+    if (W.lane < nw) {
+        W.lat.xset(s, cur, nsp);
+        if (W.track) {
+            if (cur < W.S)
+                atomicAdd(&W.occ[cur * W.d.spuck + t - 1], -1);
+            if (nsp < W.S)
+                atomicAdd(&W.occ[nsp * W.d.spuck + t - 1], 1);
+        }
+    }
+    wave_sync();
+    const int2 op = q1.op;
+    int pd = 0, pa = 0, so = 0, rnd = 0;
+    if (W.lane < nops) {
+        rnd = (op.x >> 3) & 0xFF;
+        const int arg = (int)((unsigned)op.x >> 23);
+        const int dx = sext7(op.y & 0x7F), dy = sext7((op.y >> 7) & 0x7F), dz = sext7((op.y >> 14) & 0x7F);
+        const int cx = wrap1(bx + dx, W.d.X);
+        const int cy = W.d.dim >= 2 ? wrap1(by + dy, W.d.Y) : 0;
+        const int cz = W.d.dim >= 3 ? wrap1(bz + dz, W.d.Z) : 0;
+        so = W.d.lin(cx, cy, cz);
+        nli_eval_pair<PPL, BITS, true>(W, arg, cx, cy, cz, sA, oA, sB, oB, pd, pa);
+    }
+    int sd = 0, sa = 0;
+    for (int it = 0; it < R; ++it)
+        if (pd > 0 && rnd == it)
+            sd = atomicAdd(&W.cnt[pd - 1], 1);
+    for (int it = 0; it < R; ++it)
+        if (pa > 0 && rnd == it)
+            sa = atomicAdd(&W.cnt[pa - 1], 1);
+    if (pd > 0)
+        W.mbox[sd * W.mstride + pd - 1] = ((so + 1) << 2) | MB_DEL_STRICT;
+    if (pa > 0)
+        W.mbox[sa * W.mstride + pa - 1] = ((so + 1) << 2) | MB_ADD;
+    wave_sync();
+    return ST_OK;
+}
+
 // base cells away from the faces take the path without wrapping
 template <int PPL, int BITS>
 __device__ __forceinline__ int post_segment(Wave<PPL, BITS> &W, const SegPre &q, int bx, int by, int bz)
@@ -710,13 +816,16 @@ __device__ __forceinline__ void proc_prefetch(const int *prog, int off_proc, int
     seg_prefetch(prog, off_seg, q.nit > 0 ? q.it[0] : 0, lane, q.seg);
 }
 
-// run_proc_nr(proc, site); procstat is counted by the caller.  With
-// `merged`, all segments of the process are queued and applied once, else
-// each segment is applied before the next is posted.  The first segment's
+// run_proc_nr(proc, site); procstat is counted by the caller.  With bit 0
+// of `mode`, all segments of the process are queued and applied once, else
+// each segment is applied before the next is posted.  LIP compiles the
+// one-pass posting of a lat_int group's two segments, taken when bit 1 of
+// `mode` is set (the model's nli groups are all tabulated).  The first segment's
 // static words come read ahead in `pre`.
-template <int PPL, int BITS, bool RB2 = false, class Mid = NoMid>
-__device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, bool merged, Mid &mid)
+template <int PPL, int BITS, bool RB2 = false, class Mid = NoMid, bool LIP = false>
+__device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, int mode, Mid &mid)
 {
+    const bool merged = mode & 1;
     const int s0 = site - 1;
     const int cell = divm(s0, W.ms);
     const int yz = divm(cell, W.mx);
@@ -727,6 +836,8 @@ __device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, bool m
     const int *it = pre.it;
     SegPre cur = pre.seg;
     int e = ST_OK;
+    // both segments posted in one pass (merged only): applied below
+    bool posted = false;
     if (merged && W.local_smart && nit == 2 && cur.nw == 1 && !cur.as_add) {
         SegPre q2;
         seg_prefetch(W.prog, W.off_seg, it[IT_W], W.lane, q2);
@@ -739,16 +850,24 @@ __device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, bool m
             const int by2 = W.d.dim >= 2 ? wrap1(y + unpack_y(o2), W.d.Y) : 0;
             const int bz2 = W.d.dim >= 3 ? wrap1(z + unpack_z(o2), W.d.Z) : 0;
             // near the faces the segments run one by one
-            if (W.d.interior(bx1, by1, bz1) && W.d.interior(bx2, by2, bz2) &&
-                post_pair<PPL, BITS, true>(W, cur, q2, bx1, by1, bz1, bx2, by2, bz2) >= 0) {
-                PH_MARK(W, 2);
-                const int a = apply_queues<PPL, BITS, RB2>(W, mid);
-                PH_MARK(W, 3);
-                return a;
-            }
+            posted = W.d.interior(bx1, by1, bz1) && W.d.interior(bx2, by2, bz2) &&
+                     post_pair<PPL, BITS, true>(W, cur, q2, bx1, by1, bz1, bx2, by2, bz2) >= 0;
         }
     }
-    for (int i = 0; i < nit; ++i, it += IT_W) {
+    if (LIP && (mode & 2) && nit == 2 && cur.nw == 0 && !cur.as_add && cur.nops <= 64 &&
+        it[1] == it[IT_W + 1]) {
+        // a lat_int group: deletions, writes, additions over the same ops
+        // (bit 1 is set only with bit 0)
+        const int *sg = W.prog + W.off_seg + SEG_W * it[IT_W];
+        const int nw2 = sg[0], woff2 = sg[1], h2 = sg[2];
+        const int o = it[1];
+        const int bx = wrap1(x + unpack_x(o), W.d.X);
+        const int by = W.d.dim >= 2 ? wrap1(y + unpack_y(o), W.d.Y) : 0;
+        const int bz = W.d.dim >= 3 ? wrap1(z + unpack_z(o), W.d.Z) : 0;
+        posted = nw2 <= 2 && ((h2 >> 24) & 1) && (h2 & 0xFFFF) == cur.nops && sg[3] == cur.ooff &&
+                 W.d.interior(bx, by, bz) && post_latint_pair<PPL, BITS>(W, cur, nw2, woff2, bx, by, bz) >= 0;
+    }
+    for (int i = 0; i < (posted ? 0 : nit); ++i, it += IT_W) {
         const int o = it[1];
         const int bx = wrap1(x + unpack_x(o), W.d.X);
         const int by = W.d.dim >= 2 ? wrap1(y + unpack_y(o), W.d.Y) : 0;
@@ -1216,7 +1335,7 @@ __device__ __forceinline__ void acf_after(const KParams &kp, int r)
 // and search run during the replay's HBM reads.  It shortens a trajectory's
 // chain of dependent steps and adds instructions, so it pays where few
 // replica-waves share a SIMD (latency-bound) and not where many do.
-template <int PPL, int BITS, bool LDS_LAT, bool MSD, bool OV = false>
+template <int PPL, int BITS, bool LDS_LAT, bool MSD, bool OV = false, bool LIP = false>
 __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4))) void kmc_steps_kernel(KParams kp)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1368,7 +1487,7 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
             s_pick = nx.pick;
         } else {
             NoMid none;
-            st = run_proc<PPL, BITS, !MSD>(W, pre, site, kp.merged, none);
+            st = run_proc<PPL, BITS, !MSD, NoMid, LIP>(W, pre, site, kp.merged, none);
         }
         if (MSD && kp.acf_pa && lane == 0)
             acf_after(kp, r);
@@ -1806,7 +1925,7 @@ struct kmcg_model {
     int moff = 0;        // largest offset component any segment reaches
     int rstep = 1;       // mailbox rows of one executed process (all its segments)
     int mrows = 1;       // mailbox rows per wave
-    int merged = 0;      // 1: one apply per step
+    int merged = 0;      // bit 0: one apply per step; bit 1: lat_int groups in one pass
     int off_seg = 0, off_proc = 0, off_touch = 0, off_nli = 0, off_init = 0, off_nlitab = 0;
     // a species id a lattice site may hold: 0..S-1, or the null species S of
     // a multi-layer model (sites of layers that are not populated)
@@ -2403,6 +2522,15 @@ int build_prog(kmcg_model *m)
             }
             pg.insert(pg.end(), 8, 0);            // coordinate reads go 8 words ahead
         }
+        // a lat_int model whose nli groups are all tabulated posts a group's
+        // two segments in one pass (run_proc's mode bit 1)
+        if (m->backend == 1 && m->merged && !nli_nodes.empty()) {
+            bool all = true;
+            for (size_t g = 0; g < nli_nodes.size(); ++g)
+                all &= pg[m->off_nlitab + g] >= 0;
+            if (all)
+                m->merged |= 2;
+        }
         // per process: anchor site type, its touchup addition op (local_smart)
         // or its group (lat_int), for the parallel initialisation.  A
         // local_smart process anchored on a site type that initialize_state
@@ -2540,6 +2668,12 @@ hipError_t launch_t(bool steps, const Shape &s, hipStream_t st, const KParams &k
             hipLaunchKernelGGL((kmc_steps_kernel<1, BITS, true, false, true>), grid, block, s.lds, st, kp);
         else
             hipLaunchKernelGGL((kmc_steps_kernel<1, BITS, false, false, true>), grid, block, s.lds, st, kp);
+    } else if (steps && PPL == 1 && (kp.merged & 2)) {
+        // lat_int with tabulated nli groups: a group's segments in one pass
+        if (s.lds_lat)
+            hipLaunchKernelGGL((kmc_steps_kernel<1, BITS, true, false, false, true>), grid, block, s.lds, st, kp);
+        else
+            hipLaunchKernelGGL((kmc_steps_kernel<1, BITS, false, false, false, true>), grid, block, s.lds, st, kp);
     } else if (steps) {
         if (s.lds_lat)
             hipLaunchKernelGGL((kmc_steps_kernel<PPL, BITS, true, false>), grid, block, s.lds, st, kp);
