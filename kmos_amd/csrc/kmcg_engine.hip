@@ -765,12 +765,36 @@ __device__ __forceinline__ int post_latint_pair(Wave<PPL, BITS> &W, const SegPre
         nli_eval_pair<PPL, BITS, true>(W, arg, cx, cy, cz, sA, oA, sB, oB, pd, pa);
     }
     int sd = 0, sa = 0;
-    for (int it = 0; it < R; ++it)
-        if (pd > 0 && rnd == it)
-            sd = atomicAdd(&W.cnt[pd - 1], 1);
-    for (int it = 0; it < R; ++it)
-        if (pa > 0 && rnd == it)
-            sa = atomicAdd(&W.cnt[pa - 1], 1);
+    if (R <= 8) {
+        // one result register per round: a pass's atomics issue back to
+        // back and are waited for once
+        int sl[8];
+#pragma unroll
+        for (int it = 0; it < 8; ++it) {
+            sl[it] = 0;
+            if (it < R && pd > 0 && rnd == it)
+                sl[it] = atomicAdd(&W.cnt[pd - 1], 1);
+        }
+#pragma unroll
+        for (int it = 0; it < 8; ++it)
+            sd = (rnd == it) ? sl[it] : sd;
+#pragma unroll
+        for (int it = 0; it < 8; ++it) {
+            sl[it] = 0;
+            if (it < R && pa > 0 && rnd == it)
+                sl[it] = atomicAdd(&W.cnt[pa - 1], 1);
+        }
+#pragma unroll
+        for (int it = 0; it < 8; ++it)
+            sa = (rnd == it) ? sl[it] : sa;
+    } else {
+        for (int it = 0; it < R; ++it)
+            if (pd > 0 && rnd == it)
+                sd = atomicAdd(&W.cnt[pd - 1], 1);
+        for (int it = 0; it < R; ++it)
+            if (pa > 0 && rnd == it)
+                sa = atomicAdd(&W.cnt[pa - 1], 1);
+    }
     if (pd > 0)
         W.mbox[sd * W.mstride + pd - 1] = ((so + 1) << 2) | MB_DEL_STRICT;
     if (pa > 0)
