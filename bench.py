@@ -481,6 +481,9 @@ def phase_profile(engine, wl, K, warmup):
         return {name: {"ticks_per_step": float(v), "frac": float(v / tot.sum())} for name, v in zip(PHASES, tot)}
 
     out["all"] = table(np.ones(wl.n, bool))
+    # replay blocks: batches of list reads (one HBM round trip each)
+    out["replay_blocks_per_step"] = float(ph[:, 6].sum() / max(steps.sum(), 1.0))
+    out["replay_blocks_per_replay"] = float(ph[:, 6].sum() / max(ph[:, 7].sum(), 1.0))
     vol = wl.sizes.prod(axis=1)
     out["by_size"] = {"%dx%d" % tuple(wl.sizes[np.nonzero(vol == v)[0][0]][:2]): table(vol == v)
                       for v in np.unique(vol)}

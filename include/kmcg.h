@@ -140,8 +140,9 @@ int kmcg_last_run_ms(kmcg_batch *b, float *ms);
 /* Phase profile of the step loop, for replicas r0..r0+n-1: out[n][8] holds
  * the shader-clock ticks each replica's wave spent selecting (accumulate,
  * clock, search), reading its site, posting the process's ops, replaying
- * the lists and on the rest of its steps, then the steps counted; the
- * counters are cleared after the read.  Only a library built with
+ * the lists and on the rest of its steps, then the steps counted, the
+ * replay blocks (batches of list reads) and the replays; the counters are
+ * cleared after the read.  Only a library built with
  * -DKMCG_PHASES=1 (libkmcgpu_phases.so) counts; the plain library returns
  * KMCG_EUNSUPPORTED.  No counterpart in the reference (a profiling aid). */
 int kmcg_phase_profile(kmcg_batch *b, int32_t r0, int32_t n, uint64_t *out);

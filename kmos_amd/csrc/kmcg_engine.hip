@@ -35,7 +35,8 @@ namespace {
 
 // Phase profile of the step loop (a build with -DKMCG_PHASES=1 only): each
 // wave adds the shader-clock ticks between marks to one of five counters
-// (0 select, 1 site read, 2 post, 3 replay, 4 the rest of the step).  The
+// (0 select, 1 site read, 2 post, 3 replay, 4 the rest of the step) and
+// counts the replay's blocks (batches of HBM reads) and replays.  The
 // clock reads wait for outstanding scalar and LDS work, so the instrumented
 // loop runs a little slower than the plain one.
 #ifndef KMCG_PHASES
@@ -84,6 +85,7 @@ struct Wave {
     long long pstat[PPL];
 #if KMCG_PHASES
     unsigned long long tph[5];
+    unsigned long long nblk, napply;   // replay blocks, replays
     unsigned long long tmark;
 #endif
 
@@ -99,42 +101,6 @@ __device__ __forceinline__ T slot_get(const T (&a)[PPL], int k)
     for (int i = 1; i < PPL; ++i)
         if (k == i) v = a[i];
     return v;
-}
-
-// nli_<group>(cell) of a tabulated group on the lattice before a step's
-// writes (`before`, the written sites sA and sB read as their old species
-// oA and oB) and after them (`after`), from one set of lattice reads.
-template <int PPL, int BITS, bool IN>
-__device__ __forceinline__ void nli_eval_pair(const Wave<PPL, BITS> &W, int group, int cx, int cy, int cz, int sA,
-                                              int oA, int sB, int oB, int &before, int &after)
-{
-    const int *prog = W.prog;
-    const int lc = W.d.lin(cx, cy, cz);
-    const int th = prog[W.off_nlitab + group];
-    {
-        const int hdr = prog[th], K = hdr & 0xFF, S = (hdr >> 8) & 0xFF;
-        int cw[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-            cw[k] = prog[th + 1 + k];
-        int sn[8], so[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int c = (k < K) ? cw[k] : (1 << 24);
-            const int st = site_from<IN>(W.d, lc, cx, cy, cz, unpack_x(c), unpack_y(c), unpack_z(c), (c >> 24) & 0xFF);
-            sn[k] = W.lat.get(st);
-            so[k] = st == sA ? oA : (st == sB ? oB : sn[k]);
-        }
-        int i0 = 0, i1 = 0;
-#pragma unroll
-        for (int k = 7; k >= 0; --k) {
-            i0 = (k < K) ? i0 * S + so[k] : i0;
-            i1 = (k < K) ? i1 * S + sn[k] : i1;
-        }
-        const unsigned char *tab = (const unsigned char *)(prog + th + 1 + K);
-        before = tab[i0];
-        after = tab[i1];
-    }
 }
 
 // nli_<group>(cell): walk the case tree (kmos/io/__init__.py:1985-2057).
@@ -347,8 +313,14 @@ __device__ __forceinline__ int apply_queues(Wave<PPL, BITS> &W, Mid &mid)
             v[i] = q[min(i, W.mrows - 1) * W.mstride];
         n = own ? n : 0;
         W.cnt[p] = 0;
+#if KMCG_PHASES
+        W.napply += 1;
+#endif
         for (int i0 = 0; __ballot(i0 < n); i0 += RBLK) {
             const int m = min(n - i0, RBLK);
+#if KMCG_PHASES
+            W.nblk += 1;
+#endif
             if (i0 > 0) {
 #pragma unroll
                 for (int i = 0; i < RBLK; ++i)
@@ -709,40 +681,93 @@ __device__ __forceinline__ int post_pair(Wave<PPL, BITS> &W, const SegPre &q1, c
     return ST_OK;
 }
 
+// The static words of a lat_int group's one-pass posting, read while the
+// step's site is on its way from HBM: whether the process qualifies (two
+// segments at one offset, the first the deletions without writes, the
+// second its alias with at most two writes), the item offset, each write
+// lane's write words and each op lane's nli table header and coordinates.
+struct LiPre {
+    bool ok = false;
+    int off, nw;
+    int w0, w1;          // write lanes (lane < nw)
+    int th, hdr;         // op lanes: the group's table and its header
+    int cw[8];           // op lanes: the table's coordinate words
+};
+
+__device__ __forceinline__ void li_prefetch(const int *prog, int off_seg, int off_nlitab, int lane, int nit,
+                                            const int *it, const SegPre &cur, LiPre &q)
+{
+    // every read stays inside the program whatever the process: a process
+    // that does not qualify reads the program's first words instead
+    const bool ok0 = nit == 2 && cur.nw == 0 && !cur.as_add && cur.nops <= 64;
+    const int *ip = ok0 ? it : prog;
+    const int i1 = ip[1], s2 = ip[IT_W], i2 = ip[IT_W + 1];
+    const int *sg = prog + off_seg + SEG_W * (ok0 ? s2 : 0);
+    const int nw2 = sg[0], woff2 = sg[1], h2 = sg[2], ooff2 = sg[3];
+    q.ok = ok0 && i1 == i2 && nw2 <= 2 && ((h2 >> 24) & 1) && (h2 & 0xFFFF) == cur.nops && ooff2 == cur.ooff;
+    q.off = i1;
+    q.nw = q.ok ? nw2 : 0;
+    const int wl = lane < q.nw ? lane : 0;
+    q.w0 = prog[woff2 + WR_W * wl];
+    q.w1 = prog[woff2 + WR_W * wl + 1];
+    const int arg = lane < cur.nops ? (int)((unsigned)cur.op.x >> 23) : 0;
+    q.th = prog[off_nlitab + arg];
+    const int th = q.th < 0 ? 0 : q.th;
+    q.hdr = prog[th];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        q.cw[k] = prog[th + 1 + k];
+}
+
 // The two segments of a lat_int group posted in one pass
 // (run_proc_<group>, kmos/io/__init__.py:1497-1624): del_proc of every
 // modified cell's process as nli_<group> finds it before the writes, the
 // put/take writes, then add_proc as nli_<group> finds it after them.  Both
-// segments hold the same ops, so each lane evaluates its op's nli on both
-// lattices from one set of reads: the writes are done first and the
-// deletion's reads of a written site return its old species.  The queues
-// get the deletions in rounds 0..R-1 and the additions in rounds R..2R-1,
-// the order of the two passes.  Returns -1 (and changes nothing) when a
-// write would fail its species check: the caller then runs the segments
-// one by one, which reproduces the reference's behaviour on a mismatch.
+// segments hold the same ops, so each lane looks up its op's variant on
+// both lattices from one read of its cluster: every read (the written
+// sites and the clusters) sees the lattice before the writes, and the
+// additions' lookups take the written sites' new species.  The queues get
+// the deletions in rounds 0..R-1 and the additions in rounds R..2R-1, the
+// order of the two passes.  Returns -1 (and changes nothing) when a write
+// would fail its species check: the caller then runs the segments one by
+// one, which reproduces the reference's behaviour on a mismatch.  Every
+// nli group of the model is tabulated.
 template <int PPL, int BITS>
-__device__ __forceinline__ int post_latint_pair(Wave<PPL, BITS> &W, const SegPre &q1, int nw, int woff, int bx,
+__device__ __forceinline__ int post_latint_pair(Wave<PPL, BITS> &W, const SegPre &q1, const LiPre &lp, int bx,
                                                 int by, int bz)
 {
     const int *prog = W.prog;
-    const int nops = q1.nops, R = q1.nrounds;
+    const int nops = q1.nops, R = q1.nrounds, nw = lp.nw;
     const int lb = W.d.lin(bx, by, bz);
-    // the writes, one lane each (at most two)
-    int s = -1, cur = 0, nsp = 0, t = 1;
-    bool bad = false;
-    if (W.lane < nw) {
-        const int w0 = prog[woff + WR_W * W.lane], w1 = prog[woff + WR_W * W.lane + 1];
-        t = (w0 >> 24) & 0xFF;
-        s = site_from<true>(W.d, lb, bx, by, bz, unpack_x(w0), unpack_y(w0), unpack_z(w0), t);
-        cur = W.lat.get(s);
-        nsp = w1 >> 16;
-        bad = cur < 0 || !((w1 >> cur) & 1);
+    // the written sites (one lane each) and every op lane's cluster, read
+    // together on the lattice before the writes
+    const bool wr = W.lane < nw;
+    const int t = (lp.w0 >> 24) & 0xFF;
+    const int s = wr ? site_from<true>(W.d, lb, bx, by, bz, unpack_x(lp.w0), unpack_y(lp.w0), unpack_z(lp.w0), t)
+                     : -1;
+    const int nsp = lp.w1 >> 16;
+    const int cur = W.lat.get(wr ? s : 0);
+    const bool isop = W.lane < nops;
+    const int2 op = q1.op;
+    const int dx = sext7(op.y & 0x7F), dy = sext7((op.y >> 7) & 0x7F), dz = sext7((op.y >> 14) & 0x7F);
+    const int cx = wrap1(bx + dx, W.d.X);
+    const int cy = W.d.dim >= 2 ? wrap1(by + dy, W.d.Y) : 0;
+    const int cz = W.d.dim >= 3 ? wrap1(bz + dz, W.d.Z) : 0;
+    const int lc = W.d.lin(cx, cy, cz);
+    const int K = lp.hdr & 0xFF, S = (lp.hdr >> 8) & 0xFF;
+    int st[8], so[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int c = (isop && k < K) ? lp.cw[k] : (1 << 24);
+        st[k] = site_from<true>(W.d, lc, cx, cy, cz, unpack_x(c), unpack_y(c), unpack_z(c), (c >> 24) & 0xFF);
+        so[k] = W.lat.get(st[k]);
     }
-    const int sA = readlane_i(s, 0), oA = readlane_i(cur, 0);
-    const int sB = readlane_i(s, 1), oB = readlane_i(cur, 1);
-    if (__ballot(W.lane < nw && bad))
+    const bool bad = wr && (cur < 0 || !((lp.w1 >> cur) & 1));
+    if (__ballot(bad))
         return -1;
-    if (W.lane < nw) {
+    const int sA = readlane_i(s, 0), nA = readlane_i(nsp, 0);
+    const int sB = readlane_i(s, 1), nB = readlane_i(nsp, 1);
+    if (wr) {
         W.lat.xset(s, cur, nsp);
         if (W.track) {
             if (cur < W.S)
@@ -751,19 +776,22 @@ __device__ __forceinline__ int post_latint_pair(Wave<PPL, BITS> &W, const SegPre
                 atomicAdd(&W.occ[nsp * W.d.spuck + t - 1], 1);
         }
     }
-    wave_sync();
-    const int2 op = q1.op;
-    int pd = 0, pa = 0, so = 0, rnd = 0;
-    if (W.lane < nops) {
+    int pd = 0, pa = 0, rnd = -1;
+    if (isop) {
+        int i0 = 0, i1 = 0;
+#pragma unroll
+        for (int k = 7; k >= 0; --k) {
+            const int sn = st[k] == sA ? nA : (st[k] == sB ? nB : so[k]);
+            i0 = (k < K) ? i0 * S + so[k] : i0;
+            i1 = (k < K) ? i1 * S + sn : i1;
+        }
+        const unsigned char *tab = (const unsigned char *)(prog + lp.th + 1 + K);
+        pd = tab[i0];
+        pa = tab[i1];
         rnd = (op.x >> 3) & 0xFF;
-        const int arg = (int)((unsigned)op.x >> 23);
-        const int dx = sext7(op.y & 0x7F), dy = sext7((op.y >> 7) & 0x7F), dz = sext7((op.y >> 14) & 0x7F);
-        const int cx = wrap1(bx + dx, W.d.X);
-        const int cy = W.d.dim >= 2 ? wrap1(by + dy, W.d.Y) : 0;
-        const int cz = W.d.dim >= 3 ? wrap1(bz + dz, W.d.Z) : 0;
-        so = W.d.lin(cx, cy, cz);
-        nli_eval_pair<PPL, BITS, true>(W, arg, cx, cy, cz, sA, oA, sB, oB, pd, pa);
     }
+    // the round of each pass's entry, -1 for none
+    const int rd = pd > 0 ? rnd : -1, ra = pa > 0 ? rnd : -1;
     int sd = 0, sa = 0;
     if (R <= 8) {
         // one result register per round: a pass's atomics issue back to
@@ -772,33 +800,33 @@ __device__ __forceinline__ int post_latint_pair(Wave<PPL, BITS> &W, const SegPre
 #pragma unroll
         for (int it = 0; it < 8; ++it) {
             sl[it] = 0;
-            if (it < R && pd > 0 && rnd == it)
+            if (rd == it)
                 sl[it] = atomicAdd(&W.cnt[pd - 1], 1);
         }
 #pragma unroll
         for (int it = 0; it < 8; ++it)
-            sd = (rnd == it) ? sl[it] : sd;
+            sd = (rd == it) ? sl[it] : sd;
 #pragma unroll
         for (int it = 0; it < 8; ++it) {
             sl[it] = 0;
-            if (it < R && pa > 0 && rnd == it)
+            if (ra == it)
                 sl[it] = atomicAdd(&W.cnt[pa - 1], 1);
         }
 #pragma unroll
         for (int it = 0; it < 8; ++it)
-            sa = (rnd == it) ? sl[it] : sa;
+            sa = (ra == it) ? sl[it] : sa;
     } else {
         for (int it = 0; it < R; ++it)
-            if (pd > 0 && rnd == it)
+            if (rd == it)
                 sd = atomicAdd(&W.cnt[pd - 1], 1);
         for (int it = 0; it < R; ++it)
-            if (pa > 0 && rnd == it)
+            if (ra == it)
                 sa = atomicAdd(&W.cnt[pa - 1], 1);
     }
     if (pd > 0)
-        W.mbox[sd * W.mstride + pd - 1] = ((so + 1) << 2) | MB_DEL_STRICT;
+        W.mbox[sd * W.mstride + pd - 1] = ((lc + 1) << 2) | MB_DEL_STRICT;
     if (pa > 0)
-        W.mbox[sa * W.mstride + pa - 1] = ((so + 1) << 2) | MB_ADD;
+        W.mbox[sa * W.mstride + pa - 1] = ((lc + 1) << 2) | MB_ADD;
     wave_sync();
     return ST_OK;
 }
@@ -847,7 +875,8 @@ __device__ __forceinline__ void proc_prefetch(const int *prog, int off_proc, int
 // `mode` is set (the model's nli groups are all tabulated).  The first segment's
 // static words come read ahead in `pre`.
 template <int PPL, int BITS, bool RB2 = false, class Mid = NoMid, bool LIP = false>
-__device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, int mode, Mid &mid)
+__device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, int mode, Mid &mid,
+                        const LiPre &li = LiPre{})
 {
     const bool merged = mode & 1;
     const int s0 = site - 1;
@@ -878,18 +907,14 @@ __device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, int mo
                      post_pair<PPL, BITS, true>(W, cur, q2, bx1, by1, bz1, bx2, by2, bz2) >= 0;
         }
     }
-    if (LIP && (mode & 2) && nit == 2 && cur.nw == 0 && !cur.as_add && cur.nops <= 64 &&
-        it[1] == it[IT_W + 1]) {
+    if (LIP && li.ok) {
         // a lat_int group: deletions, writes, additions over the same ops
-        // (bit 1 is set only with bit 0)
-        const int *sg = W.prog + W.off_seg + SEG_W * it[IT_W];
-        const int nw2 = sg[0], woff2 = sg[1], h2 = sg[2];
-        const int o = it[1];
+        // (bit 1 is set only with bit 0; the static words in li)
+        const int o = li.off;
         const int bx = wrap1(x + unpack_x(o), W.d.X);
         const int by = W.d.dim >= 2 ? wrap1(y + unpack_y(o), W.d.Y) : 0;
         const int bz = W.d.dim >= 3 ? wrap1(z + unpack_z(o), W.d.Z) : 0;
-        posted = nw2 <= 2 && ((h2 >> 24) & 1) && (h2 & 0xFFFF) == cur.nops && sg[3] == cur.ooff &&
-                 W.d.interior(bx, by, bz) && post_latint_pair<PPL, BITS>(W, cur, nw2, woff2, bx, by, bz) >= 0;
+        posted = W.d.interior(bx, by, bz) && post_latint_pair<PPL, BITS>(W, cur, li, bx, by, bz) >= 0;
     }
     for (int i = 0; i < (posted ? 0 : nit); ++i, it += IT_W) {
         const int o = it[1];
@@ -1099,6 +1124,7 @@ __device__ __forceinline__ int wave_setup(Wave<PPL, BITS> &W, const KParams &kp,
 #if KMCG_PHASES
     for (int i = 0; i < 5; ++i)
         W.tph[i] = 0;
+    W.nblk = W.napply = 0;
     W.tmark = __builtin_amdgcn_s_memtime();
 #endif
     for (int i = W.lane; i < W.mstride; i += 64)
@@ -1359,10 +1385,9 @@ __device__ __forceinline__ void acf_after(const KParams &kp, int r)
 // and search run during the replay's HBM reads.  It shortens a trajectory's
 // chain of dependent steps and adds instructions, so it pays where few
 // replica-waves share a SIMD (latency-bound) and not where many do.
-template <int PPL, int BITS, bool LDS_LAT, bool MSD, bool OV = false, bool LIP = false>
-__global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4))) void kmc_steps_kernel(KParams kp)
+template <int PPL, int BITS, bool LDS_LAT, bool MSD, bool OV, bool LIP>
+__device__ __forceinline__ void steps_loop(const KParams &kp, unsigned char *smem)
 {
-    extern __shared__ __align__(16) unsigned char smem[];
     Wave<PPL, BITS> W;
     const int r = wave_setup<PPL, BITS, LDS_LAT>(W, kp, smem);
     if (r < 0)
@@ -1430,6 +1455,7 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
 #if KMCG_PHASES
     for (int i = 0; i < 5; ++i)
         W.tph[i] = 0;
+    W.nblk = W.napply = 0;
     W.tmark = __builtin_amdgcn_s_memtime();
 #endif
     while (done < n) {
@@ -1487,6 +1513,10 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
         // the process's program is read while its site is on the way
         ProcPre pre;
         proc_prefetch(W.prog, W.off_proc, W.off_seg, pick + 1, lane, pre);
+        // (the LIP form runs only models with bit 1 of kp.merged)
+        LiPre li;
+        if constexpr (LIP)
+            li_prefetch(W.prog, W.off_seg, W.off_nlitab, lane, pre.nit, pre.it, pre.seg, li);
         uj += 1;
         const int site = readlane_i(sv, pl);
         PH_MARK(W, 1);
@@ -1511,7 +1541,7 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
             s_pick = nx.pick;
         } else {
             NoMid none;
-            st = run_proc<PPL, BITS, !MSD, NoMid, LIP>(W, pre, site, kp.merged, none);
+            st = run_proc<PPL, BITS, !MSD, NoMid, LIP>(W, pre, site, kp.merged, none, li);
         }
         if (MSD && kp.acf_pa && lane == 0)
             acf_after(kp, r);
@@ -1538,6 +1568,8 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
         for (int i = 0; i < 5; ++i)
             kp.phase[PHASE_W * r + i] += W.tph[i];
         kp.phase[PHASE_W * r + 5] += (unsigned long long)done;
+        kp.phase[PHASE_W * r + 6] += W.nblk;
+        kp.phase[PHASE_W * r + 7] += W.napply;
     }
 #endif
     if (lane == 0) {
@@ -1550,6 +1582,23 @@ __global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4)
         if (kp.budget)
             kp.budget[slot] -= done;
     }
+}
+
+template <int PPL, int BITS, bool LDS_LAT, bool MSD, bool OV = false>
+__global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4))) void kmc_steps_kernel(KParams kp)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    steps_loop<PPL, BITS, LDS_LAT, MSD, OV, false>(kp, smem);
+}
+
+// The LIP form (lat_int models whose nli groups are all tabulated): a
+// group's two segments posted in one pass, their static words read while
+// the step's site is on its way.
+template <int PPL, int BITS, bool LDS_LAT>
+__global__ __launch_bounds__(64 * MAX_WPG) __attribute__((amdgpu_waves_per_eu(4))) void kmc_steps_kernel_li(KParams kp)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    steps_loop<PPL, BITS, LDS_LAT, false, false, true>(kp, smem);
 }
 
 // touchups, run_proc_nr, get_next_kmc_step (one replica per wave)
@@ -2645,6 +2694,7 @@ KParams make_params(kmcg_batch *b)
 struct Shape {
     bool lds_lat;
     bool ov;           // the step kernel's overlapped form
+    bool li;           // the step kernel's one-pass lat_int form
     int wpg, grid;
     size_t lds;
 };
@@ -2667,6 +2717,8 @@ Shape launch_shape(kmcg_batch *b, const int *ids, int n, KParams &kp, int max_wp
     // queue more than one replay block, where the overlap does not engage
     s.ov = b->ov_mode == 1 ||
            (b->ov_mode == -1 && b->m->backend == 0 && b->m->P > 16 && (long long)n <= 8LL * b->n_cu);
+    // lat_int with every nli group tabulated
+    s.li = (b->m->merged & 2) && ppl_of(b->m->P) == 1;
     while (s.wpg > 1 && prog + s.wpg * per_wave > (size_t)b->max_lds)
         s.wpg -= 1;
     s.grid = (n + s.wpg - 1) / s.wpg;
@@ -2692,12 +2744,12 @@ hipError_t launch_t(bool steps, const Shape &s, hipStream_t st, const KParams &k
             hipLaunchKernelGGL((kmc_steps_kernel<1, BITS, true, false, true>), grid, block, s.lds, st, kp);
         else
             hipLaunchKernelGGL((kmc_steps_kernel<1, BITS, false, false, true>), grid, block, s.lds, st, kp);
-    } else if (steps && PPL == 1 && (kp.merged & 2)) {
+    } else if (steps && PPL == 1 && s.li) {
         // lat_int with tabulated nli groups: a group's segments in one pass
         if (s.lds_lat)
-            hipLaunchKernelGGL((kmc_steps_kernel<1, BITS, true, false, false, true>), grid, block, s.lds, st, kp);
+            hipLaunchKernelGGL((kmc_steps_kernel_li<1, BITS, true>), grid, block, s.lds, st, kp);
         else
-            hipLaunchKernelGGL((kmc_steps_kernel<1, BITS, false, false, false, true>), grid, block, s.lds, st, kp);
+            hipLaunchKernelGGL((kmc_steps_kernel_li<1, BITS, false>), grid, block, s.lds, st, kp);
     } else if (steps) {
         if (s.lds_lat)
             hipLaunchKernelGGL((kmc_steps_kernel<PPL, BITS, true, false>), grid, block, s.lds, st, kp);
