@@ -316,7 +316,7 @@ def batched_leg(engine, wl, K, warmup, tune, cpu, spot=(0,)):
            "gpu_steps_per_s": executed / wall, "wall_s": wall, "avg_launch_ms": avg_s * 1e3,
            "launch": launch_summary(choice), "parity_spot_check": ok,
            "replica_status_counts": {str(k): int(v) for k, v in zip(*np.unique(st, return_counts=True))},
-           "roofline": roofline(wl.name, algorithmic_bytes_per_step(wl), executed / K, avg_s)}
+           "roofline": roofline(wl.name, algorithmic_bytes_per_step(wl), executed / K, avg_s, model.step_kernel)}
     if cpu:
         out["cpu_baseline"] = oracle_cpu_baseline(wl, 10.0)
         out["gpu_over_cpu"] = out["gpu_steps_per_s"] / out["cpu_baseline"]["value"]

@@ -65,6 +65,11 @@ int kmcg_model_load(const int32_t *blob, int64_t nwords, kmcg_model **out);
 void kmcg_model_free(kmcg_model *m);
 /* nr_of_proc, nr_of_species, spuck, backend, dimension */
 int kmcg_model_info(const kmcg_model *m, int32_t *info5);
+/* The step kernel's forms this model can run: *li = 1 when a lat_int
+ * group's two segments are posted in one pass (kmc_steps_kernel_li: every
+ * nli group tabulated, at most 64 processes).  No counterpart in the
+ * reference (a profiling aid). */
+int kmcg_model_forms(const kmcg_model *m, int32_t *li);
 
 /* Device bytes a batch of n replicas of sizes[n][3] needs. */
 int kmcg_batch_footprint(const kmcg_model *m, int32_t n, const int32_t *sizes, int64_t *bytes);

@@ -27,6 +27,7 @@ SIGNATURES = [
     ("kmcg_model_load", _i32, [_vp, _i64, _vp]),
     ("kmcg_model_free", None, [_vp]),
     ("kmcg_model_info", _i32, [_vp, _vp]),
+    ("kmcg_model_forms", _i32, [_vp, _vp]),
     ("kmcg_batch_footprint", _i32, [_vp, _i32, _vp, _vp]),
     ("kmcg_batch_create", _i32, [_vp, _i32, _vp, _i32, _vp]),
     ("kmcg_batch_free", None, [_vp]),

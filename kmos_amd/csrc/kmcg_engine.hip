@@ -2913,6 +2913,14 @@ int kmcg_model_info(const kmcg_model *m, int32_t *info5)
     return KMCG_OK;
 }
 
+int kmcg_model_forms(const kmcg_model *m, int32_t *li)
+{
+    if (!m || !li)
+        return fail(KMCG_EINVAL, "null argument");
+    *li = ((m->merged & 2) && ppl_of(m->P) == 1) ? 1 : 0;
+    return KMCG_OK;
+}
+
 static int sizes_ok(const kmcg_model *m, int32_t n, const int32_t *sizes)
 {
     if (!m || n < 1 || !sizes)

@@ -39,6 +39,10 @@ class Model:
         # the species id of an unpopulated site: -1 for one layer (never
         # stored), nr_of_species for several layers (kmos' null_species)
         self.null_species = int(compiled.null_species)
+        li = ctypes.c_int32(0)
+        _lib.check(lib.kmcg_model_forms(self.h, ctypes.byref(li)))
+        # the step kernel the library launches for this model's batches
+        self.step_kernel = "kmc_steps_kernel_li" if li.value else "kmc_steps_kernel"
 
     def __del__(self):
         if getattr(self, "h", None) is not None and _lib._lib is not None:
