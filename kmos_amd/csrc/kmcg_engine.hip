@@ -72,6 +72,7 @@ struct Wave {
     int *cnt;          // [64*PPL] queue lengths
     int *win;          // [64][WIN] list-tail window of each lane (replay)
     int *occ;          // [S][spuck] species counts, kept while `track`
+    int *bvq;          // [RBLK][64*PPL] read-ahead back-indices (LIP form)
     bool track;
     bool chk;          // lists match the lattice: disable checks may test conditions
     bool local_smart;
@@ -183,6 +184,16 @@ struct Rows {
 // w: this lane's window in LDS, slot q at w[q - (nr0 - RB + 1)].  RB (1, 2
 // or RBLK) bounds the entries any lane replays in this block: the reads,
 // the window and the pairwise tests are set up for RB entries only.
+// List words of a lane's first replay block read ahead of the replay (the
+// LIP form): back[site] of each entry that deletes (from the post that
+// queued it) and the list slots nr, nr-1, ... (read before the post).
+// Nothing in the step changes them before the block runs.
+struct RePre {
+    bool on = false;
+    int bv[RBLK];
+    int tv[RBLK];
+};
+
 // Work to run while a replay block's reads are on their way (wave-uniform,
 // every lane active); the default does nothing.
 struct NoMid {
@@ -194,7 +205,7 @@ struct NoMid {
 
 template <int PPL, int BITS, int RB, class Mid>
 __device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, const Rows &L, int *w,
-                                             const int (&v)[RBLK], int m, int &err, Mid &mid)
+                                             const int (&v)[RBLK], int m, int &err, Mid &mid, const RePre &pre)
 {
     const int nr0 = nr;
     // per entry: deletion?, site; pairwise equal sites, computed once
@@ -211,13 +222,15 @@ __device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, const 
     int nd = 0;
 #pragma unroll
     for (int i = 0; i < RB; ++i) {
-        bv[i] = L.b(isdel[i] ? st[i] : 0);
+        bv[i] = pre.on ? pre.bv[i] : L.b(isdel[i] ? st[i] : 0);
         nd += isdel[i] ? 1 : 0;
     }
+    // (read ahead, every slot is read: the window's slots below nr0-nd+1
+    // are written but never read)
     int tv[RB];
 #pragma unroll
     for (int j = 0; j < RB; ++j)
-        tv[j] = L.s((j < nd && nr0 - j >= 1) ? nr0 - j : 0);
+        tv[j] = pre.on ? pre.tv[j] : L.s((j < nd && nr0 - j >= 1) ? nr0 - j : 0);
     if (Mid::active && mid.armed) {
         // the count this lane's block leaves when every deletion finds its
         // site registered
@@ -295,9 +308,10 @@ __device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, const 
 
 // Apply the queued entries of the step: each owner lane replays its
 // processes' queues in order, RBLK entries per batch of HBM reads, and
-// empties them.
+// empties them.  With pre.on (one process slot per lane) the first block
+// takes its list words from pre.tv and W.bvq instead of reading them.
 template <int PPL, int BITS, bool RB2 = false, class Mid = NoMid>
-__device__ __forceinline__ int apply_queues(Wave<PPL, BITS> &W, Mid &mid)
+__device__ __forceinline__ int apply_queues(Wave<PPL, BITS> &W, Mid &mid, RePre &pre)
 {
     int err = 0;
 #pragma unroll
@@ -311,6 +325,11 @@ __device__ __forceinline__ int apply_queues(Wave<PPL, BITS> &W, Mid &mid)
 #pragma unroll
         for (int i = 0; i < RBLK; ++i)
             v[i] = q[min(i, W.mrows - 1) * W.mstride];
+        if (PPL == 1 && pre.on) {
+#pragma unroll
+            for (int i = 0; i < RBLK; ++i)
+                pre.bv[i] = W.bvq[i * W.mstride + p];
+        }
         n = own ? n : 0;
         W.cnt[p] = 0;
 #if KMCG_PHASES
@@ -325,6 +344,7 @@ __device__ __forceinline__ int apply_queues(Wave<PPL, BITS> &W, Mid &mid)
 #pragma unroll
                 for (int i = 0; i < RBLK; ++i)
                     v[i] = q[min(i0 + i, W.mrows - 1) * W.mstride];
+                pre.on = false;
             }
 #pragma unroll
             for (int i = 0; i < RBLK; ++i)
@@ -344,11 +364,11 @@ __device__ __forceinline__ int apply_queues(Wave<PPL, BITS> &W, Mid &mid)
                 // two-entry form as well (RB2) costs code size, so only the
                 // plain step loop takes it
                 if (__ballot(m > 2) || (!RB2 && __ballot(m > 1)))
-                    replay_block<PPL, BITS, RBLK>(W, nrk, L, W.win + WIN * W.lane, v, max(m, 0), err, mid);
+                    replay_block<PPL, BITS, RBLK>(W, nrk, L, W.win + WIN * W.lane, v, max(m, 0), err, mid, pre);
                 else if (RB2 && __ballot(m > 1))
-                    replay_block<PPL, BITS, 2>(W, nrk, L, W.win + WIN * W.lane, v, max(m, 0), err, mid);
+                    replay_block<PPL, BITS, 2>(W, nrk, L, W.win + WIN * W.lane, v, max(m, 0), err, mid, pre);
                 else
-                    replay_block<PPL, BITS, 1>(W, nrk, L, W.win + WIN * W.lane, v, max(m, 0), err, mid);
+                    replay_block<PPL, BITS, 1>(W, nrk, L, W.win + WIN * W.lane, v, max(m, 0), err, mid, pre);
                 W.nr[k] = nrk;
             }
         }
@@ -790,6 +810,9 @@ __device__ __forceinline__ int post_latint_pair(Wave<PPL, BITS> &W, const SegPre
         pa = tab[i1];
         rnd = (op.x >> 3) & 0xFF;
     }
+    // back[site] of the deletion, for the replay's first block (read now,
+    // stored once its queue slot is known)
+    const int bpre = (PPL == 1 && pd > 0) ? W.back(pd - 1)[lc + 1] : 0;
     // the round of each pass's entry, -1 for none
     const int rd = pd > 0 ? rnd : -1, ra = pa > 0 ? rnd : -1;
     int sd = 0, sa = 0;
@@ -827,6 +850,8 @@ __device__ __forceinline__ int post_latint_pair(Wave<PPL, BITS> &W, const SegPre
         W.mbox[sd * W.mstride + pd - 1] = ((lc + 1) << 2) | MB_DEL_STRICT;
     if (pa > 0)
         W.mbox[sa * W.mstride + pa - 1] = ((lc + 1) << 2) | MB_ADD;
+    if (PPL == 1 && pd > 0 && sd < RBLK)
+        W.bvq[sd * W.mstride + pd - 1] = bpre;
     wave_sync();
     return ST_OK;
 }
@@ -848,7 +873,8 @@ __device__ int run_segment(Wave<PPL, BITS> &W, int seg, int bx, int by, int bz)
     seg_prefetch(W.prog, W.off_seg, seg, W.lane, q);
     const int e = post_segment<PPL, BITS>(W, q, bx, by, bz);
     NoMid none;
-    const int a = apply_queues<PPL, BITS>(W, none);
+    RePre nopre;
+    const int a = apply_queues<PPL, BITS>(W, none, nopre);
     return e ? e : a;
 }
 
@@ -891,6 +917,7 @@ __device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, int mo
     int e = ST_OK;
     // both segments posted in one pass (merged only): applied below
     bool posted = false;
+    RePre rpre;
     if (merged && W.local_smart && nit == 2 && cur.nw == 1 && !cur.as_add) {
         SegPre q2;
         seg_prefetch(W.prog, W.off_seg, it[IT_W], W.lane, q2);
@@ -914,7 +941,18 @@ __device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, int mo
         const int bx = wrap1(x + unpack_x(o), W.d.X);
         const int by = W.d.dim >= 2 ? wrap1(y + unpack_y(o), W.d.Y) : 0;
         const int bz = W.d.dim >= 3 ? wrap1(z + unpack_z(o), W.d.Z) : 0;
-        posted = W.d.interior(bx, by, bz) && post_latint_pair<PPL, BITS>(W, cur, li, bx, by, bz) >= 0;
+        if (W.d.interior(bx, by, bz)) {
+            // the owner lanes' list tails, read while the pass posts
+            if (PPL == 1) {
+                const int nr0 = W.nr[0];
+                const int *sl = W.sites(W.lane < W.P ? W.lane : 0);
+#pragma unroll
+                for (int j = 0; j < RBLK; ++j)
+                    rpre.tv[j] = sl[nr0 - j >= 1 ? nr0 - j : 0];
+            }
+            posted = post_latint_pair<PPL, BITS>(W, cur, li, bx, by, bz) >= 0;
+            rpre.on = PPL == 1 && posted;
+        }
     }
     for (int i = 0; i < (posted ? 0 : nit); ++i, it += IT_W) {
         const int o = it[1];
@@ -929,14 +967,15 @@ __device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, int mo
         if (!merged) {
             NoMid none;
             PH_MARK(W, 2);
-            e = apply_queues<PPL, BITS, RB2>(W, none);
+            RePre nopre;
+            e = apply_queues<PPL, BITS, RB2>(W, none, nopre);
             PH_MARK(W, 3);
             if (e)
                 return e;
         }
     }
     PH_MARK(W, 2);
-    const int a = merged ? apply_queues<PPL, BITS, RB2>(W, mid) : ST_OK;
+    const int a = merged ? apply_queues<PPL, BITS, RB2>(W, mid, rpre) : ST_OK;
     PH_MARK(W, 3);
     return e ? e : a;
 }
@@ -1118,6 +1157,7 @@ __device__ __forceinline__ int wave_setup(Wave<PPL, BITS> &W, const KParams &kp,
     W.cnt = W.mbox + kp.mrows * W.mstride;
     W.win = W.cnt + W.mstride;
     W.occ = W.win + 64 * WIN;
+    W.bvq = W.occ + ((kp.S * kp.spuck + 3) & ~3);
     W.track = false;
     W.chk = false;
     W.local_smart = kp.backend == 0;
@@ -2705,9 +2745,13 @@ Shape launch_shape(kmcg_batch *b, const int *ids, int n, KParams &kp, int max_wp
     for (int i = 0; i < n; ++i)
         maxlb = std::max(maxlb, lattice_bytes(b->V[ids[i]], b->bits));
     const size_t occ = ((size_t)b->m->S * b->m->spuck + 3) & ~(size_t)3;
-    const size_t mbox = ((size_t)(b->m->mrows + 1) * 64 * ppl_of(b->m->P) + 64 * WIN + occ) * sizeof(int);
-    const size_t prog = (size_t)kp.prog_lds;
     Shape s;
+    // lat_int with every nli group tabulated: the one-pass form, which
+    // keeps read-ahead back-indices of the first replay block (bvq)
+    s.li = (b->m->merged & 2) && ppl_of(b->m->P) == 1;
+    const size_t bvq = s.li ? (size_t)RBLK * 64 : 0;
+    const size_t mbox = ((size_t)(b->m->mrows + 1) * 64 * ppl_of(b->m->P) + 64 * WIN + occ + bvq) * sizeof(int);
+    const size_t prog = (size_t)kp.prog_lds;
     s.lds_lat = b->lds_mode == 0 ? false : (prog + (size_t)maxlb + mbox <= (size_t)b->max_lds);
     const size_t per_wave = (s.lds_lat ? (size_t)maxlb : 0) + mbox;
     s.wpg = std::max(1, std::min(max_wpg, std::min(MAX_WPG, n)));
@@ -2717,8 +2761,6 @@ Shape launch_shape(kmcg_batch *b, const int *ids, int n, KParams &kp, int max_wp
     // queue more than one replay block, where the overlap does not engage
     s.ov = b->ov_mode == 1 ||
            (b->ov_mode == -1 && b->m->backend == 0 && b->m->P > 16 && (long long)n <= 8LL * b->n_cu);
-    // lat_int with every nli group tabulated
-    s.li = (b->m->merged & 2) && ppl_of(b->m->P) == 1;
     while (s.wpg > 1 && prog + s.wpg * per_wave > (size_t)b->max_lds)
         s.wpg -= 1;
     s.grid = (n + s.wpg - 1) / s.wpg;
