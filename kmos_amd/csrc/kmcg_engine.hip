@@ -598,7 +598,7 @@ __device__ __forceinline__ int post_segment_t(Wave<PPL, BITS> &W, const SegPre &
 // write would fail its species check: the caller then runs the segments
 // one by one, which also reproduces the reference's behaviour on a
 // mismatch.
-template <int PPL, int BITS, bool IN>
+template <int PPL, int BITS, bool IN, bool RA = false>
 __device__ __forceinline__ int post_pair(Wave<PPL, BITS> &W, const SegPre &q1, const SegPre &q2, int bx1, int by1,
                                          int bz1, int bx2, int by2, int bz2)
 {
@@ -690,6 +690,9 @@ __device__ __forceinline__ int post_pair(Wave<PPL, BITS> &W, const SegPre &q1, c
             code = del ? MB_DEL_IF : MB_ADD;
         }
     }
+    // RA: back[site] of a deletion, for the replay's first block (read now,
+    // stored once its queue slot is known)
+    const int bpre = (RA && PPL == 1 && proc > 0 && code != MB_ADD) ? W.back(proc - 1)[so + 1] : 0;
     const int nrt = q1.nrounds + q2.nrounds;
     int slot = 0;
     for (int it = 0; it < nrt; ++it)
@@ -697,6 +700,8 @@ __device__ __forceinline__ int post_pair(Wave<PPL, BITS> &W, const SegPre &q1, c
             slot = atomicAdd(&W.cnt[proc - 1], 1);
     if (proc > 0)
         W.mbox[slot * W.mstride + proc - 1] = ((so + 1) << 2) | code;
+    if (RA && PPL == 1 && proc > 0 && code != MB_ADD && slot < RBLK)
+        W.bvq[slot * W.mstride + proc - 1] = bpre;
     wave_sync();
     return ST_OK;
 }
@@ -879,6 +884,18 @@ __device__ int run_segment(Wave<PPL, BITS> &W, int seg, int bx, int by, int bz)
 }
 
 // The process's items and its first segment's static words, read ahead.
+// The owner lanes' list slots nr, nr-1, ... (one process slot per lane),
+// read before a post so that the first replay block need not read them.
+template <int PPL, int BITS>
+__device__ __forceinline__ void tail_ahead(const Wave<PPL, BITS> &W, RePre &q)
+{
+    const int nr0 = W.nr[0];
+    const int *sl = W.sites(W.lane < W.P ? W.lane : 0);
+#pragma unroll
+    for (int j = 0; j < RBLK; ++j)
+        q.tv[j] = sl[nr0 - j >= 1 ? nr0 - j : 0];
+}
+
 struct ProcPre {
     int nit;
     const int *it;
@@ -900,7 +917,7 @@ __device__ __forceinline__ void proc_prefetch(const int *prog, int off_proc, int
 // one-pass posting of a lat_int group's two segments, taken when bit 1 of
 // `mode` is set (the model's nli groups are all tabulated).  The first segment's
 // static words come read ahead in `pre`.
-template <int PPL, int BITS, bool RB2 = false, class Mid = NoMid, bool LIP = false>
+template <int PPL, int BITS, bool RB2 = false, class Mid = NoMid, bool LIP = false, bool RA = false>
 __device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, int mode, Mid &mid,
                         const LiPre &li = LiPre{})
 {
@@ -917,7 +934,7 @@ __device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, int mo
     int e = ST_OK;
     // both segments posted in one pass (merged only): applied below
     bool posted = false;
-    RePre rpre;
+    RePre rpre;       // read-ahead list words of the first replay block
     if (merged && W.local_smart && nit == 2 && cur.nw == 1 && !cur.as_add) {
         SegPre q2;
         seg_prefetch(W.prog, W.off_seg, it[IT_W], W.lane, q2);
@@ -930,8 +947,12 @@ __device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, int mo
             const int by2 = W.d.dim >= 2 ? wrap1(y + unpack_y(o2), W.d.Y) : 0;
             const int bz2 = W.d.dim >= 3 ? wrap1(z + unpack_z(o2), W.d.Z) : 0;
             // near the faces the segments run one by one
-            posted = W.d.interior(bx1, by1, bz1) && W.d.interior(bx2, by2, bz2) &&
-                     post_pair<PPL, BITS, true>(W, cur, q2, bx1, by1, bz1, bx2, by2, bz2) >= 0;
+            if (W.d.interior(bx1, by1, bz1) && W.d.interior(bx2, by2, bz2)) {
+                if (RA && PPL == 1)
+                    tail_ahead(W, rpre);
+                posted = post_pair<PPL, BITS, true, RA>(W, cur, q2, bx1, by1, bz1, bx2, by2, bz2) >= 0;
+                rpre.on = RA && PPL == 1 && posted;
+            }
         }
     }
     if (LIP && li.ok) {
@@ -943,13 +964,8 @@ __device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, int mo
         const int bz = W.d.dim >= 3 ? wrap1(z + unpack_z(o), W.d.Z) : 0;
         if (W.d.interior(bx, by, bz)) {
             // the owner lanes' list tails, read while the pass posts
-            if (PPL == 1) {
-                const int nr0 = W.nr[0];
-                const int *sl = W.sites(W.lane < W.P ? W.lane : 0);
-#pragma unroll
-                for (int j = 0; j < RBLK; ++j)
-                    rpre.tv[j] = sl[nr0 - j >= 1 ? nr0 - j : 0];
-            }
+            if (PPL == 1)
+                tail_ahead(W, rpre);
             posted = post_latint_pair<PPL, BITS>(W, cur, li, bx, by, bz) >= 0;
             rpre.on = PPL == 1 && posted;
         }
@@ -1574,6 +1590,8 @@ __device__ __forceinline__ void steps_loop(const KParams &kp, unsigned char *sme
         bool redo = false;
         if constexpr (OV && PPL == 1 && !MSD) {
             NextSel nx(uj < 64, readlane_d(lu1, uj & 63), acc[0]);
+            // (no read-ahead of the replay's list words here: their HBM
+            // round trip is what hides the next step's select)
             st = run_proc<PPL, BITS, true>(W, pre, site, kp.merged, nx);
             have = nx.made && !st && !__ballot(W.nr[0] != nx.nrs);
             redo = nx.made && !have;
@@ -1581,7 +1599,7 @@ __device__ __forceinline__ void steps_loop(const KParams &kp, unsigned char *sme
             s_pick = nx.pick;
         } else {
             NoMid none;
-            st = run_proc<PPL, BITS, !MSD, NoMid, LIP>(W, pre, site, kp.merged, none, li);
+            st = run_proc<PPL, BITS, !MSD, NoMid, LIP, PPL == 1 && !MSD>(W, pre, site, kp.merged, none, li);
         }
         if (MSD && kp.acf_pa && lane == 0)
             acf_after(kp, r);
@@ -2746,10 +2764,11 @@ Shape launch_shape(kmcg_batch *b, const int *ids, int n, KParams &kp, int max_wp
         maxlb = std::max(maxlb, lattice_bytes(b->V[ids[i]], b->bits));
     const size_t occ = ((size_t)b->m->S * b->m->spuck + 3) & ~(size_t)3;
     Shape s;
-    // lat_int with every nli group tabulated: the one-pass form, which
-    // keeps read-ahead back-indices of the first replay block (bvq)
+    // lat_int with every nli group tabulated: the one-pass form; every
+    // model with one process slot per lane keeps read-ahead back-indices
+    // of the first replay block (bvq)
     s.li = (b->m->merged & 2) && ppl_of(b->m->P) == 1;
-    const size_t bvq = s.li ? (size_t)RBLK * 64 : 0;
+    const size_t bvq = ppl_of(b->m->P) == 1 ? (size_t)RBLK * 64 : 0;
     const size_t mbox = ((size_t)(b->m->mrows + 1) * 64 * ppl_of(b->m->P) + 64 * WIN + occ + bvq) * sizeof(int);
     const size_t prog = (size_t)kp.prog_lds;
     s.lds_lat = b->lds_mode == 0 ? false : (prog + (size_t)maxlb + mbox <= (size_t)b->max_lds);
