@@ -184,12 +184,15 @@ struct Rows {
 // w: this lane's window in LDS, slot q at w[q - (nr0 - RB + 1)].  RB (1, 2
 // or RBLK) bounds the entries any lane replays in this block: the reads,
 // the window and the pairwise tests are set up for RB entries only.
-// List words of a lane's first replay block read ahead of the replay (the
-// LIP form): back[site] of each entry that deletes (from the post that
-// queued it) and the list slots nr, nr-1, ... (read before the post).
-// Nothing in the step changes them before the block runs.
+// List words of a lane's first replay block read ahead of the replay:
+// back[site] of each entry that deletes (read by the op lane that queued
+// it, qv, and stored at qi of W.bvq when the replay starts, so that the
+// read is waited for only then) and the list slots nr, nr-1, ... (read
+// before the post).  Nothing in the step changes them before the block
+// runs.
 struct RePre {
     bool on = false;
+    int qv = 0, qi = -1;
     int bv[RBLK];
     int tv[RBLK];
 };
@@ -309,7 +312,8 @@ __device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, const 
 // Apply the queued entries of the step: each owner lane replays its
 // processes' queues in order, RBLK entries per batch of HBM reads, and
 // empties them.  With pre.on (one process slot per lane) the first block
-// takes its list words from pre.tv and W.bvq instead of reading them.
+// takes its list words from pre.tv and W.bvq instead of reading them, and
+// an active hook runs before the read-ahead back-indices are waited for.
 template <int PPL, int BITS, bool RB2 = false, class Mid = NoMid>
 __device__ __forceinline__ int apply_queues(Wave<PPL, BITS> &W, Mid &mid, RePre &pre)
 {
@@ -325,12 +329,27 @@ __device__ __forceinline__ int apply_queues(Wave<PPL, BITS> &W, Mid &mid, RePre 
 #pragma unroll
         for (int i = 0; i < RBLK; ++i)
             v[i] = q[min(i, W.mrows - 1) * W.mstride];
+        n = own ? n : 0;
+        bool ran = false;
         if (PPL == 1 && pre.on) {
+            if (Mid::active && !__ballot(n > RBLK)) {
+                // the count this lane's block leaves when every deletion
+                // finds its site registered
+                int dn = 0;
+#pragma unroll
+                for (int i = 0; i < RBLK; ++i)
+                    dn += (i < n) ? (((v[i] & 3) != MB_ADD) ? -1 : 1) : 0;
+                mid.armed = true;
+                mid(W, W.nr[k] + dn);
+                ran = true;
+            }
+            if (pre.qi >= 0)
+                W.bvq[pre.qi] = pre.qv;
+            wave_sync();
 #pragma unroll
             for (int i = 0; i < RBLK; ++i)
                 pre.bv[i] = W.bvq[i * W.mstride + p];
         }
-        n = own ? n : 0;
         W.cnt[p] = 0;
 #if KMCG_PHASES
         W.napply += 1;
@@ -352,7 +371,7 @@ __device__ __forceinline__ int apply_queues(Wave<PPL, BITS> &W, Mid &mid, RePre 
             // with an active hook (one process slot per lane) every lane
             // takes part in the block; a lane without entries reads the
             // table's first word only
-            mid.armed = Mid::active && i0 == 0 && !__ballot(n > RBLK);
+            mid.armed = Mid::active && !ran && i0 == 0 && !__ballot(n > RBLK);
             if (Mid::active || m > 0) {
                 const bool any = m > 0;
                 int nrk = W.nr[k];
@@ -600,7 +619,7 @@ __device__ __forceinline__ int post_segment_t(Wave<PPL, BITS> &W, const SegPre &
 // mismatch.
 template <int PPL, int BITS, bool IN, bool RA = false>
 __device__ __forceinline__ int post_pair(Wave<PPL, BITS> &W, const SegPre &q1, const SegPre &q2, int bx1, int by1,
-                                         int bz1, int bx2, int by2, int bz2)
+                                         int bz1, int bx2, int by2, int bz2, RePre &pre)
 {
     const int *prog = W.prog;
     const int n1 = q1.nops, n2 = q2.nops;
@@ -691,7 +710,7 @@ __device__ __forceinline__ int post_pair(Wave<PPL, BITS> &W, const SegPre &q1, c
         }
     }
     // RA: back[site] of a deletion, for the replay's first block (read now,
-    // stored once its queue slot is known)
+    // stored when the replay starts)
     const int bpre = (RA && PPL == 1 && proc > 0 && code != MB_ADD) ? W.back(proc - 1)[so + 1] : 0;
     const int nrt = q1.nrounds + q2.nrounds;
     int slot = 0;
@@ -700,8 +719,10 @@ __device__ __forceinline__ int post_pair(Wave<PPL, BITS> &W, const SegPre &q1, c
             slot = atomicAdd(&W.cnt[proc - 1], 1);
     if (proc > 0)
         W.mbox[slot * W.mstride + proc - 1] = ((so + 1) << 2) | code;
-    if (RA && PPL == 1 && proc > 0 && code != MB_ADD && slot < RBLK)
-        W.bvq[slot * W.mstride + proc - 1] = bpre;
+    if (RA && PPL == 1) {
+        pre.qv = bpre;
+        pre.qi = (proc > 0 && code != MB_ADD && slot < RBLK) ? slot * W.mstride + proc - 1 : -1;
+    }
     wave_sync();
     return ST_OK;
 }
@@ -759,7 +780,7 @@ __device__ __forceinline__ void li_prefetch(const int *prog, int off_seg, int of
 // nli group of the model is tabulated.
 template <int PPL, int BITS>
 __device__ __forceinline__ int post_latint_pair(Wave<PPL, BITS> &W, const SegPre &q1, const LiPre &lp, int bx,
-                                                int by, int bz)
+                                                int by, int bz, RePre &pre)
 {
     const int *prog = W.prog;
     const int nops = q1.nops, R = q1.nrounds, nw = lp.nw;
@@ -816,7 +837,7 @@ __device__ __forceinline__ int post_latint_pair(Wave<PPL, BITS> &W, const SegPre
         rnd = (op.x >> 3) & 0xFF;
     }
     // back[site] of the deletion, for the replay's first block (read now,
-    // stored once its queue slot is known)
+    // stored when the replay starts)
     const int bpre = (PPL == 1 && pd > 0) ? W.back(pd - 1)[lc + 1] : 0;
     // the round of each pass's entry, -1 for none
     const int rd = pd > 0 ? rnd : -1, ra = pa > 0 ? rnd : -1;
@@ -855,8 +876,10 @@ __device__ __forceinline__ int post_latint_pair(Wave<PPL, BITS> &W, const SegPre
         W.mbox[sd * W.mstride + pd - 1] = ((lc + 1) << 2) | MB_DEL_STRICT;
     if (pa > 0)
         W.mbox[sa * W.mstride + pa - 1] = ((lc + 1) << 2) | MB_ADD;
-    if (PPL == 1 && pd > 0 && sd < RBLK)
-        W.bvq[sd * W.mstride + pd - 1] = bpre;
+    if (PPL == 1) {
+        pre.qv = bpre;
+        pre.qi = (pd > 0 && sd < RBLK) ? sd * W.mstride + pd - 1 : -1;
+    }
     wave_sync();
     return ST_OK;
 }
@@ -950,7 +973,7 @@ __device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, int mo
             if (W.d.interior(bx1, by1, bz1) && W.d.interior(bx2, by2, bz2)) {
                 if (RA && PPL == 1)
                     tail_ahead(W, rpre);
-                posted = post_pair<PPL, BITS, true, RA>(W, cur, q2, bx1, by1, bz1, bx2, by2, bz2) >= 0;
+                posted = post_pair<PPL, BITS, true, RA>(W, cur, q2, bx1, by1, bz1, bx2, by2, bz2, rpre) >= 0;
                 rpre.on = RA && PPL == 1 && posted;
             }
         }
@@ -966,7 +989,7 @@ __device__ int run_proc(Wave<PPL, BITS> &W, const ProcPre &pre, int site, int mo
             // the owner lanes' list tails, read while the pass posts
             if (PPL == 1)
                 tail_ahead(W, rpre);
-            posted = post_latint_pair<PPL, BITS>(W, cur, li, bx, by, bz) >= 0;
+            posted = post_latint_pair<PPL, BITS>(W, cur, li, bx, by, bz, rpre) >= 0;
             rpre.on = PPL == 1 && posted;
         }
     }
@@ -1591,7 +1614,8 @@ __device__ __forceinline__ void steps_loop(const KParams &kp, unsigned char *sme
         if constexpr (OV && PPL == 1 && !MSD) {
             NextSel nx(uj < 64, readlane_d(lu1, uj & 63), acc[0]);
             // (no read-ahead of the replay's list words here: their HBM
-            // round trip is what hides the next step's select)
+            // round trip is what hides the next step's select; measured
+            // slower with it, the hook run before the read-ahead's wait)
             st = run_proc<PPL, BITS, true>(W, pre, site, kp.merged, nx);
             have = nx.made && !st && !__ballot(W.nr[0] != nx.nrs);
             redo = nx.made && !have;
