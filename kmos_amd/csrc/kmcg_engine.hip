@@ -312,8 +312,7 @@ __device__ __forceinline__ void replay_block(Wave<PPL, BITS> &W, int &nr, const 
 // Apply the queued entries of the step: each owner lane replays its
 // processes' queues in order, RBLK entries per batch of HBM reads, and
 // empties them.  With pre.on (one process slot per lane) the first block
-// takes its list words from pre.tv and W.bvq instead of reading them, and
-// an active hook runs before the read-ahead back-indices are waited for.
+// takes its list words from pre.tv and W.bvq instead of reading them.
 template <int PPL, int BITS, bool RB2 = false, class Mid = NoMid>
 __device__ __forceinline__ int apply_queues(Wave<PPL, BITS> &W, Mid &mid, RePre &pre)
 {
@@ -330,19 +329,7 @@ __device__ __forceinline__ int apply_queues(Wave<PPL, BITS> &W, Mid &mid, RePre 
         for (int i = 0; i < RBLK; ++i)
             v[i] = q[min(i, W.mrows - 1) * W.mstride];
         n = own ? n : 0;
-        bool ran = false;
         if (PPL == 1 && pre.on) {
-            if (Mid::active && !__ballot(n > RBLK)) {
-                // the count this lane's block leaves when every deletion
-                // finds its site registered
-                int dn = 0;
-#pragma unroll
-                for (int i = 0; i < RBLK; ++i)
-                    dn += (i < n) ? (((v[i] & 3) != MB_ADD) ? -1 : 1) : 0;
-                mid.armed = true;
-                mid(W, W.nr[k] + dn);
-                ran = true;
-            }
             if (pre.qi >= 0)
                 W.bvq[pre.qi] = pre.qv;
             wave_sync();
@@ -371,7 +358,7 @@ __device__ __forceinline__ int apply_queues(Wave<PPL, BITS> &W, Mid &mid, RePre 
             // with an active hook (one process slot per lane) every lane
             // takes part in the block; a lane without entries reads the
             // table's first word only
-            mid.armed = Mid::active && !ran && i0 == 0 && !__ballot(n > RBLK);
+            mid.armed = Mid::active && i0 == 0 && !__ballot(n > RBLK);
             if (Mid::active || m > 0) {
                 const bool any = m > 0;
                 int nrk = W.nr[k];
@@ -1615,7 +1602,8 @@ __device__ __forceinline__ void steps_loop(const KParams &kp, unsigned char *sme
             NextSel nx(uj < 64, readlane_d(lu1, uj & 63), acc[0]);
             // (no read-ahead of the replay's list words here: their HBM
             // round trip is what hides the next step's select; measured
-            // slower with it, the hook run before the read-ahead's wait)
+            // slower with read-ahead and the select run before its wait,
+            // local_smart and lat_int alike)
             st = run_proc<PPL, BITS, true>(W, pre, site, kp.merged, nx);
             have = nx.made && !st && !__ballot(W.nr[0] != nx.nrs);
             redo = nx.made && !have;
