@@ -16,7 +16,9 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, REPO)
 import bench  # noqa: E402
 
-WORKLOADS = ("zgb_128x128_x4096", "ruo2_local_smart_mix_x2048", "pairwise_lat_int_256x256_x512")
+# workload -> the step kernel the library launches for it (kmcg_model_forms)
+WORKLOADS = {"zgb_128x128_x4096": "kmc_steps_kernel", "ruo2_local_smart_mix_x2048": "kmc_steps_kernel",
+             "pairwise_lat_int_256x256_x512": "kmc_steps_kernel_li"}
 
 
 def test_more_than_one_gpu_is_refused():
@@ -26,14 +28,15 @@ def test_more_than_one_gpu_is_refused():
     assert "one GPU" in r.stderr and r.stdout == ""
 
 
-@pytest.mark.parametrize("workload", WORKLOADS)
+@pytest.mark.parametrize("workload", sorted(WORKLOADS))
 def test_profile_figures_scale_with_the_run(workload):
-    traffic = bench.profile_figure(workload, "pmc", "traffic_bytes_per_kmc_step")
-    lds = bench.profile_figure(workload, "lds", "lds_bytes_per_kmc_step")
+    kernel = WORKLOADS[workload]
+    traffic = bench.profile_figure(workload, "pmc", "traffic_bytes_per_kmc_step", kernel)
+    lds = bench.profile_figure(workload, "lds", "lds_bytes_per_kmc_step", kernel)
     assert traffic is not None and traffic > 0
     assert lds is not None and lds > traffic        # the LDS moves far more bytes than HBM
     for spl, avg_s in ((4.096e7, 0.1), (2.048e7, 0.06)):
-        f = bench.roofline(workload, 200.0, spl, avg_s)
+        f = bench.roofline(workload, 200.0, spl, avg_s, kernel)
         kernel_rate = spl / avg_s                     # kMC steps/s inside the kernel
         assert f["achieved"] == pytest.approx(200.0 * kernel_rate / 1e9)
         assert f["frac"] == pytest.approx(f["achieved"] / 8000.0)
@@ -78,3 +81,15 @@ def test_oracle_cpu_baseline_states_its_cores():
     out = bench.oracle_cpu_baseline(wl, seconds=0.5)
     assert out["kind"] == "port" and out["value"] > 0
     assert out["cores_used"] == min(out["cores_available"], 16) == out["cores"]
+
+
+def test_profiles_name_the_kernel_the_library_launches():
+    """The kernel each workload's profiles are filed under is the form the
+    library reports for its model (kmcg_model_forms)."""
+    from kmos_amd import engine
+    from kmos_amd import workloads as W
+    from kmos_amd.compiler import compile_model
+    for key, wl in (("zgb_128x128_x4096", W.zgb(n=2)), ("ruo2_local_smart_mix_x2048", W.ruo2(n=4)),
+                    ("pairwise_lat_int_256x256_x512", W.pairwise(n=2))):
+        assert wl.name.rsplit("_x", 1)[0] == key.rsplit("_x", 1)[0]
+        assert engine.Model(compile_model(wl.project, wl.backend)).step_kernel == WORKLOADS[key]
