@@ -446,3 +446,45 @@ def test_many_seeds_match_oracle(engine, oracle, name, factory, backend, sizes, 
         o.run(2000)
         compare_state(b, r, o, where="%s r%d size %s" % (name, r, sz[r]))
     b.close()
+
+
+@pytest.mark.parametrize("backend", ["local_smart", "lat_int"])
+def test_species_mismatch_in_the_step_loop(engine, oracle, backend):
+    """A stale database met inside the step loop (empty sites turned to CO
+    by replace_species, so a later adsorption finds CO) stops the replica.
+
+    local_smart: at the same step and in the same state as the reference.
+    lat_int: at the same step, with a stop status.  Here the reference's
+    del_proc of an entry that is not registered is undefined (an ASSERT
+    that only debug builds compile); the oracle stops at that deletion, and
+    the engine, which queues a step's deletions and applies them after the
+    writes, stops after the step with the missing deletion skipped, so its
+    lists and status may differ from the oracle's there."""
+    pt = W.zgb_project()
+    cm = compile_model(pt, backend)
+    sid = pt.species_ids()
+    m = engine.Model(cm)
+    sizes = [(8, 8), (10, 6)]
+    b = engine.Batch(m, [s + (1,) for s in sizes])
+    seeds = np.array([5, 6], np.uint32)
+    b.init(seeds)
+    rates = np.tile([5.0, 1.0, 1.0, 1.0, 1.0, 0.5, 0.5], (2, 1))
+    b.set_rates(rates)
+    orcs = [oracle_replica(oracle, cm, s, int(seeds[r]), 0, rates[r]) for r, s in enumerate(sizes)]
+    b.run(40)
+    for o in orcs:
+        o.run(40)
+    for r, o in enumerate(orcs):
+        lat = o.lattice()
+        for s in np.nonzero(lat == sid["empty"])[0][:6]:
+            b.replace_species(r, int(s) + 1, sid["empty"], sid["CO"])
+            oracle.lib().orc_replace_species(o.h, int(s) + 1, sid["empty"], sid["CO"])
+    b.run(200)
+    for r, o in enumerate(orcs):
+        o.run(200)
+        assert o.status != 0
+        if backend == "local_smart":
+            compare_state(b, r, o, where="stale database, %s r%d" % (backend, r))
+        else:
+            _, _, ks, ctr, st = b.scalars(r)
+            assert st in (2, 3) and ks == o.kmc_step and ctr == o.rng_ctr, (st, ks, o.kmc_step)
