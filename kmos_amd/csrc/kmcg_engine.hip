@@ -2788,8 +2788,9 @@ Shape launch_shape(kmcg_batch *b, const int *ids, int n, KParams &kp, int max_wp
     s.wpg = std::max(1, std::min(max_wpg, std::min(MAX_WPG, n)));
     // automatic: local_smart runs with at most two replica-waves per SIMD
     // (four SIMDs per CU) and a prefix chain long enough to hide (more than
-    // 16 processes; RuO2's 36 gain, ZGB's 7 lose); lat_int steps mostly
-    // queue more than one replay block, where the overlap does not engage
+    // 16 processes; RuO2's 36 gain, ZGB's 7 lose); lat_int runs the
+    // one-pass form with the replay's list words read ahead, which measured
+    // faster than the overlap
     s.ov = b->ov_mode == 1 ||
            (b->ov_mode == -1 && b->m->backend == 0 && b->m->P > 16 && (long long)n <= 8LL * b->n_cu);
     while (s.wpg > 1 && prog + s.wpg * per_wave > (size_t)b->max_lds)
