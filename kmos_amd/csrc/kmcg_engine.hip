@@ -759,9 +759,9 @@ __device__ __forceinline__ void li_prefetch(const int *prog, int off_seg, int of
 // segments hold the same ops, so each lane looks up its op's variant on
 // both lattices from one read of its cluster: every read (the written
 // sites and the clusters) sees the lattice before the writes, and the
-// additions' lookups take the written sites' new species.  The queues get
-// the deletions in rounds 0..R-1 and the additions in rounds R..2R-1, the
-// order of the two passes.  Returns -1 (and changes nothing) when a write
+// additions' lookups take the written sites' new species.  Every process's
+// queue gets its deletions, then its additions, each in op order: the order
+// of the two passes.  Returns -1 (and changes nothing) when a write
 // would fail its species check: the caller then runs the segments one by
 // one, which reproduces the reference's behaviour on a mismatch.  Every
 // nli group of the model is tabulated.
@@ -830,27 +830,28 @@ __device__ __forceinline__ int post_latint_pair(Wave<PPL, BITS> &W, const SegPre
     const int rd = pd > 0 ? rnd : -1, ra = pa > 0 ? rnd : -1;
     int sd = 0, sa = 0;
     if (R <= 8) {
-        // one result register per round: a pass's atomics issue back to
-        // back and are waited for once
-        int sl[8];
+        // queue slots from ranks (the queues are empty before the step's
+        // only pass): a group's ops sit next to each other, at most 8, and
+        // name processes no other group names, so the lanes within 7 of
+        // this one hold every entry this lane's slots depend on; lanes of
+        // other groups and lanes without an entry never match
+        int lpd = pd, lpa = pa, rpd = pd, cda = (pa > 0 && pd == pa) ? 1 : 0;
 #pragma unroll
-        for (int it = 0; it < 8; ++it) {
-            sl[it] = 0;
-            if (rd == it)
-                sl[it] = atomicAdd(&W.cnt[pd - 1], 1);
+        for (int k = 1; k < 8; ++k) {
+            lpd = __builtin_amdgcn_mov_dpp(lpd, 0x138, 0xF, 0xF, true);   // lane - k (0 below lane 0)
+            lpa = __builtin_amdgcn_mov_dpp(lpa, 0x138, 0xF, 0xF, true);
+            rpd = __builtin_amdgcn_mov_dpp(rpd, 0x130, 0xF, 0xF, true);   // lane + k (0 past lane 63)
+            sd += (pd > 0 && lpd == pd) ? 1 : 0;
+            sa += (pa > 0 && lpa == pa) ? 1 : 0;
+            cda += (pa > 0 && lpd == pa) ? 1 : 0;
+            cda += (pa > 0 && rpd == pa) ? 1 : 0;
         }
-#pragma unroll
-        for (int it = 0; it < 8; ++it)
-            sd = (rd == it) ? sl[it] : sd;
-#pragma unroll
-        for (int it = 0; it < 8; ++it) {
-            sl[it] = 0;
-            if (ra == it)
-                sl[it] = atomicAdd(&W.cnt[pa - 1], 1);
-        }
-#pragma unroll
-        for (int it = 0; it < 8; ++it)
-            sa = (ra == it) ? sl[it] : sa;
+        // an addition follows every deletion of its process
+        sa += cda;
+        if (pd > 0)
+            atomicAdd(&W.cnt[pd - 1], 1);
+        if (pa > 0)
+            atomicAdd(&W.cnt[pa - 1], 1);
     } else {
         for (int it = 0; it < R; ++it)
             if (rd == it)
@@ -2324,6 +2325,13 @@ int build_prog(kmcg_model *m)
                     del.ops.push_back({OP_DEL_NLI, 0, W(o), W(o + 1), W(o + 2), W(o + 3), 1, {}});
                     add.ops.push_back({OP_ADD_NLI, 0, W(o), W(o + 1), W(o + 2), W(o + 3), 1, {}});
                 }
+                // the ops of one nli group next to each other, in their
+                // order (ops of different groups name disjoint processes, so
+                // every process's queue keeps the reference order): the
+                // one-pass posting ranks a group's entries within its block
+                auto by_group = [](const OpH &a, const OpH &b) { return a.arg < b.arg; };
+                std::stable_sort(del.ops.begin(), del.ops.end(), by_group);
+                std::stable_sort(add.ops.begin(), add.ops.end(), by_group);
                 const int nupd = W(base + 2), uoff = W(base + 3);
                 for (int i = 0; i < nupd; ++i)
                     for (int q = 0; q < 6; ++q)
