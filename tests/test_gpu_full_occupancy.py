@@ -1,8 +1,10 @@
-"""Parity at full occupancy: the ZGB workload (4096 replicas of 128^2) and
-the RuO2 mix (2048 replicas of 20^2 / 64^2 / 128^2, random starting
-coverage) run as whole batches in the launch shape the benchmark uses, and
-32 replicas of each, drawn at random over the batch, equal the oracle bit
-for bit after 2000 steps (lattice, both avail tables, counters, clock)."""
+"""Parity at full occupancy: the ZGB workload (4096 replicas of 128^2), the
+RuO2 mix (2048 replicas of 20^2 / 64^2 / 128^2, random starting coverage)
+and the pairwise lat_int workload (512 replicas of 256^2, the one-pass
+lat_int kernel) run as whole batches in the launch shape the benchmark
+uses, and 32 replicas of each, drawn at random over the batch, equal the
+oracle bit for bit after 2000 steps (lattice, both avail tables, counters,
+clock)."""
 import numpy as np
 import pytest
 
@@ -13,7 +15,7 @@ from paritylib import compare_state, oracle_replica
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", ["zgb", "ruo2"])
+@pytest.mark.parametrize("name", ["zgb", "ruo2", "pairwise"])
 def test_full_batch_sampled_replicas_match_oracle(engine, oracle, name):
     wl = W.WORKLOADS[name]()
     print("case", wl.name, flush=True)
