@@ -142,6 +142,29 @@ def test_otf_full_lattices_sampled(otf_engine, otf_oracle):
         _compare(b, r, o, "64x64 r%d" % r)
 
 
+def test_otf_full_occupancy_sampled(otf_engine, otf_oracle):
+    """The benchmark's otf workload at full size (2048 replicas of 64 x 64
+    intZGB, every replica its own yCO and interaction energies) as one
+    batch; 16 replicas drawn over the batch equal the oracle after 1000
+    steps."""
+    wl = W.intzgb_otf()
+    print("case", wl.name, flush=True)
+    cm = compile_model(wl.project, "otf")
+    m = otf_engine.OtfModel(cm)
+    b = otf_engine.OtfBatch(m, [list(sz) for sz in wl.sizes])
+    b.set_params(wl.rates, wl.userpar)
+    b.init(wl.seeds, wl.streams)
+    b.run(1000)
+    blob = cm.to_blob()
+    for r in np.sort(np.random.default_rng(8).choice(wl.n, 16, replace=False)):
+        o = otf_oracle.OtfOracle(blob, tuple(wl.sizes[r][:2]))
+        o.init(int(wl.seeds[r]), int(wl.streams[r]))
+        o.set_params(wl.rates[r], wl.userpar[r])
+        assert o.run(1000) == 1000
+        _compare(b, int(r), o, "%s r%d" % (wl.name, r))
+    b.close()
+
+
 def test_otf_rejects_bad_input(otf_engine):
     from kmos_amd import _lib
     cm, base = _intzgb()
