@@ -371,6 +371,51 @@ def graph_leg(engine, model, wl, steps_per_run=50, runs=300):
     return out
 
 
+def tracking_leg(engine, K, steps=20000, warm=20000):
+    """Displacement tracking (the MSD/ACF side channel: the tracking form of
+    the step kernel) against the same batch without it.  The pairwise
+    model compiled for local_smart (49 processes, hops in four directions),
+    512 replicas of 256^2; after `warm` steps the A particles are numbered
+    (msd_init) and K launches of `steps` steps are timed, through
+    do_kmc_steps_displacement and do_kmc_steps respectively, the plain form
+    (no overlapped select) in both runs."""
+    from kmos_amd import acf
+    wl = W.pairwise()
+    cm = compile_model(wl.project, "local_smart")
+    model = engine.Model(cm)
+    hops, disp = acf.displacement_table(cm)
+    trace = cm.project.species_ids()["A"]
+    out = {"workload": "pairwise_local_smart_256x256_x%d" % wl.n, "replicas": wl.n, "tracked_species": "A",
+           "kmc_steps_per_launch": wl.n * steps, "launches": K}
+    for mode in ("plain", "tracking"):
+        b = make_batch(engine, model, wl)
+        b.set_overlap(0)
+        b.run(warm)
+        if mode == "tracking":
+            b.msd_init(trace, hops, disp)
+        # do_kmc_steps or do_kmc_steps_displacement (synchronous)
+        step = (lambda k: b.run_displacement(k)) if mode == "tracking" else (lambda k: b.run(k))
+        step(2000)
+        ms = []
+        t0 = time.perf_counter()
+        for _ in range(K):
+            step(steps)
+            ms.append(b.last_run_ms())
+        wall = time.perf_counter() - t0
+        out[mode + "_gpu_steps_per_s"] = wl.n * steps * K / wall
+        out[mode + "_avg_launch_ms"] = float(np.mean(ms))
+        if mode == "tracking":
+            st = b.msd_state(0)
+            ni = int(st["nr_of_ions"])
+            d = st["displacement"][1:ni + 1]
+            out["tracked_particles_replica0"] = ni
+            # calc_msd: mean of |displacement|^2 over the numbered particles
+            out["msd_replica0"] = float(((d[:, 0] ** 2 + d[:, 1] ** 2) + d[:, 2] ** 2).mean()) if ni else 0.0
+        b.close()
+    out["tracking_over_plain"] = out["tracking_gpu_steps_per_s"] / out["plain_gpu_steps_per_s"]
+    return out
+
+
 def serving_leg(rate, n, seed=1, tune=True):
     """The serving stream; with `tune`, the sampling intervals per launch
     are first chosen by the autotuner on a shorter stream of the same
@@ -628,7 +673,7 @@ def main():
     ap.add_argument("--kmc-steps", type=int, default=None, help="kMC steps per replica")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=20000)
-    ap.add_argument("--legs", default="zgb,pairwise,otf,quickstart,serving",
+    ap.add_argument("--legs", default="zgb,pairwise,otf,tracking,quickstart,serving",
                     help="comma-separated legs beside the contract line (empty: none)")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="request arrivals per second")
     ap.add_argument("--serve-requests", type=int, default=200)
@@ -728,6 +773,9 @@ def main():
     if "otf" in legs:
         out["otf"] = otf_leg(K, 1, not a.no_cpu)
         _progress("otf leg done")
+    if "tracking" in legs:
+        out["tracking"] = tracking_leg(engine, K)
+        _progress("tracking leg done")
     if "quickstart" in legs and a.workload != "quickstart":
         out["quickstart"] = quickstart_leg(engine, not a.no_cpu)
         _progress("quickstart leg done")
